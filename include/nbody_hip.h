@@ -1,0 +1,187 @@
+/*
+ * nbody_hip.h -- C ABI of libnbody_hip.so, the MI355X (gfx950) FMM force
+ * pipeline behind nbody::HipSimulation.
+ *
+ * The reference has no C ABI: its only boundary is the C++ interface
+ * nbody::Simulation (include/nbody/simulation.h:6-37) implemented by
+ * NaiveSimulation (include/nbody/naive_simulation.h:12-41) and
+ * OpenClSimulation (include/nbody/open_cl_simulation.h:19-203).  Each entry
+ * point below names the reference interface or function it replaces.
+ *
+ * Conventions
+ *   - plain pointers and sizes only; no exceptions cross this boundary;
+ *   - every function returns 0 (NBH_OK) or a negative NBH_E* status, and the
+ *     handle's message is available from nbh_last_error();
+ *   - positions/velocities are FP64 AoS xyz triples; masses and charges FP64;
+ *   - particle arrays passed in and out are in the caller's input order;
+ *   - calls on one handle are not thread-safe; distinct handles are
+ *     independent.  Each handle owns one HIP stream on its device.
+ */
+#ifndef NBODY_HIP_H_
+#define NBODY_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBH_ABI_VERSION 1
+
+/* status codes */
+#define NBH_OK 0
+#define NBH_E_INVALID (-1)     /* bad argument or configuration */
+#define NBH_E_HIP (-2)         /* HIP runtime error (no device, launch failure ...) */
+#define NBH_E_NOMEM (-3)       /* device allocation failed */
+#define NBH_E_NONFINITE (-4)   /* NaN/Inf in the particle state */
+#define NBH_E_INTERNAL (-5)    /* an internal consistency check failed */
+
+/* solver */
+#define NBH_SOLVER_FMM 0       /* replaces OpenClSimulation's pipeline */
+#define NBH_SOLVER_DIRECT 1    /* O(N^2) on the GPU; replaces NaiveSimulation */
+
+/* far-field (M2L) kernel */
+#define NBH_M2L_PLAIN 0        /* per-pair FP32 VALU, derivative tensor per pair */
+#define NBH_M2L_GROUPED 1      /* FP32, grouped by translation offset */
+#define NBH_M2L_BF16X3 2       /* grouped, MFMA bf16x3 split products, FP32 accumulate */
+
+/* integrator */
+#define NBH_INT_KDK 0          /* kick-drift-kick leapfrog, FP64 state */
+#define NBH_INT_EULER 1        /* explicit Euler as open_cl_simulation.cpp:600-607 */
+#define NBH_INT_KICK_DRIFT 2   /* kick then drift as naive_simulation.cpp:28-41 */
+
+/* force law (direct solver only) */
+#define NBH_LAW_SOFTENED 0     /* field.cl:17-32 (Plummer softened, exact separation) */
+#define NBH_LAW_COMPAT_NAIVE 1 /* naive_simulation.cpp:10-26: separation (dx,dx,dx), no eps */
+
+typedef struct nbh_config {
+    uint32_t struct_size;   /* = sizeof(nbh_config), checked by nbh_create */
+    uint32_t solver;        /* NBH_SOLVER_* */
+    double G;               /* gravitational constant; reference FORCE_CONSTANT = -G (field.cl:7-9) */
+    double eps;             /* Plummer softening (reference PARTICLE_RADIUS, field.cl:3-5) */
+    double theta;           /* opening ratio (reference NODE_APPROX_RATIO, interaction.cl:3-5);
+                               must equal num/den with den <= 64 */
+    double dt;              /* time step (OpenClSimulation timeStep) */
+    uint32_t leaf_capacity; /* max particles per leaf cell (reference hard-codes 8,
+                               open_cl_simulation.cpp:41-47) */
+    uint32_t order;         /* Cartesian Taylor order p, 1..6 */
+    uint32_t max_depth;     /* deepest octree level, <= bits per axis (21 for one system) */
+    uint32_t m2l;           /* NBH_M2L_* */
+    uint32_t integrator;    /* NBH_INT_* */
+    uint32_t force_law;     /* NBH_LAW_* (direct solver) */
+    int32_t device;         /* HIP device ordinal */
+    uint32_t fixed_root;    /* 0: root cube = bounding cube each step; 1: [root_lo, root_lo+root_size) */
+    double root_lo[3];
+    double root_size;
+} nbh_config;
+
+typedef struct nbh_handle nbh_handle;
+
+/* Per-step statistics of the last force evaluation (FMM). */
+typedef struct nbh_stats {
+    uint64_t num_nodes, num_leaves, depth;
+    uint64_t m2l_pairs;      /* directed M2L list entries */
+    uint64_t p2p_pairs;      /* directed P2P leaf-pair list entries */
+    uint64_t p2p_interactions; /* directed particle-particle interactions (self excluded) */
+    uint64_t split_pairs;    /* pairs opened by the walk */
+    double ms_tree, ms_moments, ms_walk, ms_m2l, ms_l2l, ms_p2p, ms_total; /* if timing enabled */
+} nbh_stats;
+
+int nbh_abi_version(void);
+size_t nbh_config_size(void);
+
+/* Fills the reference's defaults: G=1 (FORCE_CONSTANT=-1), eps=0.01,
+ * theta=0.5, leaf capacity 8, p=4, KDK, FMM, plain M2L. */
+int nbh_config_default(nbh_config* cfg);
+
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int nbh_device_count(void);
+
+/* Replaces the OpenClSimulation / NaiveSimulation constructors
+ * (open_cl_simulation.cpp:15-51, naive_simulation.h:25-33): copies the
+ * particles to the device.  pos, vel: 3n doubles; mass, charge: n doubles.
+ * n >= 1. */
+int nbh_create(const nbh_config* cfg, size_t n, const double* pos, const double* vel,
+               const double* mass, const double* charge, nbh_handle** out);
+
+/* Ensemble of nsys independent systems in one handle (no counterpart in the
+ * reference): sys_off[nsys+1] splits the particle arrays into systems that
+ * never interact; every FMM phase processes all systems in one launch. */
+int nbh_create_ensemble(const nbh_config* cfg, size_t nsys, const uint32_t* sys_off,
+                        const double* pos, const double* vel, const double* mass,
+                        const double* charge, nbh_handle** out);
+
+void nbh_destroy(nbh_handle* h);
+
+/* Message of the last failing call on h (or of the last failing nbh_create
+ * when h is NULL).  Never NULL. */
+const char* nbh_last_error(const nbh_handle* h);
+
+/* Replaces Simulation::step() (simulation.h:34; open_cl_simulation.cpp:70-106,
+ * naive_simulation.cpp:7-46): advances nsteps steps of dt and returns the new
+ * time in *t_out (may be NULL). */
+int nbh_step(nbh_handle* h, int nsteps, double* t_out);
+
+/* Replaces Simulation::particles() (simulation.h:35; open_cl_simulation.cpp:
+ * 53-68, naive_simulation.h:37-39), in input order.  Any pointer may be NULL. */
+int nbh_get_state(nbh_handle* h, double* pos, double* vel, double* mass,
+                  double* charge, double* t);
+int nbh_set_state(nbh_handle* h, const double* pos, const double* vel, double t);
+
+/* One force evaluation at the current state (no integration), input order:
+ * acc[3n] = (q_i/m_i) * field (force.cl:4-19 divided by mass), pot[n] =
+ * potential per unit charge.  Either may be NULL. */
+int nbh_compute_forces(nbh_handle* h, float* acc, float* pot);
+
+/* Reference check: FP64 direct sum on the GPU for the given target particle
+ * indices (input order) over all sources of their system; acc[3*nt], pot[nt]
+ * include the q_i/m_i factor like nbh_compute_forces. */
+int nbh_direct_fp64(nbh_handle* h, size_t nt, const int64_t* targets, double* acc, double* pot);
+
+/* Energy and momentum of the current state (FP64, on the GPU):
+ * out[0] kinetic, out[1] potential (uses the last force evaluation),
+ * out[2..4] linear momentum, out[5..7] angular momentum. */
+int nbh_diagnostics(nbh_handle* h, double* out);
+
+int nbh_get_stats(nbh_handle* h, nbh_stats* s);
+/* Enables per-phase hipEvent timing of force evaluations (adds syncs). */
+int nbh_set_timing(nbh_handle* h, int enable);
+
+/* ---- adjoint (no counterpart in the reference) --------------------------- */
+/* Vector-Jacobian product of the accelerations a(x, m) of the current state
+ * (gravity, charge == mass) with cotangent w[3n]: gx[3n] = (da/dx)^T w,
+ * gm[n] = (da/dm)^T w, input order.  Uses the tree and M2L/P2P lists of a
+ * force evaluation at the current positions (FP32 kernels). */
+int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm);
+
+/* ---- tree / list export for parity tests (read-only views) -------------- */
+/* sizes[0]=nodes, [1]=leaves, [2]=depth, [3]=m2l entries, [4]=p2p entries,
+ * [5]=number of systems, of the last force evaluation. */
+int nbh_debug_sizes(nbh_handle* h, int64_t* sizes);
+/* keys[n] (sorted), perm: id[n] (input index of each sorted slot),
+ * box[4*nsys] (lo xyz, size) */
+int nbh_debug_keys(nbh_handle* h, uint64_t* keys, uint32_t* ids, double* box);
+/* per node: level, sys, ix, iy, iz, begin, count, first_child, nchild, parent */
+int nbh_debug_tree(nbh_handle* h, int32_t* level, int32_t* sys, int32_t* ix, int32_t* iy,
+                   int32_t* iz, int32_t* begin, int32_t* count, int32_t* first_child,
+                   int32_t* nchild, int32_t* parent);
+/* CSR lists: off[nodes+1] (int64), src (int32) */
+int nbh_debug_lists(nbh_handle* h, int64_t* m2l_off, int32_t* m2l_src, int64_t* p2p_off,
+                    int32_t* p2p_src);
+/* multipole and local coefficients per node: [nodes * ncoef] float */
+int nbh_debug_expansions(nbh_handle* h, float* M, float* L);
+
+/* ---- seeded initial conditions (host-side, no GPU needed) ---------------- */
+/* Uniform cube [0,1)^3 positions, zero velocities, masses = charges = 1/n. */
+int nbh_ic_uniform(uint64_t seed, size_t n, double* pos, double* vel, double* mass);
+/* Plummer sphere (scale a, total mass M, G): radii from the cumulative mass
+ * profile truncated at rmax, velocities by von Neumann rejection from the
+ * distribution function (Aarseth, Henon & Wielen 1974).  Masses M/n. */
+int nbh_ic_plummer(uint64_t seed, size_t n, double a, double M, double G, double rmax,
+                   double* pos, double* vel, double* mass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBODY_HIP_H_ */

@@ -1,0 +1,69 @@
+// common.h -- shared device types and helpers of libnbody_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nbh {
+
+constexpr int WAVE = 64;
+constexpr int MAX_LEVELS = 22;   // levels 0..21
+
+// Node records (canonical BFS order: level, system, Morton prefix).
+//   geo  = {ix, iy, iz, level | nchild << 8}   integer cell coords at `level`
+//   link = {begin, count, first_child, parent} particle range [begin, begin+count)
+struct NodeView {
+    int4* geo;
+    int4* link;
+    int* sys;
+};
+
+__device__ __forceinline__ int geo_level(int4 g) { return g.w & 0xff; }
+__device__ __forceinline__ int geo_nchild(int4 g) { return g.w >> 8; }
+
+// Device-side counters of one force evaluation.  Written by kernels, read by
+// the host once after the tree build and once after the walk's first phase.
+struct Counters {
+    uint32_t num_nodes;
+    uint32_t num_leaves;
+    uint32_t depth;
+    uint32_t overflow;                 // bit flags, see OVF_*
+    uint32_t level_count[MAX_LEVELS + 1];
+    uint32_t level_off[MAX_LEVELS + 2];
+    uint32_t carried[MAX_LEVELS + 2];  // walk: carried leaf rows per depth
+    uint32_t sl_used;                  // walk: split-list entries used
+    uint32_t rows_used;
+    uint32_t pad;
+    unsigned long long m2l_total, p2p_total, p2p_inter, split_total;
+    uint32_t nonfinite;
+};
+
+enum : uint32_t { OVF_NODES = 1, OVF_SL = 2, OVF_ROWS = 4, OVF_M2L = 8, OVF_P2P = 16 };
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int l = __lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, WAVE);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__host__ __device__ __forceinline__ unsigned ceil_div(unsigned a, unsigned b) { return (a + b - 1) / b; }
+
+}  // namespace nbh

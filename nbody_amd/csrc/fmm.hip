@@ -1,0 +1,513 @@
+// fmm.hip -- moments, far field, near field and integration kernels.
+//
+// Replaces moment.cl (P2M/M2M, :6-160), field.cl (P2P and cell fields,
+// :17-211), force.cl (per-particle reductions, :21-81) and the host
+// integration loop (open_cl_simulation.cpp:572-616).  Fields are accumulated
+// in registers and reduced inside the producing kernel; there are no
+// per-interaction slot buffers.
+//
+// Precision: particle positions and velocities are FP64.  Each particle is
+// stored for the force kernels as an FP32 offset from the centre of its leaf
+// cell; centre differences between cells are exact integers (units of half
+// the finest cell) scaled once to FP32, so FP32 separations keep their
+// relative precision however large the root box is.
+#include "common.h"
+#include "expansion.h"
+#include "launch.h"
+
+namespace nbh {
+
+// centre difference c_B - c_A of two cells of one system, FP32
+__device__ __forceinline__ float3 centre_diff(int4 ga, int4 gb, int bits, float unit) {
+    const int sa = bits - (ga.w & 0xff), sb = bits - (gb.w & 0xff);
+    const int dx = ((2 * gb.x + 1) << sb) - ((2 * ga.x + 1) << sa);
+    const int dy = ((2 * gb.y + 1) << sb) - ((2 * ga.y + 1) << sa);
+    const int dz = ((2 * gb.z + 1) << sb) - ((2 * ga.z + 1) << sa);
+    return make_float3((float)dx * unit, (float)dy * unit, (float)dz * unit);
+}
+
+// FP64 centre of a cell
+__device__ __forceinline__ void centre64(int4 g, double4 box, int bits, double c[3]) {
+    const int l = g.w & 0xff;
+    const double h = ldexp(box.w, -l);
+    c[0] = box.x + ((double)g.x + 0.5) * h;
+    c[1] = box.y + ((double)g.y + 0.5) * h;
+    c[2] = box.z + ((double)g.z + 0.5) * h;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void gather_state_kernel(State a, State b, const uint32_t* __restrict__ perm, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = perm[i];
+    b.x[i] = a.x[j]; b.y[i] = a.y[j]; b.z[i] = a.z[j];
+    b.vx[i] = a.vx[j]; b.vy[i] = a.vy[j]; b.vz[i] = a.vz[j];
+    b.mass[i] = a.mass[j]; b.charge[i] = a.charge[j];
+    b.ax[i] = a.ax[j]; b.ay[i] = a.ay[j]; b.az[i] = a.az[j];
+    b.pot[i] = a.pot[j];
+    b.id[i] = a.id[j];
+}
+
+void launch_gather_state(const State& src, const State& dst, const uint32_t* perm, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(gather_state_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, src, dst, perm, n);
+}
+
+// ---------------------------------------------------------------------------
+// P2M (moment.cl:6-67 at order P): one wavefront per leaf cell.  Also writes
+// the particles' FP32 offsets from the leaf centre (used by P2P and L2P).
+template <int P>
+__global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t num_leaves) {
+    constexpr int NC = ncoef(P);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (wid >= num_leaves) return;
+    const int A = (int)F.leaf_ids[wid];
+    const int4 g = F.nodes.geo[A];
+    const int4 lk = F.nodes.link[A];
+    const int s = F.nodes.sys[A];
+    double c[3];
+    centre64(g, F.box[s], F.bits, c);
+    float M[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) M[k] = 0.f;
+    for (int j = lane_id(); j < lk.y; j += WAVE) {
+        const int i = lk.x + j;
+        const float bx = (float)(S.x[i] - c[0]), by = (float)(S.y[i] - c[1]), bz = (float)(S.z[i] - c[2]);
+        const float q = (float)S.charge[i];
+        F.pf[i] = make_float4(bx, by, bz, q);
+        float mono[NC];
+        monomials<P>(bx, by, bz, mono);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) M[k] += q * mono[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) M[k] = wave_sum(M[k]);
+    if (lane_id() == 0) {
+        float* out = F.M + (size_t)A * F.ncp;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) out[k] = M[k];
+    }
+}
+
+// M2M (replaces moment.cl:72-160, which sums untranslated moments): one
+// thread per cell of the level; children's moments shifted to the parent
+// centre.  Levels run bottom-up.
+template <int P>
+__global__ void m2m_kernel(FmmParams F, uint32_t count, uint32_t off) {
+    constexpr int NC = ncoef(P);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int A = (int)(off + t);
+    const int4 g = F.nodes.geo[A];
+    const int nch = g.w >> 8;
+    if (nch == 0) return;
+    const int4 lk = F.nodes.link[A];
+    const float unit = F.unit[F.nodes.sys[A]];
+    float Mp[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) Mp[k] = 0.f;
+    for (int c = 0; c < nch; ++c) {
+        const int ch = lk.z + c;
+        const int4 gc = F.nodes.geo[ch];
+        const float3 d = centre_diff(g, gc, F.bits, unit);  // c_child - c_parent
+        float Mc[NC];
+        const float* src = F.M + (size_t)ch * F.ncp;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Mc[k] = src[k];
+        m2m_apply<P>(d.x, d.y, d.z, Mc, Mp);
+    }
+    float* out = F.M + (size_t)A * F.ncp;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) out[k] = Mp[k];
+}
+
+// M2L, plain: one thread per target cell, the derivative tensor rebuilt per
+// pair (replaces field.cl:153-211's order-0 cell field).
+template <int P>
+__global__ __launch_bounds__(256) void m2l_plain_kernel(FmmParams F, uint32_t num_nodes) {
+    constexpr int NC = ncoef(P);
+    const uint32_t A = blockIdx.x * blockDim.x + threadIdx.x;
+    if (A >= num_nodes) return;
+    const int4 ga = F.nodes.geo[A];
+    const float unit = F.unit[F.nodes.sys[A]];
+    float L[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) L[k] = 0.f;
+    const unsigned long long e0 = F.m2l_off[A], e1 = F.m2l_off[A + 1];
+    for (unsigned long long e = e0; e < e1; ++e) {
+        const int B = F.m2l_src[e];
+        const int4 gb = F.nodes.geo[B];
+        const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
+        float D[NC];
+        derivs<P>(d.x, d.y, d.z, F.eps2, D);
+        float M[NC];
+        const float4* src = reinterpret_cast<const float4*>(F.M + (size_t)B * F.ncp);
+#pragma unroll
+        for (int k = 0; k < (NC + 3) / 4; ++k) {
+            const float4 v = src[k];
+            if (4 * k + 0 < NC) M[4 * k + 0] = v.x;
+            if (4 * k + 1 < NC) M[4 * k + 1] = v.y;
+            if (4 * k + 2 < NC) M[4 * k + 2] = v.z;
+            if (4 * k + 3 < NC) M[4 * k + 3] = v.w;
+        }
+        m2l_apply<P>(D, M, L);
+    }
+    float* out = F.L + (size_t)A * F.ncp;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) out[k] = L[k];
+}
+
+// L2L: one thread per cell of the level (top-down), adds the parent's local
+// expansion shifted to this cell's centre.
+template <int P>
+__global__ void l2l_kernel(FmmParams F, uint32_t count, uint32_t off) {
+    constexpr int NC = ncoef(P);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int A = (int)(off + t);
+    const int4 g = F.nodes.geo[A];
+    const int par = F.nodes.link[A].w;
+    if (par < 0) return;
+    const int4 gp = F.nodes.geo[par];
+    const float3 d = centre_diff(gp, g, F.bits, F.unit[F.nodes.sys[A]]);  // c_child - c_parent
+    float Lp[NC], Lc[NC];
+    const float* sp = F.L + (size_t)par * F.ncp;
+    float* sc = F.L + (size_t)A * F.ncp;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) { Lp[k] = sp[k]; Lc[k] = sc[k]; }
+    l2l_apply<P>(d.x, d.y, d.z, Lp, Lc);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) sc[k] = Lc[k];
+}
+
+// ---------------------------------------------------------------------------
+// P2P + L2P + force + (optional) closing half kick: one wavefront per target
+// leaf.  Replaces field.cl:49-148 (pair fields into slots), force.cl:21-81
+// (slot reductions) and the host integration loop.  Lanes are split as
+// (target particle) x (source stride): with nt targets in a chunk, lane l
+// handles target l % nt and every (64/nt)-th particle of each source leaf, so
+// all 64 lanes work for any leaf occupancy; partial sums meet in LDS.
+template <int P>
+__global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t num_leaves, double kick_dt) {
+    constexpr int NC = ncoef(P);
+    const int lane = lane_id();
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (wid >= num_leaves) return;  // whole wave exits together
+    const int A = (int)F.leaf_ids[wid];
+    const int4 ga = F.nodes.geo[A];
+    const int4 la = F.nodes.link[A];
+    const float unit = F.unit[F.nodes.sys[A]];
+    const float eps2 = F.eps2;
+    const unsigned long long e0 = F.p2p_off[A], e1 = F.p2p_off[A + 1];
+    for (int t0 = 0; t0 < la.y; t0 += WAVE) {
+        const int nt = min(WAVE, la.y - t0);
+        const int splits = WAVE / nt;
+        const int ti = lane % nt;
+        const int sp = lane / nt;
+        const bool active = sp < splits;
+        const int i = la.x + t0 + ti;
+        const float4 me = F.pf[i];
+        float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+        for (unsigned long long e = e0; e < e1; ++e) {
+            const int B = F.p2p_src[e];
+            const int4 gb = F.nodes.geo[B];
+            const int4 lb = F.nodes.link[B];
+            const float3 d = centre_diff(ga, gb, F.bits, unit);
+            const float dx = d.x - me.x, dy = d.y - me.y, dz = d.z - me.z;
+            const float4* src = F.pf + lb.x;
+            const int jn = active ? lb.y : 0;
+            if (B != A) {
+                for (int j = sp; j < jn; j += splits) {
+                    const float4 s = src[j];
+                    const float rx = dx + s.x, ry = dy + s.y, rz = dz + s.z;
+                    const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+                    const float inv = rsqrtf(r2);
+                    const float qi = s.w * inv;
+                    const float qi3 = qi * inv * inv;
+                    ax = fmaf(rx, qi3, ax);
+                    ay = fmaf(ry, qi3, ay);
+                    az = fmaf(rz, qi3, az);
+                    ph += qi;
+                }
+            } else {
+                const int self = t0 + ti;
+                for (int j = sp; j < jn; j += splits) {
+                    const float4 s = src[j];
+                    const float rx = dx + s.x, ry = dy + s.y, rz = dz + s.z;
+                    const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+                    const float inv = rsqrtf(r2);
+                    const float q = j == self ? 0.f : s.w;
+                    const float qi = q * inv;
+                    const float qi3 = qi * inv * inv;
+                    ax = fmaf(rx, qi3, ax);
+                    ay = fmaf(ry, qi3, ay);
+                    az = fmaf(rz, qi3, az);
+                    ph += qi;
+                }
+            }
+        }
+        // combine the source strides of each target (lane ti + q*nt holds
+        // stride q of target ti; every lane takes part in the shuffles)
+        {
+            const float ax0 = ax, ay0 = ay, az0 = az, ph0 = ph;
+            for (int q = 1; q < splits; ++q) {
+                const int srcl = ti + q * nt;
+                const float ox = __shfl(ax0, srcl, WAVE), oy = __shfl(ay0, srcl, WAVE);
+                const float oz = __shfl(az0, srcl, WAVE), ow = __shfl(ph0, srcl, WAVE);
+                ax += ox; ay += oy; az += oz; ph += ow;
+            }
+        }
+        if (sp == 0) {
+            float lphi, gx, gy, gz;
+            const float* Lsrc = F.L + (size_t)A * F.ncp;
+            float Lr[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) Lr[k] = Lsrc[k];
+            l2p_eval<P>(me.x, me.y, me.z, Lr, lphi, gx, gy, gz);
+            const double qm = S.charge[i] / S.mass[i];
+            const float fac = (float)qm * F.G;
+            const float axo = fac * (ax + gx), ayo = fac * (ay + gy), azo = fac * (az + gz);
+            S.ax[i] = axo;
+            S.ay[i] = ayo;
+            S.az[i] = azo;
+            S.pot[i] = -F.G * (ph + lphi);
+            if (kick_dt != 0.0) {
+                S.vx[i] += (double)axo * kick_dt;
+                S.vy[i] += (double)ayo * kick_dt;
+                S.vz[i] += (double)azo * kick_dt;
+            }
+        }
+    }
+}
+
+template <int P>
+static void launch_p2m_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
+    hipLaunchKernelGGL((p2m_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves);
+}
+template <int P>
+static void launch_m2m_t(const FmmParams& F, uint32_t count, uint32_t off, hipStream_t st) {
+    hipLaunchKernelGGL((m2m_kernel<P>), dim3(ceil_div(count, 64)), dim3(64), 0, st, F, count, off);
+}
+template <int P>
+static void launch_m2l_t(const FmmParams& F, uint32_t num_nodes, hipStream_t st) {
+    hipLaunchKernelGGL((m2l_plain_kernel<P>), dim3(ceil_div(num_nodes, 256)), dim3(256), 0, st, F, num_nodes);
+}
+template <int P>
+static void launch_l2l_t(const FmmParams& F, uint32_t count, uint32_t off, hipStream_t st) {
+    hipLaunchKernelGGL((l2l_kernel<P>), dim3(ceil_div(count, 64)), dim3(64), 0, st, F, count, off);
+}
+template <int P>
+static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st) {
+    hipLaunchKernelGGL((p2p_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves, kick_dt);
+}
+
+#define NBH_DISPATCH_P(p, call)          \
+    switch (p) {                         \
+        case 1: { constexpr int PP = 1; call; } break; \
+        case 2: { constexpr int PP = 2; call; } break; \
+        case 3: { constexpr int PP = 3; call; } break; \
+        case 4: { constexpr int PP = 4; call; } break; \
+        case 5: { constexpr int PP = 5; call; } break; \
+        case 6: { constexpr int PP = 6; call; } break; \
+        default: break;                  \
+    }
+
+void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
+    if (!num_leaves) return;
+    NBH_DISPATCH_P(p, launch_p2m_t<PP>(F, S, num_leaves, st));
+}
+void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st) {
+    (void)level;
+    if (!count) return;
+    NBH_DISPATCH_P(p, launch_m2m_t<PP>(F, count, off, st));
+}
+void launch_m2l_plain(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st) {
+    if (!num_nodes) return;
+    NBH_DISPATCH_P(p, launch_m2l_t<PP>(F, num_nodes, st));
+}
+void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st) {
+    (void)level;
+    if (!count) return;
+    NBH_DISPATCH_P(p, launch_l2l_t<PP>(F, count, off, st));
+}
+void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st) {
+    if (!num_leaves) return;
+    NBH_DISPATCH_P(p, launch_p2p_t<PP>(F, S, num_leaves, kick_dt, st));
+}
+
+// ---------------------------------------------------------------------------
+__global__ void integrate_kernel(State S, uint32_t n, int mode, double dtk, double dtd) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v[3] = {S.vx[i], S.vy[i], S.vz[i]};
+    const double a[3] = {S.ax[i], S.ay[i], S.az[i]};
+    double x[3] = {S.x[i], S.y[i], S.z[i]};
+    if (mode == 0) {          // KDK opening half kick + drift
+        for (int k = 0; k < 3; ++k) { v[k] += a[k] * dtk; x[k] += v[k] * dtd; }
+    } else if (mode == 1) {   // explicit Euler, open_cl_simulation.cpp:602-607
+        for (int k = 0; k < 3; ++k) { x[k] += v[k] * dtd; v[k] += a[k] * dtk; }
+    } else {                  // kick-drift, naive_simulation.cpp:28-41
+        for (int k = 0; k < 3; ++k) { v[k] += a[k] * dtk; x[k] += v[k] * dtd; }
+    }
+    S.vx[i] = v[0]; S.vy[i] = v[1]; S.vz[i] = v[2];
+    S.x[i] = x[0]; S.y[i] = x[1]; S.z[i] = x[2];
+}
+
+void launch_integrate(const State& S, uint32_t n, int mode, double dt_k, double dt_d, hipStream_t st) {
+    hipLaunchKernelGGL(integrate_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, S, n, mode, dt_k, dt_d);
+}
+
+// ---------------------------------------------------------------------------
+// Direct sum on the GPU (replaces NaiveSimulation::step's O(N^2) loop,
+// naive_simulation.cpp:8-35).  Source tiles staged in LDS; FP32 positions
+// relative to the system's box centre.  compat_law reproduces the reference
+// naive force law: separation (dx, dx, dx), no softening (:10-17).
+constexpr int DS_THREADS = 256;
+
+__global__ __launch_bounds__(DS_THREADS) void direct_f32_kernel(State S, uint32_t n, const uint32_t* sys_off,
+                                                                int nsys, const double4* box, float G, float eps2,
+                                                                int compat, double kick_dt) {
+    __shared__ float4 tile[DS_THREADS];
+    const int s = blockIdx.y;
+    const uint32_t b = sys_off[s], e = sys_off[s + 1];
+    const uint32_t i = b + blockIdx.x * DS_THREADS + threadIdx.x;
+    if (b + blockIdx.x * DS_THREADS >= e) return;
+    const double4 bx = box[s];
+    const double h = 0.5 * bx.w;
+    const double cx = bx.x + h, cy = bx.y + h, cz = bx.z + h;
+    const bool valid = i < e;
+    float px = 0, py = 0, pz = 0;
+    if (valid) { px = (float)(S.x[i] - cx); py = (float)(S.y[i] - cy); pz = (float)(S.z[i] - cz); }
+    float ax = 0, ay = 0, az = 0, ph = 0;
+    for (uint32_t j0 = b; j0 < e; j0 += DS_THREADS) {
+        const uint32_t j = j0 + threadIdx.x;
+        __syncthreads();
+        tile[threadIdx.x] = j < e ? make_float4((float)(S.x[j] - cx), (float)(S.y[j] - cy), (float)(S.z[j] - cz),
+                                                (float)S.charge[j])
+                                  : make_float4(0, 0, 0, 0);
+        __syncthreads();
+        const uint32_t jn = min((uint32_t)DS_THREADS, e - j0);
+        if (!valid) continue;
+        if (!compat) {
+            for (uint32_t k = 0; k < jn; ++k) {
+                const float4 q = tile[k];
+                const float rx = q.x - px, ry = q.y - py, rz = q.z - pz;
+                const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+                const float inv = rsqrtf(r2);
+                const float w = (j0 + k == i) ? 0.f : q.w * inv;
+                const float w3 = w * inv * inv;
+                ax = fmaf(rx, w3, ax); ay = fmaf(ry, w3, ay); az = fmaf(rz, w3, az);
+                ph += w;
+            }
+        } else {
+            for (uint32_t k = 0; k < jn; ++k) {
+                if (j0 + k == i) continue;
+                const float4 q = tile[k];
+                const float d = q.x - px;
+                const float r = sqrtf(3.0f * d * d);
+                const float w = q.w * d / (r * r * r);
+                ax += w; ay += w; az += w;
+                ph += q.w / r;
+            }
+        }
+    }
+    if (!valid) return;
+    const float fac = (float)(S.charge[i] / S.mass[i]) * G;
+    S.ax[i] = fac * ax; S.ay[i] = fac * ay; S.az[i] = fac * az;
+    S.pot[i] = -G * ph;
+    if (kick_dt != 0.0) {
+        S.vx[i] += (double)(fac * ax) * kick_dt;
+        S.vy[i] += (double)(fac * ay) * kick_dt;
+        S.vz[i] += (double)(fac * az) * kick_dt;
+    }
+}
+
+void launch_direct_f32(const State& S, uint32_t n, const uint32_t* sys_off, int nsys, const double4* box, float G,
+                       float eps2, int compat_law, double kick_dt, hipStream_t st) {
+    // grid.x sized for the largest system: callers pass n = max system size
+    hipLaunchKernelGGL(direct_f32_kernel, dim3(ceil_div(n, DS_THREADS), nsys), dim3(DS_THREADS), 0, st, S, n, sys_off,
+                       nsys, box, G, eps2, compat_law, kick_dt);
+}
+
+// FP64 direct sum for selected targets (sorted slots): one wavefront per target.
+__global__ __launch_bounds__(256) void direct_fp64_kernel(State S, const uint32_t* sys_off, int nsys,
+                                                          const uint32_t* targets, uint32_t nt, double G,
+                                                          double eps2, double* acc, double* pot) {
+    const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (w >= nt) return;
+    const uint32_t i = targets[w];
+    int s = 0;
+    {
+        int lo = 0, hi = nsys - 1;
+        while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (sys_off[mid] <= i) lo = mid; else hi = mid - 1; }
+        s = lo;
+    }
+    const uint32_t b = sys_off[s], e = sys_off[s + 1];
+    const double px = S.x[i], py = S.y[i], pz = S.z[i];
+    double ax = 0, ay = 0, az = 0, ph = 0;
+    for (uint32_t j = b + lane_id(); j < e; j += WAVE) {
+        if (j == i) continue;
+        const double rx = S.x[j] - px, ry = S.y[j] - py, rz = S.z[j] - pz;
+        const double r2 = rx * rx + ry * ry + rz * rz + eps2;
+        const double inv = 1.0 / sqrt(r2);
+        const double q = S.charge[j] * inv;
+        const double q3 = q * inv * inv;
+        ax += rx * q3; ay += ry * q3; az += rz * q3;
+        ph += q;
+    }
+    ax = wave_sum(ax); ay = wave_sum(ay); az = wave_sum(az); ph = wave_sum(ph);
+    if (lane_id() == 0) {
+        const double fac = S.charge[i] / S.mass[i] * G;
+        acc[3 * w] = fac * ax; acc[3 * w + 1] = fac * ay; acc[3 * w + 2] = fac * az;
+        pot[w] = -G * ph;
+    }
+}
+
+void launch_direct_fp64(const State& S, const uint32_t* sys_off, int nsys, const uint32_t* targets, uint32_t nt,
+                        double G, double eps2, double* acc, double* pot, hipStream_t st) {
+    hipLaunchKernelGGL(direct_fp64_kernel, dim3(ceil_div(nt, 4)), dim3(256), 0, st, S, sys_off, nsys, targets, nt, G,
+                       eps2, acc, pot);
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics partials per block: KE, PE, P[3], Lang[3]
+constexpr int DG_THREADS = 256;
+uint32_t diagnostics_blocks(uint32_t n) { return min(1024u, ceil_div(n, DG_THREADS)); }
+
+__global__ __launch_bounds__(DG_THREADS) void diagnostics_kernel(State S, uint32_t n, double* partial) {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = blockIdx.x * DG_THREADS + threadIdx.x; i < n; i += gridDim.x * DG_THREADS) {
+        const double m = S.mass[i];
+        const double vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+        const double x = S.x[i], y = S.y[i], z = S.z[i];
+        v[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        v[1] += 0.5 * S.charge[i] * (double)S.pot[i];
+        v[2] += m * vx; v[3] += m * vy; v[4] += m * vz;
+        v[5] += m * (y * vz - z * vy); v[6] += m * (z * vx - x * vz); v[7] += m * (x * vy - y * vx);
+    }
+    __shared__ double sh[8][DG_THREADS / WAVE];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const double r = wave_sum(v[k]);
+        if (lane_id() == 0) sh[k][threadIdx.x / WAVE] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        double r = 0;
+        for (int w = 0; w < DG_THREADS / WAVE; ++w) r += sh[threadIdx.x][w];
+        partial[blockIdx.x * 8 + threadIdx.x] = r;
+    }
+}
+
+void launch_diagnostics(const State& S, uint32_t n, double* partial, uint32_t nblk, hipStream_t st) {
+    hipLaunchKernelGGL(diagnostics_kernel, dim3(nblk), dim3(DG_THREADS), 0, st, S, n, partial);
+}
+
+__global__ void inverse_perm_kernel(const uint32_t* id, uint32_t* slot_of, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) slot_of[id[i]] = i;
+}
+void launch_inverse_perm(const uint32_t* id, uint32_t* slot_of, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(inverse_perm_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, id, slot_of, n);
+}
+
+}  // namespace nbh

@@ -1,0 +1,112 @@
+// launch.h -- host-side launchers of the device kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace nbh {
+
+// ---- scan / sort (sort.hip)
+template <typename TI, typename TO>
+void exclusive_scan(const TI* in, TO* out, const uint32_t* n_dev, uint32_t n_max, TO* total, void* tmp,
+                    hipStream_t st);
+size_t scan_tmp_bytes(uint32_t n_max);
+size_t radix_tmp_bytes(uint32_t n);
+void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t n,
+                      int nbits, void* tmp, hipStream_t st);
+
+// ---- tree (tree.hip)
+void launch_bbox(const double* x, const double* y, const double* z, const uint32_t* sys_off, int nsys,
+                 uint32_t max_sys_n, int bits, int fixed, const double* flo, double fsize, double* part,
+                 double4* box, double* inv, float* unit, uint32_t* nonfinite, hipStream_t st);
+void launch_keys(const double* x, const double* y, const double* z, uint32_t n, const uint32_t* sys_off,
+                 int nsys, int bits, const double4* box, const double* inv, uint64_t* keys, uint32_t* perm,
+                 hipStream_t st);
+size_t tree_tmp_bytes(uint32_t n, int max_depth);
+void launch_tree_build(const uint64_t* keys, uint32_t n, const uint32_t* sys_off, int nsys, int bits, int cap,
+                       int max_depth, NodeView nodes, uint32_t* node_begin, uint32_t* leaf_ids, uint32_t node_cap,
+                       Counters* C, void* tmp, hipStream_t st);
+void launch_tree_emit(const uint64_t* keys, uint32_t n, const uint32_t* sys_off, int nsys, int bits, int cap,
+                      int max_depth, NodeView nodes, uint32_t* node_begin, uint32_t* leaf_ids, uint32_t node_cap,
+                      Counters* C, void* tmp, hipStream_t st);
+void launch_tree_link(const uint64_t* keys, uint32_t n, const uint32_t* sys_off, int nsys, int bits, int cap,
+                      int max_depth, const Counters* C, uint32_t num_nodes, const uint32_t* node_begin,
+                      NodeView nodes, hipStream_t st);
+
+// ---- walk (walk.hip)
+struct WalkParams {
+    NodeView nodes;
+    Counters* C;
+    int bits;
+    uint32_t num4;  // 4 num^2
+    uint32_t den3;  // 3 den^2
+    int* row_node;
+    int* row_src;
+    uint32_t* row_sl_start;
+    uint32_t* row_sl_cnt;
+    uint32_t row_cap;
+    int* sl;
+    uint32_t sl_cap;
+    uint32_t* m2l_cnt;
+    uint32_t* p2p_cnt;
+    unsigned long long* m2l_cur;
+    unsigned long long* p2p_cur;
+    int* m2l_src;
+    int* p2p_src;
+    unsigned long long m2l_cap, p2p_cap;
+};
+void launch_walk_phase1(const WalkParams& W, int s, uint32_t grid, hipStream_t st);
+void launch_walk_phase2(const WalkParams& W, int s, uint32_t grid, hipStream_t st);
+
+// ---- particle state in sorted order (FP64 positions/velocities)
+struct State {
+    double *x, *y, *z, *vx, *vy, *vz, *mass, *charge;
+    float *ax, *ay, *az, *pot;
+    uint32_t* id;
+};
+
+// ---- force pipeline (fmm.hip)
+struct FmmParams {
+    NodeView nodes;
+    const Counters* C;
+    const double4* box;
+    const float* unit;     // per system: half of the finest cell, float
+    int bits;
+    float eps2;
+    float G;
+    float* M;              // [nodes][ncp]
+    float* L;              // [nodes][ncp]
+    int ncp;               // padded coefficient stride
+    const uint32_t* leaf_ids;
+    float4* pf;            // per particle: offset from its leaf centre, charge
+    const unsigned long long* m2l_off;
+    const int* m2l_src;
+    const unsigned long long* p2p_off;
+    const int* p2p_src;
+};
+
+void launch_gather_state(const State& src, const State& dst, const uint32_t* perm, uint32_t n, hipStream_t st);
+void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
+void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
+void launch_m2l_plain(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st);
+void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
+void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st);
+
+// integrator updates: mode 0 = KDK first half (kick dt_k, drift dt_d),
+// 1 = Euler (x += v dt; v += a dt), 2 = kick-drift (v += a dt; x += v dt)
+void launch_integrate(const State& S, uint32_t n, int mode, double dt_k, double dt_d, hipStream_t st);
+
+// direct sums
+void launch_direct_f32(const State& S, uint32_t n, const uint32_t* sys_off, int nsys, const double4* box,
+                       float G, float eps2, int compat_law, double kick_dt, hipStream_t st);
+void launch_direct_fp64(const State& S, const uint32_t* sys_off, int nsys, const uint32_t* targets, uint32_t nt,
+                        double G, double eps2, double* acc, double* pot, hipStream_t st);
+
+void launch_diagnostics(const State& S, uint32_t n, double* partial, uint32_t nblk, hipStream_t st);
+uint32_t diagnostics_blocks(uint32_t n);
+
+void launch_inverse_perm(const uint32_t* id, uint32_t* slot_of, uint32_t n, hipStream_t st);
+
+}  // namespace nbh

@@ -1,0 +1,905 @@
+// nbody_hip.hip -- host orchestration and C ABI of libnbody_hip.
+//
+// Replaces OpenClSimulation's host side (open_cl_simulation.cpp:15-616): the
+// constructor's upload and octree build, step()'s phase sequence, and
+// particles().  All per-step phases run on one HIP stream; the host reads the
+// device counters twice per force evaluation (after the tree count, to size
+// the node arrays, and after the walk's counting phase, to size the lists).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nbody_hip.h"
+#include "common.h"
+#include "expansion.h"
+#include "launch.h"
+
+using namespace nbh;
+
+namespace {
+
+thread_local std::string g_create_error = "no error";
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes && p) return hipSuccess;
+        size_t nb = std::max(need, bytes + bytes / 2);
+        nb = std::max<size_t>(nb, 256);
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            bytes = 0;
+        }
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) { p = nullptr; return e; }
+        bytes = nb;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+struct StateBufs {
+    DevBuf x, y, z, vx, vy, vz, mass, charge, ax, ay, az, pot, id;
+    hipError_t alloc(size_t n) {
+        DevBuf* d[] = {&x, &y, &z, &vx, &vy, &vz, &mass, &charge};
+        for (DevBuf* b : d) { hipError_t e = b->ensure(n * 8); if (e) return e; }
+        DevBuf* f[] = {&ax, &ay, &az, &pot, &id};
+        for (DevBuf* b : f) { hipError_t e = b->ensure(n * 4); if (e) return e; }
+        return hipSuccess;
+    }
+    State view() const {
+        return State{x.as<double>(), y.as<double>(), z.as<double>(), vx.as<double>(), vy.as<double>(),
+                     vz.as<double>(), mass.as<double>(), charge.as<double>(), ax.as<float>(), ay.as<float>(),
+                     az.as<float>(), pot.as<float>(), id.as<uint32_t>()};
+    }
+    void release() {
+        DevBuf* all[] = {&x, &y, &z, &vx, &vy, &vz, &mass, &charge, &ax, &ay, &az, &pot, &id};
+        for (DevBuf* b : all) b->release();
+    }
+};
+
+}  // namespace
+
+struct nbh_handle {
+    nbh_config cfg{};
+    std::string err = "no error";
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t n = 0;
+    int nsys = 1;
+    int bits = 21;
+    uint32_t max_sys_n = 0;
+    int theta_num = 1, theta_den = 2;
+    double time = 0.0;
+    bool acc_valid = false;
+    bool timing = false;
+    std::vector<uint32_t> sys_off_h;
+
+    StateBufs st[2];
+    int cur = 0;
+    DevBuf sys_off, keys, keys_alt, perm, perm_alt, sort_tmp, tree_tmp, scan_tmp;
+    DevBuf box, inv, unit, bbox_part;
+    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf;
+    DevBuf row_node, row_src, row_sl_start, row_sl_cnt, sl, m2l_cnt, p2p_cnt, m2l_off, p2p_off, m2l_cur, p2p_cur,
+        m2l_src, p2p_src;
+    DevBuf counters, diag, tmp_small;
+    Counters* hc = nullptr;  // pinned host copy
+    uint32_t node_cap = 0, row_cap = 0, sl_cap = 0;
+    unsigned long long m2l_cap = 0, p2p_cap = 0;
+    uint32_t num_nodes = 0, num_leaves = 0, depth = 0;
+    nbh_stats stats{};
+    hipEvent_t ev[8] = {};
+
+    int fail(int code, const char* fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+#define HIPCHK(h, expr)                                                                                     \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            return (h)->fail(e_ == hipErrorOutOfMemory ? NBH_E_NOMEM : NBH_E_HIP, "%s failed: %s (%s:%d)", \
+                             #expr, hipGetErrorString(e_), __FILE__, __LINE__);                             \
+    } while (0)
+
+#define CHK(expr)                  \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != NBH_OK) return r_; \
+    } while (0)
+
+namespace {
+
+int ncp_of(int p) { return (ncoef(p) + 3) & ~3; }
+
+bool theta_rational(double theta, int* num, int* den) {
+    for (int d = 1; d <= 64; ++d) {
+        double nf = theta * d;
+        int nn = (int)std::lround(nf);
+        if (nn >= 1 && nn <= 64 && std::fabs(nf - nn) <= 1e-9 * std::max(1.0, nf)) {
+            *num = nn;
+            *den = d;
+            return true;
+        }
+    }
+    return false;
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void zero_u32(void* p, size_t n, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)ceil_div((unsigned)std::min<size_t>(n, 0xffffffffu), 256)),
+                       dim3(256), 0, st, (uint32_t*)p, n);
+}
+
+int validate_config(const nbh_config* c, std::string& why) {
+    char b[256];
+    if (!c) { why = "config is NULL"; return NBH_E_INVALID; }
+    if (c->struct_size != sizeof(nbh_config)) {
+        snprintf(b, sizeof b, "config struct_size %u != %zu (ABI mismatch)", c->struct_size, sizeof(nbh_config));
+        why = b;
+        return NBH_E_INVALID;
+    }
+    if (c->solver > NBH_SOLVER_DIRECT) { why = "unknown solver"; return NBH_E_INVALID; }
+    if (!(c->dt > 0) || !std::isfinite(c->dt)) { why = "dt must be finite and > 0"; return NBH_E_INVALID; }
+    if (!(c->eps >= 0) || !std::isfinite(c->eps)) { why = "eps must be finite and >= 0"; return NBH_E_INVALID; }
+    if (!std::isfinite(c->G)) { why = "G must be finite"; return NBH_E_INVALID; }
+    if (c->integrator > NBH_INT_KICK_DRIFT) { why = "unknown integrator"; return NBH_E_INVALID; }
+    if (c->force_law > NBH_LAW_COMPAT_NAIVE) { why = "unknown force law"; return NBH_E_INVALID; }
+    if (c->force_law == NBH_LAW_COMPAT_NAIVE && c->solver != NBH_SOLVER_DIRECT) {
+        why = "the compat_naive force law needs the direct solver";
+        return NBH_E_INVALID;
+    }
+    if (c->solver == NBH_SOLVER_FMM) {
+        int num, den;
+        if (!(c->theta > 0) || !theta_rational(c->theta, &num, &den)) {
+            snprintf(b, sizeof b, "theta=%g must be > 0 and equal num/den with num, den <= 64", c->theta);
+            why = b;
+            return NBH_E_INVALID;
+        }
+        if (c->leaf_capacity < 1 || c->leaf_capacity > 2048) { why = "leaf_capacity must be in [1, 2048]"; return NBH_E_INVALID; }
+        if (c->order < 1 || c->order > 6) { why = "order must be in [1, 6]"; return NBH_E_INVALID; }
+        if (c->max_depth > 21) { why = "max_depth must be <= 21"; return NBH_E_INVALID; }
+        if (c->m2l > NBH_M2L_BF16X3) { why = "unknown m2l kernel"; return NBH_E_INVALID; }
+    }
+    if (c->fixed_root && !(c->root_size > 0)) { why = "fixed_root needs root_size > 0"; return NBH_E_INVALID; }
+    return NBH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+static int alloc_nodes(nbh_handle* h, uint32_t need) {
+    if (need <= h->node_cap) return NBH_OK;
+    uint32_t cap = std::max<uint32_t>(need + need / 4, 1024);
+    const int ncp = ncp_of((int)h->cfg.order);
+    HIPCHK(h, h->geo.ensure((size_t)cap * 16));
+    HIPCHK(h, h->link.ensure((size_t)cap * 16));
+    HIPCHK(h, h->nsys_of.ensure((size_t)cap * 4));
+    HIPCHK(h, h->node_begin.ensure((size_t)cap * 4));
+    HIPCHK(h, h->M.ensure((size_t)cap * ncp * 4));
+    HIPCHK(h, h->L.ensure((size_t)cap * ncp * 4));
+    HIPCHK(h, h->m2l_cnt.ensure((size_t)cap * 4));
+    HIPCHK(h, h->p2p_cnt.ensure((size_t)cap * 4));
+    HIPCHK(h, h->m2l_off.ensure((size_t)(cap + 1) * 8));
+    HIPCHK(h, h->p2p_off.ensure((size_t)(cap + 1) * 8));
+    HIPCHK(h, h->m2l_cur.ensure((size_t)(cap + 1) * 8));
+    HIPCHK(h, h->p2p_cur.ensure((size_t)(cap + 1) * 8));
+    HIPCHK(h, h->scan_tmp.ensure(scan_tmp_bytes(cap + 1) + 256));
+    h->node_cap = cap;
+    return NBH_OK;
+}
+
+static int read_counters(nbh_handle* h) {
+    HIPCHK(h, hipMemcpyAsync(h->hc, h->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return NBH_OK;
+}
+
+static NodeView node_view(nbh_handle* h) {
+    return NodeView{h->geo.as<int4>(), h->link.as<int4>(), h->nsys_of.as<int>()};
+}
+
+static void record(nbh_handle* h, int k) {
+    if (h->timing) (void)hipEventRecord(h->ev[k], h->stream);
+}
+
+// bbox -> keys -> sort -> gather: the state ends in Morton order
+static int sort_particles(nbh_handle* h) {
+    const uint32_t n = (uint32_t)h->n;
+    State a = h->st[h->cur].view(), b = h->st[1 - h->cur].view();
+    Counters* dC = h->counters.as<Counters>();
+    launch_bbox(a.x, a.y, a.z, h->sys_off.as<uint32_t>(), h->nsys, h->max_sys_n, h->bits, (int)h->cfg.fixed_root,
+                h->cfg.root_lo, h->cfg.root_size, h->bbox_part.as<double>(), h->box.as<double4>(),
+                h->inv.as<double>(), h->unit.as<float>(), &dC->nonfinite, h->stream);
+    launch_keys(a.x, a.y, a.z, n, h->sys_off.as<uint32_t>(), h->nsys, h->bits, h->box.as<double4>(),
+                h->inv.as<double>(), h->keys.as<uint64_t>(), h->perm.as<uint32_t>(), h->stream);
+    int keybits = 3 * h->bits;
+    if (h->nsys > 1) {
+        int sb = 0;
+        while ((1 << sb) < h->nsys) ++sb;
+        keybits += sb;
+    }
+    radix_sort_pairs(h->keys.as<uint64_t>(), h->perm.as<uint32_t>(), h->keys_alt.as<uint64_t>(),
+                     h->perm_alt.as<uint32_t>(), n, keybits, h->sort_tmp.p, h->stream);
+    launch_gather_state(a, b, h->perm.as<uint32_t>(), n, h->stream);
+    h->cur = 1 - h->cur;
+    return NBH_OK;
+}
+
+static int build_tree(nbh_handle* h) {
+    const uint32_t n = (uint32_t)h->n;
+    Counters* dC = h->counters.as<Counters>();
+    zero_u32(dC, sizeof(Counters) / 4, h->stream);
+    CHK(sort_particles(h));
+    const int D = (int)h->cfg.max_depth;
+    launch_tree_build(h->keys.as<uint64_t>(), n, h->sys_off.as<uint32_t>(), h->nsys, h->bits,
+                      (int)h->cfg.leaf_capacity, D, node_view(h), h->node_begin.as<uint32_t>(),
+                      h->leaf_ids.as<uint32_t>(), h->node_cap, dC, h->tree_tmp.p, h->stream);
+    CHK(read_counters(h));
+    if (h->hc->nonfinite) return h->fail(NBH_E_NONFINITE, "non-finite particle position");
+    h->num_nodes = h->hc->num_nodes;
+    h->num_leaves = h->hc->num_leaves;
+    h->depth = h->hc->depth;
+    CHK(alloc_nodes(h, h->num_nodes + 1));
+    launch_tree_emit(h->keys.as<uint64_t>(), n, h->sys_off.as<uint32_t>(), h->nsys, h->bits,
+                     (int)h->cfg.leaf_capacity, D, node_view(h), h->node_begin.as<uint32_t>(),
+                     h->leaf_ids.as<uint32_t>(), h->node_cap, dC, h->tree_tmp.p, h->stream);
+    launch_tree_link(h->keys.as<uint64_t>(), n, h->sys_off.as<uint32_t>(), h->nsys, h->bits,
+                     (int)h->cfg.leaf_capacity, D, dC, h->num_nodes, h->node_begin.as<uint32_t>(), node_view(h),
+                     h->stream);
+    return NBH_OK;
+}
+
+static FmmParams fmm_params(nbh_handle* h) {
+    FmmParams F{};
+    F.nodes = node_view(h);
+    F.C = h->counters.as<Counters>();
+    F.box = h->box.as<double4>();
+    F.unit = h->unit.as<float>();
+    F.bits = h->bits;
+    F.eps2 = (float)(h->cfg.eps * h->cfg.eps);
+    F.G = (float)h->cfg.G;
+    F.M = h->M.as<float>();
+    F.L = h->L.as<float>();
+    F.ncp = ncp_of((int)h->cfg.order);
+    F.leaf_ids = h->leaf_ids.as<uint32_t>();
+    F.pf = h->pf.as<float4>();
+    F.m2l_off = h->m2l_off.as<unsigned long long>();
+    F.m2l_src = h->m2l_src.as<int>();
+    F.p2p_off = h->p2p_off.as<unsigned long long>();
+    F.p2p_src = h->p2p_src.as<int>();
+    return F;
+}
+
+static int run_walk(nbh_handle* h) {
+    Counters* dC = h->counters.as<Counters>();
+    const uint32_t nn = h->num_nodes;
+    if (h->row_cap < 2 * nn + 1024) {
+        h->row_cap = 2 * nn + 1024;
+        HIPCHK(h, h->row_node.ensure((size_t)h->row_cap * 4));
+        HIPCHK(h, h->row_src.ensure((size_t)h->row_cap * 4));
+        HIPCHK(h, h->row_sl_start.ensure((size_t)h->row_cap * 4));
+        HIPCHK(h, h->row_sl_cnt.ensure((size_t)h->row_cap * 4));
+    }
+    if (h->sl_cap < 64u * nn + 4096) {
+        h->sl_cap = 64u * nn + 4096;
+        HIPCHK(h, h->sl.ensure((size_t)h->sl_cap * 4));
+    }
+    WalkParams W{};
+    W.nodes = node_view(h);
+    W.C = dC;
+    W.bits = h->bits;
+    W.num4 = 4u * h->theta_num * h->theta_num;
+    W.den3 = 3u * h->theta_den * h->theta_den;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        W.row_node = h->row_node.as<int>();
+        W.row_src = h->row_src.as<int>();
+        W.row_sl_start = h->row_sl_start.as<uint32_t>();
+        W.row_sl_cnt = h->row_sl_cnt.as<uint32_t>();
+        W.row_cap = h->row_cap;
+        W.sl = h->sl.as<int>();
+        W.sl_cap = h->sl_cap;
+        W.m2l_cnt = h->m2l_cnt.as<uint32_t>();
+        W.p2p_cnt = h->p2p_cnt.as<uint32_t>();
+        zero_u32(h->m2l_cnt.p, nn, h->stream);
+        zero_u32(h->p2p_cnt.p, nn, h->stream);
+        // reset the walk part of the counters
+        zero_u32(&dC->carried[0], (offsetof(Counters, nonfinite) - offsetof(Counters, carried)) / 4, h->stream);
+        for (uint32_t s = 0; s <= h->depth; ++s) {
+            const uint32_t rows = h->hc->level_count[s];
+            const uint32_t grid = std::max<uint32_t>(std::min<uint32_t>(ceil_div(rows, 4) + 64, 8192), 256);
+            launch_walk_phase1(W, (int)s, grid, h->stream);
+        }
+        unsigned long long* mo = h->m2l_off.as<unsigned long long>();
+        unsigned long long* po = h->p2p_off.as<unsigned long long>();
+        exclusive_scan<uint32_t, unsigned long long>(h->m2l_cnt.as<uint32_t>(), mo, nullptr, nn, mo + nn,
+                                                     h->scan_tmp.p, h->stream);
+        exclusive_scan<uint32_t, unsigned long long>(h->p2p_cnt.as<uint32_t>(), po, nullptr, nn, po + nn,
+                                                     h->scan_tmp.p, h->stream);
+        HIPCHK(h, hipMemcpyAsync(&dC->m2l_total, mo + nn, 8, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(&dC->p2p_total, po + nn, 8, hipMemcpyDeviceToDevice, h->stream));
+        CHK(read_counters(h));
+        const uint32_t ovf = h->hc->overflow;
+        if (ovf & OVF_SL) {
+            h->sl_cap = std::max<uint32_t>(h->sl_cap * 2, h->hc->sl_used + h->hc->sl_used / 4);
+            HIPCHK(h, h->sl.ensure((size_t)h->sl_cap * 4));
+        }
+        if (ovf & OVF_ROWS) {
+            uint64_t need = 0;
+            for (uint32_t s = 0; s <= h->depth + 1; ++s) need += h->hc->level_count[s] + h->hc->carried[s];
+            h->row_cap = (uint32_t)std::max<uint64_t>((uint64_t)h->row_cap * 2, need + need / 4);
+            HIPCHK(h, h->row_node.ensure((size_t)h->row_cap * 4));
+            HIPCHK(h, h->row_src.ensure((size_t)h->row_cap * 4));
+            HIPCHK(h, h->row_sl_start.ensure((size_t)h->row_cap * 4));
+            HIPCHK(h, h->row_sl_cnt.ensure((size_t)h->row_cap * 4));
+        }
+        if (ovf & (OVF_SL | OVF_ROWS)) {
+            uint32_t keep = ovf & ~(uint32_t)(OVF_SL | OVF_ROWS);
+            HIPCHK(h, hipMemcpyAsync(&dC->overflow, &keep, 4, hipMemcpyHostToDevice, h->stream));
+            continue;
+        }
+        break;
+    }
+    if (h->hc->overflow) return h->fail(NBH_E_INTERNAL, "walk buffers overflow (flags %u)", h->hc->overflow);
+    const unsigned long long nm = h->hc->m2l_total, np = h->hc->p2p_total;
+    if (nm > h->m2l_cap) {
+        h->m2l_cap = nm + nm / 4 + 1024;
+        HIPCHK(h, h->m2l_src.ensure((size_t)h->m2l_cap * 4));
+    }
+    if (np > h->p2p_cap) {
+        h->p2p_cap = np + np / 4 + 1024;
+        HIPCHK(h, h->p2p_src.ensure((size_t)h->p2p_cap * 4));
+    }
+    W.m2l_cur = h->m2l_cur.as<unsigned long long>();
+    W.p2p_cur = h->p2p_cur.as<unsigned long long>();
+    W.m2l_src = h->m2l_src.as<int>();
+    W.p2p_src = h->p2p_src.as<int>();
+    W.m2l_cap = h->m2l_cap;
+    W.p2p_cap = h->p2p_cap;
+    HIPCHK(h, hipMemcpyAsync(h->m2l_cur.p, h->m2l_off.p, (size_t)nn * 8, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->p2p_cur.p, h->p2p_off.p, (size_t)nn * 8, hipMemcpyDeviceToDevice, h->stream));
+    for (uint32_t s = 0; s <= h->depth; ++s) {
+        const uint32_t rows = h->hc->level_count[s] + h->hc->carried[s];
+        const uint32_t grid = std::max<uint32_t>(std::min<uint32_t>(ceil_div(rows, 4), 8192), 1);
+        launch_walk_phase2(W, (int)s, grid, h->stream);
+    }
+    return NBH_OK;
+}
+
+// One force evaluation; kick_dt != 0 applies v += a * kick_dt afterwards.
+static int force_fmm(nbh_handle* h, double kick_dt) {
+    const int p = (int)h->cfg.order;
+    record(h, 0);
+    CHK(build_tree(h));
+    record(h, 1);
+    FmmParams F = fmm_params(h);
+    State S = h->st[h->cur].view();
+    launch_p2m(p, F, S, h->num_leaves, h->stream);
+    for (int l = (int)h->depth - 1; l >= 0; --l)
+        launch_m2m_level(p, F, l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
+    record(h, 2);
+    CHK(run_walk(h));
+    record(h, 3);
+    F = fmm_params(h);
+    launch_m2l_plain(p, F, h->num_nodes, h->stream);
+    record(h, 4);
+    for (uint32_t l = 1; l <= h->depth; ++l)
+        launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
+    record(h, 5);
+    launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
+    record(h, 6);
+    HIPCHK(h, hipGetLastError());
+    // stats (read at the end of the evaluation; one more sync)
+    CHK(read_counters(h));
+    if (h->hc->overflow) return h->fail(NBH_E_INTERNAL, "overflow flags %u", h->hc->overflow);
+    h->stats.num_nodes = h->num_nodes;
+    h->stats.num_leaves = h->num_leaves;
+    h->stats.depth = h->depth;
+    h->stats.m2l_pairs = h->hc->m2l_total;
+    h->stats.p2p_pairs = h->hc->p2p_total;
+    h->stats.p2p_interactions = h->hc->p2p_inter;
+    h->stats.split_pairs = h->hc->split_total;
+    if (h->timing) {
+        float ms[6];
+        for (int k = 0; k < 6; ++k) (void)hipEventElapsedTime(&ms[k], h->ev[k], h->ev[k + 1]);
+        h->stats.ms_tree = ms[0];
+        h->stats.ms_moments = ms[1];
+        h->stats.ms_walk = ms[2];
+        h->stats.ms_m2l = ms[3];
+        h->stats.ms_l2l = ms[4];
+        h->stats.ms_p2p = ms[5];
+        float tot = 0;
+        (void)hipEventElapsedTime(&tot, h->ev[0], h->ev[6]);
+        h->stats.ms_total = tot;
+    }
+    return NBH_OK;
+}
+
+static int force_direct(nbh_handle* h, double kick_dt) {
+    State S = h->st[h->cur].view();
+    Counters* dC = h->counters.as<Counters>();
+    zero_u32(dC, sizeof(Counters) / 4, h->stream);
+    launch_bbox(S.x, S.y, S.z, h->sys_off.as<uint32_t>(), h->nsys, h->max_sys_n, h->bits, (int)h->cfg.fixed_root,
+                h->cfg.root_lo, h->cfg.root_size, h->bbox_part.as<double>(), h->box.as<double4>(),
+                h->inv.as<double>(), h->unit.as<float>(), &dC->nonfinite, h->stream);
+    launch_direct_f32(S, h->max_sys_n, h->sys_off.as<uint32_t>(), h->nsys, h->box.as<double4>(), (float)h->cfg.G,
+                      (float)(h->cfg.eps * h->cfg.eps), (int)h->cfg.force_law, kick_dt, h->stream);
+    HIPCHK(h, hipGetLastError());
+    CHK(read_counters(h));
+    if (h->hc->nonfinite) return h->fail(NBH_E_NONFINITE, "non-finite particle position");
+    return NBH_OK;
+}
+
+static int force_eval(nbh_handle* h, double kick_dt) {
+    return h->cfg.solver == NBH_SOLVER_FMM ? force_fmm(h, kick_dt) : force_direct(h, kick_dt);
+}
+
+static int do_step(nbh_handle* h) {
+    const double dt = h->cfg.dt;
+    const uint32_t n = (uint32_t)h->n;
+    switch (h->cfg.integrator) {
+        case NBH_INT_KDK:
+            if (!h->acc_valid) { CHK(force_eval(h, 0.0)); h->acc_valid = true; }
+            launch_integrate(h->st[h->cur].view(), n, 0, 0.5 * dt, dt, h->stream);
+            CHK(force_eval(h, 0.5 * dt));
+            break;
+        case NBH_INT_EULER:
+            CHK(force_eval(h, 0.0));
+            launch_integrate(h->st[h->cur].view(), n, 1, dt, dt, h->stream);
+            h->acc_valid = false;
+            break;
+        default:
+            CHK(force_eval(h, 0.0));
+            launch_integrate(h->st[h->cur].view(), n, 2, dt, dt, h->stream);
+            h->acc_valid = false;
+            break;
+    }
+    h->time += dt;
+    return NBH_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int nbh_abi_version(void) { return NBH_ABI_VERSION; }
+size_t nbh_config_size(void) { return sizeof(nbh_config); }
+
+int nbh_config_default(nbh_config* c) {
+    if (!c) return NBH_E_INVALID;
+    std::memset(c, 0, sizeof *c);
+    c->struct_size = sizeof(nbh_config);
+    c->solver = NBH_SOLVER_FMM;
+    c->G = 1.0;              // FORCE_CONSTANT = -1 (field.cl:7-9): attractive, unit strength
+    c->eps = 0.01;           // PARTICLE_RADIUS (field.cl:3-5)
+    c->theta = 0.5;          // NODE_APPROX_RATIO (interaction.cl:3-5)
+    c->dt = 1e-3;
+    c->leaf_capacity = 8;    // open_cl_simulation.cpp:41-47
+    c->order = 4;
+    c->max_depth = 21;
+    c->m2l = NBH_M2L_PLAIN;
+    c->integrator = NBH_INT_KDK;
+    c->force_law = NBH_LAW_SOFTENED;
+    c->device = 0;
+    return NBH_OK;
+}
+
+int nbh_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char* nbh_last_error(const nbh_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_off, const double* pos,
+                       const double* vel, const double* mass, const double* charge, nbh_handle** out) {
+    if (!out) { g_create_error = "out is NULL"; return NBH_E_INVALID; }
+    *out = nullptr;
+    std::string why;
+    int rc = validate_config(cfg, why);
+    if (rc) { g_create_error = why; return rc; }
+    if (nsys < 1 || !sys_off) { g_create_error = "need at least one system"; return NBH_E_INVALID; }
+    const size_t n = sys_off[nsys];
+    if (sys_off[0] != 0 || n < 1 || n > 0x7fffffffu) { g_create_error = "bad particle count"; return NBH_E_INVALID; }
+    for (size_t s = 0; s < nsys; ++s)
+        if (sys_off[s + 1] <= sys_off[s]) { g_create_error = "every system needs >= 1 particle"; return NBH_E_INVALID; }
+    if (!pos || !vel || !mass) { g_create_error = "pos, vel and mass are required"; return NBH_E_INVALID; }
+    for (size_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(pos[3 * i + k]) || !std::isfinite(vel[3 * i + k])) {
+                g_create_error = "non-finite position or velocity";
+                return NBH_E_NONFINITE;
+            }
+        if (!(mass[i] > 0) || !std::isfinite(mass[i])) { g_create_error = "masses must be finite and > 0"; return NBH_E_INVALID; }
+        if (charge && !std::isfinite(charge[i])) { g_create_error = "non-finite charge"; return NBH_E_NONFINITE; }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device available"; return NBH_E_HIP; }
+    if (cfg->device < 0 || cfg->device >= ndev) { g_create_error = "device ordinal out of range"; return NBH_E_INVALID; }
+
+    nbh_handle* h = new nbh_handle;
+    h->cfg = *cfg;
+    h->n = n;
+    h->nsys = (int)nsys;
+    h->device = cfg->device;
+    h->sys_off_h.assign(sys_off, sys_off + nsys + 1);
+    for (size_t s = 0; s < nsys; ++s) h->max_sys_n = std::max(h->max_sys_n, sys_off[s + 1] - sys_off[s]);
+    if (nsys > 1) {
+        int sb = 0;
+        while ((size_t(1) << sb) < nsys) ++sb;
+        h->bits = std::min(21, (64 - sb) / 3);
+    }
+    if (h->cfg.max_depth > (uint32_t)h->bits) h->cfg.max_depth = (uint32_t)h->bits;
+    if (cfg->solver == NBH_SOLVER_FMM) theta_rational(cfg->theta, &h->theta_num, &h->theta_den);
+
+    auto bail = [&](int code) {
+        g_create_error = h->err;
+        nbh_destroy(h);
+        return code;
+    };
+#define CCHK(expr)                                                                                          \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            h->fail(e_ == hipErrorOutOfMemory ? NBH_E_NOMEM : NBH_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+            return bail(e_ == hipErrorOutOfMemory ? NBH_E_NOMEM : NBH_E_HIP);                               \
+        }                                                                                                   \
+    } while (0)
+    CCHK(hipSetDevice(h->device));
+    CCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    for (auto& e : h->ev) CCHK(hipEventCreate(&e));
+    CCHK(h->st[0].alloc(n));
+    CCHK(h->st[1].alloc(n));
+    CCHK(h->sys_off.ensure((nsys + 1) * 4));
+    CCHK(h->keys.ensure(n * 8));
+    CCHK(h->keys_alt.ensure(n * 8));
+    CCHK(h->perm.ensure(n * 4));
+    CCHK(h->perm_alt.ensure(n * 4));
+    CCHK(h->sort_tmp.ensure(radix_tmp_bytes((uint32_t)n)));
+    CCHK(h->tree_tmp.ensure(tree_tmp_bytes((uint32_t)n, (int)h->cfg.max_depth)));
+    CCHK(h->box.ensure(nsys * sizeof(double4)));
+    CCHK(h->inv.ensure(nsys * 8));
+    CCHK(h->unit.ensure(nsys * 4));
+    CCHK(h->bbox_part.ensure(nsys * (size_t)ceil_div(h->max_sys_n, 4096) * 6 * 8 + 64));
+    CCHK(h->leaf_ids.ensure(n * 4));
+    CCHK(h->pf.ensure(n * 16));
+    CCHK(h->counters.ensure(sizeof(Counters)));
+    CCHK(h->diag.ensure(1024 * 8 * 8));
+    CCHK(h->tmp_small.ensure(64));
+    CCHK(hipHostMalloc((void**)&h->hc, sizeof(Counters), hipHostMallocDefault));
+    std::memset(h->hc, 0, sizeof(Counters));
+    if (cfg->solver == NBH_SOLVER_FMM) {
+        int r = alloc_nodes(h, (uint32_t)std::min<size_t>(n / 2 + 64, 0x7fffffff));
+        if (r) return bail(r);
+    }
+    CCHK(hipMemcpy(h->sys_off.p, sys_off, (nsys + 1) * 4, hipMemcpyHostToDevice));
+    {
+        std::vector<double> buf(n);
+        State S = h->st[0].view();
+        double* dst[6] = {S.x, S.y, S.z, S.vx, S.vy, S.vz};
+        for (int k = 0; k < 6; ++k) {
+            const double* srcp = k < 3 ? pos : vel;
+            for (size_t i = 0; i < n; ++i) buf[i] = srcp[3 * i + (k % 3)];
+            CCHK(hipMemcpy(dst[k], buf.data(), n * 8, hipMemcpyHostToDevice));
+        }
+        CCHK(hipMemcpy(S.mass, mass, n * 8, hipMemcpyHostToDevice));
+        if (charge) CCHK(hipMemcpy(S.charge, charge, n * 8, hipMemcpyHostToDevice));
+        else CCHK(hipMemcpy(S.charge, mass, n * 8, hipMemcpyHostToDevice));
+        std::vector<uint32_t> ids(n);
+        for (size_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
+        CCHK(hipMemcpy(S.id, ids.data(), n * 4, hipMemcpyHostToDevice));
+        CCHK(hipMemset(S.ax, 0, n * 4));
+        CCHK(hipMemset(S.ay, 0, n * 4));
+        CCHK(hipMemset(S.az, 0, n * 4));
+        CCHK(hipMemset(S.pot, 0, n * 4));
+    }
+#undef CCHK
+    *out = h;
+    return NBH_OK;
+}
+
+int nbh_create(const nbh_config* cfg, size_t n, const double* pos, const double* vel, const double* mass,
+               const double* charge, nbh_handle** out) {
+    if (n < 1 || n > 0x7fffffffu) {
+        g_create_error = "n must be in [1, 2^31)";
+        if (out) *out = nullptr;
+        return NBH_E_INVALID;
+    }
+    uint32_t off[2] = {0, (uint32_t)n};
+    return create_impl(cfg, 1, off, pos, vel, mass, charge, out);
+}
+
+int nbh_create_ensemble(const nbh_config* cfg, size_t nsys, const uint32_t* sys_off, const double* pos,
+                        const double* vel, const double* mass, const double* charge, nbh_handle** out) {
+    if (nsys > (1u << 20)) {
+        g_create_error = "too many systems";
+        if (out) *out = nullptr;
+        return NBH_E_INVALID;
+    }
+    return create_impl(cfg, nsys, sys_off, pos, vel, mass, charge, out);
+}
+
+void nbh_destroy(nbh_handle* h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->st[0].release();
+    h->st[1].release();
+    DevBuf* all[] = {&h->sys_off, &h->keys, &h->keys_alt, &h->perm, &h->perm_alt, &h->sort_tmp, &h->tree_tmp,
+                     &h->scan_tmp, &h->box, &h->inv, &h->unit, &h->bbox_part, &h->geo, &h->link, &h->nsys_of,
+                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->row_node, &h->row_src,
+                     &h->row_sl_start, &h->row_sl_cnt, &h->sl, &h->m2l_cnt, &h->p2p_cnt, &h->m2l_off,
+                     &h->p2p_off, &h->m2l_cur, &h->p2p_cur, &h->m2l_src, &h->p2p_src, &h->counters, &h->diag,
+                     &h->tmp_small};
+    for (DevBuf* b : all) b->release();
+    if (h->hc) (void)hipHostFree(h->hc);
+    for (auto& e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int nbh_step(nbh_handle* h, int nsteps, double* t_out) {
+    if (!h) return NBH_E_INVALID;
+    if (nsteps < 0) return h->fail(NBH_E_INVALID, "nsteps must be >= 0");
+    HIPCHK(h, hipSetDevice(h->device));
+    for (int s = 0; s < nsteps; ++s) CHK(do_step(h));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (t_out) *t_out = h->time;
+    return NBH_OK;
+}
+
+int nbh_get_state(nbh_handle* h, double* pos, double* vel, double* mass, double* charge, double* t) {
+    if (!h) return NBH_E_INVALID;
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t n = h->n;
+    State S = h->st[h->cur].view();
+    std::vector<uint32_t> ids(n);
+    std::vector<double> buf(n);
+    HIPCHK(h, hipMemcpyAsync(ids.data(), S.id, n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const double* src[8] = {S.x, S.y, S.z, S.vx, S.vy, S.vz, S.mass, S.charge};
+    for (int k = 0; k < 8; ++k) {
+        double* dst = k < 3 ? pos : (k < 6 ? vel : (k == 6 ? mass : charge));
+        if (!dst) continue;
+        HIPCHK(h, hipMemcpyAsync(buf.data(), src[k], n * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (k < 6)
+            for (size_t i = 0; i < n; ++i) dst[3 * (size_t)ids[i] + (k % 3)] = buf[i];
+        else
+            for (size_t i = 0; i < n; ++i) dst[ids[i]] = buf[i];
+    }
+    if (t) *t = h->time;
+    return NBH_OK;
+}
+
+int nbh_set_state(nbh_handle* h, const double* pos, const double* vel, double t) {
+    if (!h || !pos || !vel) return h ? h->fail(NBH_E_INVALID, "pos and vel are required") : NBH_E_INVALID;
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t n = h->n;
+    for (size_t i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(pos[i]) || !std::isfinite(vel[i])) return h->fail(NBH_E_NONFINITE, "non-finite state");
+    State S = h->st[h->cur].view();
+    // masses/charges keep their current (sorted) slots: bring them to input order first
+    std::vector<uint32_t> ids(n);
+    std::vector<double> m(n), q(n), mi(n), qi(n), buf(n);
+    HIPCHK(h, hipMemcpyAsync(ids.data(), S.id, n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(m.data(), S.mass, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(q.data(), S.charge, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; ++i) { mi[ids[i]] = m[i]; qi[ids[i]] = q[i]; }
+    double* dst[6] = {S.x, S.y, S.z, S.vx, S.vy, S.vz};
+    for (int k = 0; k < 6; ++k) {
+        const double* srcp = k < 3 ? pos : vel;
+        for (size_t i = 0; i < n; ++i) buf[i] = srcp[3 * i + (k % 3)];
+        HIPCHK(h, hipMemcpyAsync(dst[k], buf.data(), n * 8, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    HIPCHK(h, hipMemcpyAsync(S.mass, mi.data(), n * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(S.charge, qi.data(), n * 8, hipMemcpyHostToDevice, h->stream));
+    for (size_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
+    HIPCHK(h, hipMemcpyAsync(S.id, ids.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->time = t;
+    h->acc_valid = false;
+    return NBH_OK;
+}
+
+int nbh_compute_forces(nbh_handle* h, float* acc, float* pot) {
+    if (!h) return NBH_E_INVALID;
+    HIPCHK(h, hipSetDevice(h->device));
+    CHK(force_eval(h, 0.0));
+    h->acc_valid = true;
+    const size_t n = h->n;
+    State S = h->st[h->cur].view();
+    std::vector<uint32_t> ids(n);
+    std::vector<float> a(n);
+    HIPCHK(h, hipMemcpyAsync(ids.data(), S.id, n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (acc) {
+        float* src[3] = {S.ax, S.ay, S.az};
+        for (int k = 0; k < 3; ++k) {
+            HIPCHK(h, hipMemcpyAsync(a.data(), src[k], n * 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            for (size_t i = 0; i < n; ++i) acc[3 * (size_t)ids[i] + k] = a[i];
+        }
+    }
+    if (pot) {
+        HIPCHK(h, hipMemcpyAsync(a.data(), S.pot, n * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (size_t i = 0; i < n; ++i) pot[ids[i]] = a[i];
+    }
+    return NBH_OK;
+}
+
+int nbh_direct_fp64(nbh_handle* h, size_t nt, const int64_t* targets, double* acc, double* pot) {
+    if (!h) return NBH_E_INVALID;
+    if (!targets || !acc || !pot) return h->fail(NBH_E_INVALID, "targets, acc and pot are required");
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t n = h->n;
+    if (!nt) return NBH_OK;
+    State S = h->st[h->cur].view();
+    std::vector<uint32_t> ids(n), slot(n), tg(nt);
+    HIPCHK(h, hipMemcpyAsync(ids.data(), S.id, n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; ++i) slot[ids[i]] = (uint32_t)i;
+    for (size_t k = 0; k < nt; ++k) {
+        if (targets[k] < 0 || (size_t)targets[k] >= n) return h->fail(NBH_E_INVALID, "target index out of range");
+        tg[k] = slot[targets[k]];
+    }
+    // the box is needed only by direct_f32; fp64 uses raw positions
+    DevBuf dt, da, dp;
+    HIPCHK(h, dt.ensure(nt * 4));
+    HIPCHK(h, da.ensure(nt * 24));
+    HIPCHK(h, dp.ensure(nt * 8));
+    HIPCHK(h, hipMemcpyAsync(dt.p, tg.data(), nt * 4, hipMemcpyHostToDevice, h->stream));
+    launch_direct_fp64(S, h->sys_off.as<uint32_t>(), h->nsys, dt.as<uint32_t>(), (uint32_t)nt, h->cfg.G,
+                       h->cfg.eps * h->cfg.eps, da.as<double>(), dp.as<double>(), h->stream);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(acc, da.p, nt * 24, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(pot, dp.p, nt * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dt.release(); da.release(); dp.release();
+    return NBH_OK;
+}
+
+int nbh_diagnostics(nbh_handle* h, double* out) {
+    if (!h || !out) return h ? h->fail(NBH_E_INVALID, "out is NULL") : NBH_E_INVALID;
+    HIPCHK(h, hipSetDevice(h->device));
+    const uint32_t nb = diagnostics_blocks((uint32_t)h->n);
+    launch_diagnostics(h->st[h->cur].view(), (uint32_t)h->n, h->diag.as<double>(), nb, h->stream);
+    std::vector<double> part((size_t)nb * 8);
+    HIPCHK(h, hipMemcpyAsync(part.data(), h->diag.p, part.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 8; ++k) {
+        double s = 0;
+        for (uint32_t b = 0; b < nb; ++b) s += part[(size_t)b * 8 + k];
+        out[k] = s;
+    }
+    return NBH_OK;
+}
+
+int nbh_get_stats(nbh_handle* h, nbh_stats* s) {
+    if (!h || !s) return NBH_E_INVALID;
+    *s = h->stats;
+    return NBH_OK;
+}
+
+int nbh_set_timing(nbh_handle* h, int enable) {
+    if (!h) return NBH_E_INVALID;
+    h->timing = enable != 0;
+    return NBH_OK;
+}
+
+int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
+    if (!h) return NBH_E_INVALID;
+    (void)w; (void)gx; (void)gm;
+    return h->fail(NBH_E_INVALID, "nbh_force_vjp is not available in this build");
+}
+
+int nbh_debug_sizes(nbh_handle* h, int64_t* sizes) {
+    if (!h || !sizes) return NBH_E_INVALID;
+    sizes[0] = h->num_nodes;
+    sizes[1] = h->num_leaves;
+    sizes[2] = h->depth;
+    sizes[3] = (int64_t)h->stats.m2l_pairs;
+    sizes[4] = (int64_t)h->stats.p2p_pairs;
+    sizes[5] = h->nsys;
+    return NBH_OK;
+}
+
+int nbh_debug_keys(nbh_handle* h, uint64_t* keys, uint32_t* ids, double* box) {
+    if (!h) return NBH_E_INVALID;
+    HIPCHK(h, hipSetDevice(h->device));
+    if (keys) HIPCHK(h, hipMemcpyAsync(keys, h->keys.p, h->n * 8, hipMemcpyDeviceToHost, h->stream));
+    if (ids) HIPCHK(h, hipMemcpyAsync(ids, h->st[h->cur].id.p, h->n * 4, hipMemcpyDeviceToHost, h->stream));
+    if (box) HIPCHK(h, hipMemcpyAsync(box, h->box.p, h->nsys * 32, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return NBH_OK;
+}
+
+int nbh_debug_tree(nbh_handle* h, int32_t* level, int32_t* sys, int32_t* ix, int32_t* iy, int32_t* iz,
+                   int32_t* begin, int32_t* count, int32_t* first_child, int32_t* nchild, int32_t* parent) {
+    if (!h) return NBH_E_INVALID;
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t nn = h->num_nodes;
+    std::vector<int4> g(nn), l(nn);
+    std::vector<int> s(nn);
+    HIPCHK(h, hipMemcpyAsync(g.data(), h->geo.p, nn * 16, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(l.data(), h->link.p, nn * 16, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(s.data(), h->nsys_of.p, nn * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < nn; ++i) {
+        if (level) level[i] = g[i].w & 0xff;
+        if (nchild) nchild[i] = g[i].w >> 8;
+        if (ix) ix[i] = g[i].x;
+        if (iy) iy[i] = g[i].y;
+        if (iz) iz[i] = g[i].z;
+        if (sys) sys[i] = s[i];
+        if (begin) begin[i] = l[i].x;
+        if (count) count[i] = l[i].y;
+        if (first_child) first_child[i] = l[i].z;
+        if (parent) parent[i] = l[i].w;
+    }
+    return NBH_OK;
+}
+
+int nbh_debug_lists(nbh_handle* h, int64_t* m2l_off, int32_t* m2l_src, int64_t* p2p_off, int32_t* p2p_src) {
+    if (!h) return NBH_E_INVALID;
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t nn = h->num_nodes;
+    if (m2l_off) HIPCHK(h, hipMemcpyAsync(m2l_off, h->m2l_off.p, (nn + 1) * 8, hipMemcpyDeviceToHost, h->stream));
+    if (p2p_off) HIPCHK(h, hipMemcpyAsync(p2p_off, h->p2p_off.p, (nn + 1) * 8, hipMemcpyDeviceToHost, h->stream));
+    if (m2l_src && h->stats.m2l_pairs)
+        HIPCHK(h, hipMemcpyAsync(m2l_src, h->m2l_src.p, h->stats.m2l_pairs * 4, hipMemcpyDeviceToHost, h->stream));
+    if (p2p_src && h->stats.p2p_pairs)
+        HIPCHK(h, hipMemcpyAsync(p2p_src, h->p2p_src.p, h->stats.p2p_pairs * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return NBH_OK;
+}
+
+int nbh_debug_expansions(nbh_handle* h, float* M, float* L) {
+    if (!h) return NBH_E_INVALID;
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t nn = h->num_nodes;
+    const int nc = ncoef((int)h->cfg.order), ncp = ncp_of((int)h->cfg.order);
+    std::vector<float> buf(nn * ncp);
+    float* outs[2] = {M, L};
+    DevBuf* srcs[2] = {&h->M, &h->L};
+    for (int k = 0; k < 2; ++k) {
+        if (!outs[k]) continue;
+        HIPCHK(h, hipMemcpyAsync(buf.data(), srcs[k]->p, buf.size() * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (size_t i = 0; i < nn; ++i)
+            for (int c = 0; c < nc; ++c) outs[k][i * nc + c] = buf[i * ncp + c];
+    }
+    return NBH_OK;
+}
+
+}  // extern "C"
