@@ -1,0 +1,158 @@
+"""Benchmark: FMM particle-steps/s on a 1,048,576-particle Plummer sphere (one MI355X).
+
+Workload (one "step" = one KDK leapfrog step = tree rebuild + moments + walk +
+M2L/L2L + P2P/L2P + integration): N = 1,048,576 Plummer (seed 1, a = 1, M = 1,
+G = 1, truncated at r = 20a), eps = 1e-3, theta = 0.5, leaf capacity 32,
+Cartesian order p = 4, FP32 force kernels, FP64 positions and velocities,
+dt = 1e-3.  One force evaluation is checked against the FP64 direct sum on
+4096 random targets; then W warm-up steps and K timed steps.
+
+Prints one JSON line.  The roofline entry is for the dominant kernel (P2P),
+timed with HIP events on the library's stream inside the timed region.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector and matrix), AMD published
+HBM_PEAK_GBS = 8000.0
+P2P_FLOP = 20              # per directed particle-particle interaction
+M2L_FLOP = {1: 2 * 10, 2: 2 * 28, 3: 2 * 84, 4: 420, 5: 2 * 462, 6: 1848}  # 2 * C(p+6, 6)
+
+
+def cpu_baseline(sample_n, budget_s):
+    """Reference NaiveSimulation (oracle/_ref, compiled from the reference's
+    sources) timed on this host, 1 core; falls back to the oracle's FP32
+    restatement of the same loop when the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    rng = np.random.default_rng(123)
+    n = sample_n
+    P = np.zeros((n, 12), dtype=np.float32)
+    P[:, 0:3] = rng.random((n, 3))
+    P[:, 8] = 1.0 / n
+    P[:, 9] = 1.0 / n
+    kind = "reference" if po.ref_lib() is not None else "port"
+    run = po.naive_reference if kind == "reference" else po.naive_compat
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        run(P, 1.0, 1e-3, 1)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s * 0.5 or steps >= 3:
+            break
+    dt = (time.perf_counter() - t0) / steps
+    pairs_per_s = n * (n - 1) / 2 / dt
+    target_n = 1 << 20
+    value = 2.0 * pairs_per_s / (target_n - 1)
+    return {"value": value, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+            "sample": f"NaiveSimulation::step at N={n} uniform, {steps} step(s) of {dt:.2f} s "
+                      f"({pairs_per_s:.3g} pairs/s, -O2), extrapolated to N=1,048,576 as 2*pairs_per_s/(N-1)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--m2l", default="plain", choices=["plain", "grouped", "bf16x3"])
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import nbody_amd as nb
+
+    if nb.device_count() < 1:
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    m2l = {"plain": nb.M2L_PLAIN, "grouped": nb.M2L_GROUPED, "bf16x3": nb.M2L_BF16X3}[args.m2l]
+    n = args.n
+    p = 4
+    x, v, m = nb.ics.plummer(n, seed=1, a=1.0, M=1.0, G=1.0, rmax=20.0)
+    cfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, dt=1e-3, leaf_capacity=32, order=p, m2l=m2l)
+    sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
+
+    # one force evaluation, checked against the FP64 direct sum
+    err = {}
+    if not args.no_error_check:
+        acc, _ = sim.forces()
+        tg = np.random.default_rng(4096).choice(n, size=min(4096, n), replace=False)
+        ad, _ = sim.direct_fp64(tg)
+        d = np.linalg.norm(acc[tg].astype(np.float64) - ad, axis=1)
+        ref = np.linalg.norm(ad, axis=1)
+        err = {"force_rms_rel_err": float(np.sqrt(np.mean(d ** 2) / np.mean(ref ** 2))),
+               "force_max_rel_err": float(np.max(d / ref)),
+               "force_err_targets": int(len(tg))}
+
+    for _ in range(args.warmup):
+        sim.step(1)
+    sim.set_timing(True)
+    agg = {k: 0.0 for k in ["ms_tree", "ms_moments", "ms_walk", "ms_m2l", "ms_l2l", "ms_p2p", "ms_total"]}
+    inter = 0
+    m2l_pairs = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step(1)
+        st = sim.stats()
+        for k in agg:
+            agg[k] += st[k]
+        inter += st["p2p_interactions"]
+        m2l_pairs += st["m2l_pairs"]
+    wall = time.perf_counter() - t0
+    st = sim.stats()
+    sim.close()
+
+    K = args.steps
+    value = n * K / wall
+    p2p_ms = agg["ms_p2p"] / K
+    p2p_flop = P2P_FLOP * inter / K
+    achieved = p2p_flop / (p2p_ms * 1e-3) / 1e12
+    m2l_ms = agg["ms_m2l"] / K
+    m2l_tf = M2L_FLOP[p] * m2l_pairs / K / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
+    out = {
+        "metric": "particle-steps/s, N=1M Plummer, force rms err<=1e-4 vs FP64 direct; %roofline",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"plummer_N{n}_seed1_eps1e-3_theta0.5_cap32_p{p}_kdk_dt1e-3",
+                   "n": n, "order": p, "theta": 0.5, "leaf_capacity": 32, "eps": 1e-3, "m2l": args.m2l,
+                   "state_dtype": "fp64"},
+        "roofline": {"bound": "mfma", "engine": "fp32-valu (P2P kernel)", "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": None, "kernel": "p2p_kernel<4>", "ms_per_launch": p2p_ms,
+                     "flop_per_launch": p2p_flop},
+        "phases_ms": {k[3:]: agg[k] / K for k in agg},
+        "m2l": {"pairs_per_step": m2l_pairs / K, "tflops": m2l_tf},
+        "tree": {"nodes": st["num_nodes"], "leaves": st["num_leaves"], "depth": st["depth"]},
+    }
+    out.update(err)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

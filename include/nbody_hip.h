@@ -46,6 +46,10 @@ extern "C" {
 #define NBH_M2L_GROUPED 1      /* FP32, grouped by translation offset */
 #define NBH_M2L_BF16X3 2       /* grouped, MFMA bf16x3 split products, FP32 accumulate */
 
+/* near-field (P2P) kernel */
+#define NBH_P2P_LEAF 0         /* one wavefront per target leaf (plain) */
+#define NBH_P2P_LANE 1         /* one lane per target particle, 64 consecutive particles per wavefront */
+
 /* integrator */
 #define NBH_INT_KDK 0          /* kick-drift-kick leapfrog, FP64 state */
 #define NBH_INT_EULER 1        /* explicit Euler as open_cl_simulation.cpp:600-607 */
@@ -74,6 +78,8 @@ typedef struct nbh_config {
     uint32_t fixed_root;    /* 0: root cube = bounding cube each step; 1: [root_lo, root_lo+root_size) */
     double root_lo[3];
     double root_size;
+    uint32_t p2p;           /* NBH_P2P_* */
+    uint32_t reserved;
 } nbh_config;
 
 typedef struct nbh_handle nbh_handle;
@@ -92,7 +98,7 @@ int nbh_abi_version(void);
 size_t nbh_config_size(void);
 
 /* Fills the reference's defaults: G=1 (FORCE_CONSTANT=-1), eps=0.01,
- * theta=0.5, leaf capacity 8, p=4, KDK, FMM, plain M2L. */
+ * theta=0.5, leaf capacity 8, p=4, KDK, FMM, grouped M2L, per-lane P2P. */
 int nbh_config_default(nbh_config* cfg);
 
 /* Number of visible HIP devices (0 without a GPU; never fails). */

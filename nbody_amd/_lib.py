@@ -20,6 +20,7 @@ NBH_E_INTERNAL = -5
 
 SOLVER_FMM, SOLVER_DIRECT = 0, 1
 M2L_PLAIN, M2L_GROUPED, M2L_BF16X3 = 0, 1, 2
+P2P_LEAF, P2P_LANE = 0, 1
 INT_KDK, INT_EULER, INT_KICK_DRIFT = 0, 1, 2
 LAW_SOFTENED, LAW_COMPAT_NAIVE = 0, 1
 
@@ -42,6 +43,8 @@ class Config(ctypes.Structure):
         ("fixed_root", ctypes.c_uint32),
         ("root_lo", ctypes.c_double * 3),
         ("root_size", ctypes.c_double),
+        ("p2p", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

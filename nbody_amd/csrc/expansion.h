@@ -23,6 +23,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 namespace nbh {
 
 constexpr int ncoef(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
@@ -97,22 +99,23 @@ struct Tables {
                 }
             }
         }
+        // ordered by source coefficient n (outer) so that M2L consumes each
+        // M_n in one run and L2L each mono_n likewise
         int t = 0;
-        for (int a = 0; a < NC; ++a)
-            for (int b = 0; b < NC; ++b) {
+        for (int b = 0; b < NC; ++b)
+            for (int a = 0; a < NC; ++a) {
                 int s = at(mx[a] + mx[b], my[a] + my[b], mz[a] + mz[b]);
                 if (s >= 0) { pk[t] = a; pn[t] = b; pkn[t] = s; ++t; }
             }
         int u = 0;
-        for (int a = 0; a < NC; ++a)
-            for (int x = 0; x <= mx[a]; ++x)
-                for (int y = 0; y <= my[a]; ++y)
-                    for (int z = 0; z <= mz[a]; ++z) {
-                        qn[u] = a;
-                        qm[u] = at(x, y, z);
-                        qr[u] = at(mx[a] - x, my[a] - y, mz[a] - z);
-                        ++u;
-                    }
+        for (int m = 0; m < NC; ++m)
+            for (int a = 0; a < NC; ++a) {
+                if (mx[m] > mx[a] || my[m] > my[a] || mz[m] > mz[a]) continue;
+                qn[u] = a;
+                qm[u] = m;
+                qr[u] = at(mx[a] - mx[m], my[a] - my[m], mz[a] - mz[m]);
+                ++u;
+            }
     }
 };
 
@@ -121,86 +124,135 @@ struct Tab {
     static constexpr Tables<P> t{};
 };
 
-// b^n / n! for all n (unrolled; NC multiplies)
+// ---------------------------------------------------------------------------
+// Straight-line code generation: each table entry becomes a constexpr
+// template argument via std::index_sequence fold expressions, so every
+// coefficient array is indexed by constants and lives in registers.
+
+template <int P, int K>
+__device__ __forceinline__ void mono_step(float bx, float by, float bz, float* mono) {
+    constexpr auto& T = Tab<P>::t;
+    constexpr int ax = T.maxis[K];
+    constexpr int e = ax == 0 ? T.mx[K] : (ax == 1 ? T.my[K] : T.mz[K]);
+    constexpr int par = T.mpar[K];
+    constexpr float inv = 1.0f / float(e);
+    const float b = ax == 0 ? bx : (ax == 1 ? by : bz);
+    mono[K] = mono[par] * (b * inv);
+}
+template <int P, size_t... I>
+__device__ __forceinline__ void mono_all(float bx, float by, float bz, float* mono, std::index_sequence<I...>) {
+    (mono_step<P, (int)I + 1>(bx, by, bz, mono), ...);
+}
+// b^n / n! for all n (NC-1 multiplies)
 template <int P>
 __device__ __forceinline__ void monomials(float bx, float by, float bz, float* mono) {
-    constexpr auto& T = Tab<P>::t;
     mono[0] = 1.0f;
-#pragma unroll
-    for (int k = 1; k < T.NC; ++k) {
-        const int ax = T.maxis[k];
-        const float b = ax == 0 ? bx : (ax == 1 ? by : bz);
-        const int e = ax == 0 ? T.mx[k] : (ax == 1 ? T.my[k] : T.mz[k]);
-        mono[k] = mono[T.mpar[k]] * (b * (1.0f / float(e)));
-    }
+    mono_all<P>(bx, by, bz, mono, std::make_index_sequence<ncoef(P) - 1>{});
 }
 
-template <int P>
-__device__ __forceinline__ void derivs(float Rx, float Ry, float Rz, float eps2, float* D) {
+template <int P, int K, typename T_>
+__device__ __forceinline__ void deriv_step(const T_* R, T_ inv_s, T_* D) {
     constexpr auto& T = Tab<P>::t;
-    const float s = Rx * Rx + Ry * Ry + Rz * Rz + eps2;
-    const float inv_s = 1.0f / s;
-    D[0] = rsqrtf(s);
-    const float R[3] = {Rx, Ry, Rz};
-#pragma unroll
-    for (int k = 1; k < T.NC; ++k) {
-        float acc = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            if (T.d1[k][j] >= 0) acc += T.c1[k][j] * R[j] * D[T.d1[k][j]];
-            if (T.d2[k][j] >= 0 && T.c2[k][j] != 0.0f) acc += T.c2[k][j] * D[T.d2[k][j]];
-        }
-        D[k] = -acc * inv_s;
-    }
+    T_ acc = T_(0);
+    if constexpr (T.d1[K][0] >= 0) acc += (T_(T.c1[K][0]) * R[0]) * D[T.d1[K][0] < 0 ? 0 : T.d1[K][0]];
+    if constexpr (T.d1[K][1] >= 0) acc += (T_(T.c1[K][1]) * R[1]) * D[T.d1[K][1] < 0 ? 0 : T.d1[K][1]];
+    if constexpr (T.d1[K][2] >= 0) acc += (T_(T.c1[K][2]) * R[2]) * D[T.d1[K][2] < 0 ? 0 : T.d1[K][2]];
+    if constexpr (T.d2[K][0] >= 0 && T.c2[K][0] != 0.0f) acc += T_(T.c2[K][0]) * D[T.d2[K][0] < 0 ? 0 : T.d2[K][0]];
+    if constexpr (T.d2[K][1] >= 0 && T.c2[K][1] != 0.0f) acc += T_(T.c2[K][1]) * D[T.d2[K][1] < 0 ? 0 : T.d2[K][1]];
+    if constexpr (T.d2[K][2] >= 0 && T.c2[K][2] != 0.0f) acc += T_(T.c2[K][2]) * D[T.d2[K][2] < 0 ? 0 : T.d2[K][2]];
+    D[K] = -acc * inv_s;
+}
+template <int P, typename T_, size_t... I>
+__device__ __forceinline__ void deriv_all(const T_* R, T_ inv_s, T_* D, std::index_sequence<I...>) {
+    (deriv_step<P, (int)I + 1, T_>(R, inv_s, D), ...);
+}
+// D_n(R), |n| <= P, of (|R|^2 + eps^2)^(-1/2); T_ = float or double
+template <int P, typename T_ = float>
+__device__ __forceinline__ void derivs(T_ Rx, T_ Ry, T_ Rz, T_ eps2, T_* D) {
+    const T_ s = Rx * Rx + Ry * Ry + Rz * Rz + eps2;
+    const T_ inv_s = T_(1) / s;
+    if constexpr (sizeof(T_) == 4) D[0] = rsqrtf(s);
+    else D[0] = T_(1) / sqrt(s);
+    const T_ R[3] = {Rx, Ry, Rz};
+    deriv_all<P, T_>(R, inv_s, D, std::make_index_sequence<ncoef(P) - 1>{});
 }
 
+template <int P, int I>
+__device__ __forceinline__ void m2l_step(const float* D, const float* M, float* L) {
+    constexpr auto& T = Tab<P>::t;
+    constexpr int k = T.pk[I], n = T.pn[I], kn = T.pkn[I];
+    if constexpr (T.deg[n] & 1) L[k] = fmaf(-D[kn], M[n], L[k]);
+    else L[k] = fmaf(D[kn], M[n], L[k]);
+}
+template <int P, size_t... I>
+__device__ __forceinline__ void m2l_all(const float* D, const float* M, float* L, std::index_sequence<I...>) {
+    (m2l_step<P, (int)I>(D, M, L), ...);
+}
 template <int P>
 __device__ __forceinline__ void m2l_apply(const float* D, const float* M, float* L) {
-    constexpr auto& T = Tab<P>::t;
-#pragma unroll
-    for (int t = 0; t < T.NP; ++t) {
-        const float v = D[T.pkn[t]] * M[T.pn[t]];
-        if (T.deg[T.pn[t]] & 1) L[T.pk[t]] -= v;
-        else L[T.pk[t]] += v;
-    }
+    m2l_all<P>(D, M, L, std::make_index_sequence<npairs(P)>{});
 }
 
+template <int P, int I>
+__device__ __forceinline__ void m2m_step(const float* mono, const float* Mc, float* Mp) {
+    constexpr auto& T = Tab<P>::t;
+    constexpr int n = T.qn[I], m = T.qm[I], r = T.qr[I];
+    Mp[n] = fmaf(Mc[m], mono[r], Mp[n]);
+}
+template <int P, size_t... I>
+__device__ __forceinline__ void m2m_all(const float* mono, const float* Mc, float* Mp, std::index_sequence<I...>) {
+    (m2m_step<P, (int)I>(mono, Mc, Mp), ...);
+}
 template <int P>
 __device__ __forceinline__ void m2m_apply(float dx, float dy, float dz, const float* Mc, float* Mp) {
-    constexpr auto& T = Tab<P>::t;
-    float mono[T.NC];
+    float mono[ncoef(P)];
     monomials<P>(dx, dy, dz, mono);
-#pragma unroll
-    for (int u = 0; u < T.NM; ++u) Mp[T.qn[u]] += Mc[T.qm[u]] * mono[T.qr[u]];
+    m2m_all<P>(mono, Mc, Mp, std::make_index_sequence<Tables<P>::NM>{});
 }
 
+template <int P, int I>
+__device__ __forceinline__ void l2l_step(const float* mono, const float* Lp, float* Lc) {
+    constexpr auto& T = Tab<P>::t;
+    constexpr int k = T.pk[I], n = T.pn[I], kn = T.pkn[I];
+    Lc[k] = fmaf(Lp[kn], mono[n], Lc[k]);
+}
+template <int P, size_t... I>
+__device__ __forceinline__ void l2l_all(const float* mono, const float* Lp, float* Lc, std::index_sequence<I...>) {
+    (l2l_step<P, (int)I>(mono, Lp, Lc), ...);
+}
 template <int P>
 __device__ __forceinline__ void l2l_apply(float dx, float dy, float dz, const float* Lp, float* Lc) {
-    constexpr auto& T = Tab<P>::t;
-    float mono[T.NC];
+    float mono[ncoef(P)];
     monomials<P>(dx, dy, dz, mono);
-    // Lc[j] += sum_m Lp[j+m] mono[m]  (pair table with k=j, n=m)
-#pragma unroll
-    for (int t = 0; t < T.NP; ++t) Lc[T.pk[t]] += Lp[T.pkn[t]] * mono[T.pn[t]];
+    l2l_all<P>(mono, Lp, Lc, std::make_index_sequence<npairs(P)>{});
 }
 
+template <int P, int K>
+__device__ __forceinline__ void l2p_step(const float* mono, const float* L, float& p, float& x, float& y, float& z) {
+    constexpr auto& T = Tab<P>::t;
+    p = fmaf(L[K], mono[K], p);
+    if constexpr (T.deg[K] < P) {
+        constexpr int ix = T.at(T.mx[K] + 1, T.my[K], T.mz[K]);
+        constexpr int iy = T.at(T.mx[K], T.my[K] + 1, T.mz[K]);
+        constexpr int iz = T.at(T.mx[K], T.my[K], T.mz[K] + 1);
+        x = fmaf(L[ix], mono[K], x);
+        y = fmaf(L[iy], mono[K], y);
+        z = fmaf(L[iz], mono[K], z);
+    }
+}
+template <int P, size_t... I>
+__device__ __forceinline__ void l2p_all(const float* mono, const float* L, float& p, float& x, float& y, float& z,
+                                        std::index_sequence<I...>) {
+    (l2p_step<P, (int)I>(mono, L, p, x, y, z), ...);
+}
 // phi = sum_k L_k a^k/k!, g = grad (sum over |k| <= P-1 of L_{k+e} a^k/k!)
 template <int P>
 __device__ __forceinline__ void l2p_eval(float ax, float ay, float az, const float* L, float& phi,
                                          float& gx, float& gy, float& gz) {
-    constexpr auto& T = Tab<P>::t;
-    float mono[T.NC];
+    float mono[ncoef(P)];
     monomials<P>(ax, ay, az, mono);
     float p = 0.f, x = 0.f, y = 0.f, z = 0.f;
-#pragma unroll
-    for (int k = 0; k < T.NC; ++k) {
-        p += L[k] * mono[k];
-        if (T.deg[k] < P) {
-            x += L[T.at(T.mx[k] + 1, T.my[k], T.mz[k])] * mono[k];
-            y += L[T.at(T.mx[k], T.my[k] + 1, T.mz[k])] * mono[k];
-            z += L[T.at(T.mx[k], T.my[k], T.mz[k] + 1)] * mono[k];
-        }
-    }
+    l2p_all<P>(mono, L, p, x, y, z, std::make_index_sequence<ncoef(P)>{});
     phi = p; gx = x; gy = y; gz = z;
 }
 

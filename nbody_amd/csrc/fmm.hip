@@ -11,6 +11,8 @@
 // cell; centre differences between cells are exact integers (units of half
 // the finest cell) scaled once to FP32, so FP32 separations keep their
 // relative precision however large the root box is.
+#include <algorithm>
+
 #include "common.h"
 #include "expansion.h"
 #include "launch.h"
@@ -74,6 +76,7 @@ __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t
         const float bx = (float)(S.x[i] - c[0]), by = (float)(S.y[i] - c[1]), bz = (float)(S.z[i] - c[2]);
         const float q = (float)S.charge[i];
         F.pf[i] = make_float4(bx, by, bz, q);
+        F.pleaf[i] = A;
         float mono[NC];
         monomials<P>(bx, by, bz, mono);
 #pragma unroll
@@ -92,7 +95,8 @@ __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t
 // thread per cell of the level; children's moments shifted to the parent
 // centre.  Levels run bottom-up.
 template <int P>
-__global__ void m2m_kernel(FmmParams F, uint32_t count, uint32_t off) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2m_kernel(FmmParams F, uint32_t count,
+                                                                                           uint32_t off) {
     constexpr int NC = ncoef(P);
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
@@ -109,11 +113,7 @@ __global__ void m2m_kernel(FmmParams F, uint32_t count, uint32_t off) {
         const int ch = lk.z + c;
         const int4 gc = F.nodes.geo[ch];
         const float3 d = centre_diff(g, gc, F.bits, unit);  // c_child - c_parent
-        float Mc[NC];
-        const float* src = F.M + (size_t)ch * F.ncp;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) Mc[k] = src[k];
-        m2m_apply<P>(d.x, d.y, d.z, Mc, Mp);
+        m2m_apply<P>(d.x, d.y, d.z, F.M + (size_t)ch * F.ncp, Mp);
     }
     float* out = F.M + (size_t)A * F.ncp;
 #pragma unroll
@@ -123,7 +123,8 @@ __global__ void m2m_kernel(FmmParams F, uint32_t count, uint32_t off) {
 // M2L, plain: one thread per target cell, the derivative tensor rebuilt per
 // pair (replaces field.cl:153-211's order-0 cell field).
 template <int P>
-__global__ __launch_bounds__(256) void m2l_plain_kernel(FmmParams F, uint32_t num_nodes) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_plain_kernel(FmmParams F,
+                                                                                                  uint32_t num_nodes) {
     constexpr int NC = ncoef(P);
     const uint32_t A = blockIdx.x * blockDim.x + threadIdx.x;
     if (A >= num_nodes) return;
@@ -139,17 +140,9 @@ __global__ __launch_bounds__(256) void m2l_plain_kernel(FmmParams F, uint32_t nu
         const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
         float D[NC];
         derivs<P>(d.x, d.y, d.z, F.eps2, D);
-        float M[NC];
-        const float4* src = reinterpret_cast<const float4*>(F.M + (size_t)B * F.ncp);
-#pragma unroll
-        for (int k = 0; k < (NC + 3) / 4; ++k) {
-            const float4 v = src[k];
-            if (4 * k + 0 < NC) M[4 * k + 0] = v.x;
-            if (4 * k + 1 < NC) M[4 * k + 1] = v.y;
-            if (4 * k + 2 < NC) M[4 * k + 2] = v.z;
-            if (4 * k + 3 < NC) M[4 * k + 3] = v.w;
-        }
-        m2l_apply<P>(D, M, L);
+        // source moments read straight from memory; the pair table is ordered
+        // by source coefficient so each M_n is loaded once and used at once
+        m2l_apply<P>(D, F.M + (size_t)B * F.ncp, L);
     }
     float* out = F.L + (size_t)A * F.ncp;
 #pragma unroll
@@ -159,7 +152,8 @@ __global__ __launch_bounds__(256) void m2l_plain_kernel(FmmParams F, uint32_t nu
 // L2L: one thread per cell of the level (top-down), adds the parent's local
 // expansion shifted to this cell's centre.
 template <int P>
-__global__ void l2l_kernel(FmmParams F, uint32_t count, uint32_t off) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void l2l_kernel(FmmParams F, uint32_t count,
+                                                                                           uint32_t off) {
     constexpr int NC = ncoef(P);
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
@@ -279,6 +273,157 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
     }
 }
 
+// ---------------------------------------------------------------------------
+// P2P, one lane per target particle: a wavefront takes 64 consecutive
+// (Morton-sorted) particles, usually several neighbouring leaves, and every
+// lane walks its own leaf's P2P list.  No lane idles for small leaves and the
+// per-source-leaf set-up (centre difference, bounds) is shared by all of that
+// leaf's particles.  Same epilogue as p2p_kernel (L2P, force, half kick).
+template <int P>
+__global__ __launch_bounds__(256) void p2p_lane_kernel(FmmParams F, State S, uint32_t n, double kick_dt) {
+    constexpr int NC = ncoef(P);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int A = F.pleaf[i];
+    const int4 ga = F.nodes.geo[A];
+    const float unit = F.unit[F.nodes.sys[A]];
+    const float eps2 = F.eps2;
+    const float4 me = F.pf[i];
+    float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+    const unsigned long long e0 = F.p2p_off[A], e1 = F.p2p_off[A + 1];
+    for (unsigned long long e = e0; e < e1; ++e) {
+        const int B = F.p2p_src[e];
+        const int4 gb = F.nodes.geo[B];
+        const int4 lb = F.nodes.link[B];
+        const float3 d = centre_diff(ga, gb, F.bits, unit);
+        const float dx = d.x - me.x, dy = d.y - me.y, dz = d.z - me.z;
+        const float4* src = F.pf + lb.x;
+        if (B != A) {
+#pragma unroll 4
+            for (int j = 0; j < lb.y; ++j) {
+                const float4 s = src[j];
+                const float rx = dx + s.x, ry = dy + s.y, rz = dz + s.z;
+                const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+                const float inv = rsqrtf(r2);
+                const float qi = s.w * inv;
+                const float qi3 = qi * inv * inv;
+                ax = fmaf(rx, qi3, ax);
+                ay = fmaf(ry, qi3, ay);
+                az = fmaf(rz, qi3, az);
+                ph += qi;
+            }
+        } else {
+            const int self = (int)i - lb.x;
+            for (int j = 0; j < lb.y; ++j) {
+                const float4 s = src[j];
+                const float rx = dx + s.x, ry = dy + s.y, rz = dz + s.z;
+                const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+                const float inv = j == self ? 0.f : rsqrtf(r2);
+                const float qi = s.w * inv;
+                const float qi3 = qi * inv * inv;
+                ax = fmaf(rx, qi3, ax);
+                ay = fmaf(ry, qi3, ay);
+                az = fmaf(rz, qi3, az);
+                ph += qi;
+            }
+        }
+    }
+    float lphi, gx, gy, gz;
+    {
+        const float* Lsrc = F.L + (size_t)A * F.ncp;
+        float Lr[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Lr[k] = Lsrc[k];
+        l2p_eval<P>(me.x, me.y, me.z, Lr, lphi, gx, gy, gz);
+    }
+    const float fac = (float)(S.charge[i] / S.mass[i]) * F.G;
+    const float axo = fac * (ax + gx), ayo = fac * (ay + gy), azo = fac * (az + gz);
+    S.ax[i] = axo;
+    S.ay[i] = ayo;
+    S.az[i] = azo;
+    S.pot[i] = -F.G * (ph + lphi);
+    if (kick_dt != 0.0) {
+        S.vx[i] += (double)axo * kick_dt;
+        S.vy[i] += (double)ayo * kick_dt;
+        S.vz[i] += (double)azo * kick_dt;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// M2L grouped by translation offset.  Same-level pairs of one system share
+// their derivative tensor whenever they share the offset o = i_B - i_A
+// (cells at level l, R = -o h_l): the tensor of every offset in [-K, K]^3 is
+// built once per level and step (FP64, stored FP32) and the M2L kernel only
+// contracts it.  Cross-level pairs (a leaf kept against a finer or coarser
+// cell) build their tensor per pair.  One wavefront per target cell; lanes
+// stride its source list and meet in a wave reduction.
+template <int P>
+__global__ void dtable_kernel(FmmParams F, int level0, int nlev, int K) {
+    constexpr int NC = ncoef(P);
+    const int side = 2 * K + 1;
+    const int per = side * side * side;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * nlev) return;
+    const int lv = t / per, o = t % per;
+    const int ox = o / (side * side) - K, oy = (o / side) % side - K, oz = o % side - K;
+    const double h = ldexp(F.box[0].w, -(level0 + lv));
+    double D[NC];
+    derivs<P, double>(-ox * h, -oy * h, -oz * h, (double)F.eps2_64, D);
+    float* out = F.dtab + (size_t)t * F.ncp;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) out[k] = (float)D[k];
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void m2l_grouped_kernel(FmmParams F, uint32_t num_nodes) {
+    constexpr int NC = ncoef(P);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / WAVE;
+    const int lane = lane_id();
+    const int K = F.dK, side = 2 * K + 1;
+    for (uint32_t A = wid; A < num_nodes; A += nw) {
+        const int4 ga = F.nodes.geo[A];
+        const int la = ga.w & 0xff;
+        const float unit = F.unit[F.nodes.sys[A]];
+        float L[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = 0.f;
+        const unsigned long long e0 = F.m2l_off[A], e1 = F.m2l_off[A + 1];
+        for (unsigned long long e = e0 + lane; e < e1; e += WAVE) {
+            const int B = F.m2l_src[e];
+            const int4 gb = F.nodes.geo[B];
+            const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
+            float D[NC];
+            const bool tab = F.dtab && (gb.w & 0xff) == la && la >= F.dlevel0 && la < F.dlevel0 + F.dnlev &&
+                             abs(ox) <= K && abs(oy) <= K && abs(oz) <= K;
+            if (tab) {
+                const int o = ((ox + K) * side + (oy + K)) * side + (oz + K);
+                const float4* src = reinterpret_cast<const float4*>(
+                    F.dtab + ((size_t)(la - F.dlevel0) * side * side * side + o) * F.ncp);
+#pragma unroll
+                for (int q = 0; q < (NC + 3) / 4; ++q) {
+                    const float4 v = src[q];
+                    if (4 * q + 0 < NC) D[4 * q + 0] = v.x;
+                    if (4 * q + 1 < NC) D[4 * q + 1] = v.y;
+                    if (4 * q + 2 < NC) D[4 * q + 2] = v.z;
+                    if (4 * q + 3 < NC) D[4 * q + 3] = v.w;
+                }
+            } else {
+                const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
+                derivs<P>(d.x, d.y, d.z, F.eps2, D);
+            }
+            m2l_apply<P>(D, F.M + (size_t)B * F.ncp, L);
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
+        if (lane == 0) {
+            float* out = F.L + (size_t)A * F.ncp;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) out[k] = L[k];
+        }
+    }
+}
+
 template <int P>
 static void launch_p2m_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     hipLaunchKernelGGL((p2m_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves);
@@ -298,6 +443,22 @@ static void launch_l2l_t(const FmmParams& F, uint32_t count, uint32_t off, hipSt
 template <int P>
 static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st) {
     hipLaunchKernelGGL((p2p_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves, kick_dt);
+}
+
+template <int P>
+static void launch_p2p_lane_t(const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st) {
+    hipLaunchKernelGGL((p2p_lane_kernel<P>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, S, n, kick_dt);
+}
+template <int P>
+static void launch_dtable_t(const FmmParams& F, int level0, int nlev, int K, hipStream_t st) {
+    const int side = 2 * K + 1;
+    const unsigned total = (unsigned)(side * side * side * nlev);
+    hipLaunchKernelGGL((dtable_kernel<P>), dim3(ceil_div(total, 128)), dim3(128), 0, st, F, level0, nlev, K);
+}
+template <int P>
+static void launch_m2l_grouped_t(const FmmParams& F, uint32_t num_nodes, hipStream_t st) {
+    const unsigned grid = std::min<unsigned>(ceil_div(num_nodes, 4), 65536u);
+    hipLaunchKernelGGL((m2l_grouped_kernel<P>), dim3(grid), dim3(256), 0, st, F, num_nodes);
 }
 
 #define NBH_DISPATCH_P(p, call)          \
@@ -328,6 +489,17 @@ void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint
     (void)level;
     if (!count) return;
     NBH_DISPATCH_P(p, launch_l2l_t<PP>(F, count, off, st));
+}
+void launch_p2p_lane(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st) {
+    NBH_DISPATCH_P(p, launch_p2p_lane_t<PP>(F, S, n, kick_dt, st));
+}
+void launch_dtable(int p, const FmmParams& F, int level0, int nlev, int K, hipStream_t st) {
+    if (nlev <= 0) return;
+    NBH_DISPATCH_P(p, launch_dtable_t<PP>(F, level0, nlev, K, st));
+}
+void launch_m2l_grouped(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st) {
+    if (!num_nodes) return;
+    NBH_DISPATCH_P(p, launch_m2l_grouped_t<PP>(F, num_nodes, st));
 }
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st) {
     if (!num_leaves) return;

@@ -85,6 +85,10 @@ struct FmmParams {
     const int* m2l_src;
     const unsigned long long* p2p_off;
     const int* p2p_src;
+    uint32_t* pleaf;       // per particle: its leaf node
+    float* dtab;           // per-level derivative tensors of same-level offsets (or null)
+    int dK, dlevel0, dnlev;
+    double eps2_64;
 };
 
 void launch_gather_state(const State& src, const State& dst, const uint32_t* perm, uint32_t n, hipStream_t st);
@@ -93,6 +97,9 @@ void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint
 void launch_m2l_plain(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st);
 void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st);
+void launch_p2p_lane(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st);
+void launch_dtable(int p, const FmmParams& F, int level0, int nlev, int K, hipStream_t st);
+void launch_m2l_grouped(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st);
 
 // integrator updates: mode 0 = KDK first half (kick dt_k, drift dt_d),
 // 1 = Euler (x += v dt; v += a dt), 2 = kick-drift (v += a dt; x += v dt)

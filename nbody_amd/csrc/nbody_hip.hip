@@ -93,7 +93,7 @@ struct nbh_handle {
     int cur = 0;
     DevBuf sys_off, keys, keys_alt, perm, perm_alt, sort_tmp, tree_tmp, scan_tmp;
     DevBuf box, inv, unit, bbox_part;
-    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf;
+    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab;
     DevBuf row_node, row_src, row_sl_start, row_sl_cnt, sl, m2l_cnt, p2p_cnt, m2l_off, p2p_off, m2l_cur, p2p_cur,
         m2l_src, p2p_src;
     DevBuf counters, diag, tmp_small;
@@ -186,6 +186,7 @@ int validate_config(const nbh_config* c, std::string& why) {
         if (c->order < 1 || c->order > 6) { why = "order must be in [1, 6]"; return NBH_E_INVALID; }
         if (c->max_depth > 21) { why = "max_depth must be <= 21"; return NBH_E_INVALID; }
         if (c->m2l > NBH_M2L_BF16X3) { why = "unknown m2l kernel"; return NBH_E_INVALID; }
+        if (c->p2p > NBH_P2P_LANE) { why = "unknown p2p kernel"; return NBH_E_INVALID; }
     }
     if (c->fixed_root && !(c->root_size > 0)) { why = "fixed_root needs root_size > 0"; return NBH_E_INVALID; }
     return NBH_OK;
@@ -294,6 +295,9 @@ static FmmParams fmm_params(nbh_handle* h) {
     F.m2l_src = h->m2l_src.as<int>();
     F.p2p_off = h->p2p_off.as<unsigned long long>();
     F.p2p_src = h->p2p_src.as<int>();
+    F.pleaf = h->pleaf.as<uint32_t>();
+    F.dtab = nullptr;
+    F.eps2_64 = h->cfg.eps * h->cfg.eps;
     return F;
 }
 
@@ -407,12 +411,31 @@ static int force_fmm(nbh_handle* h, double kick_dt) {
     CHK(run_walk(h));
     record(h, 3);
     F = fmm_params(h);
-    launch_m2l_plain(p, F, h->num_nodes, h->stream);
+    if (h->cfg.m2l == NBH_M2L_PLAIN) {
+        launch_m2l_plain(p, F, h->num_nodes, h->stream);
+    } else {
+        // derivative tensors of every same-level offset |o_i| <= K, levels
+        // 2..depth (cells of levels 0 and 1 are never well separated); one
+        // system only, since the tensors depend on the root size
+        if (h->nsys == 1 && h->depth >= 2) {
+            const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
+            const int K = 2 * Kp + 1, side = 2 * K + 1;
+            const int nlev = (int)h->depth - 1;
+            HIPCHK(h, h->dtab.ensure((size_t)side * side * side * nlev * F.ncp * 4));
+            F.dtab = h->dtab.as<float>();
+            F.dK = K;
+            F.dlevel0 = 2;
+            F.dnlev = nlev;
+            launch_dtable(p, F, 2, nlev, K, h->stream);
+        }
+        launch_m2l_grouped(p, F, h->num_nodes, h->stream);
+    }
     record(h, 4);
     for (uint32_t l = 1; l <= h->depth; ++l)
         launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
     record(h, 5);
-    launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
+    if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_lane(p, F, S, (uint32_t)h->n, kick_dt, h->stream);
+    else launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
     record(h, 6);
     HIPCHK(h, hipGetLastError());
     // stats (read at the end of the evaluation; one more sync)
@@ -502,10 +525,11 @@ int nbh_config_default(nbh_config* c) {
     c->leaf_capacity = 8;    // open_cl_simulation.cpp:41-47
     c->order = 4;
     c->max_depth = 21;
-    c->m2l = NBH_M2L_PLAIN;
+    c->m2l = NBH_M2L_GROUPED;
     c->integrator = NBH_INT_KDK;
     c->force_law = NBH_LAW_SOFTENED;
     c->device = 0;
+    c->p2p = NBH_P2P_LANE;
     return NBH_OK;
 }
 
@@ -589,6 +613,7 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
     CCHK(h->bbox_part.ensure(nsys * (size_t)ceil_div(h->max_sys_n, 4096) * 6 * 8 + 64));
     CCHK(h->leaf_ids.ensure(n * 4));
     CCHK(h->pf.ensure(n * 16));
+    CCHK(h->pleaf.ensure(n * 4));
     CCHK(h->counters.ensure(sizeof(Counters)));
     CCHK(h->diag.ensure(1024 * 8 * 8));
     CCHK(h->tmp_small.ensure(64));
@@ -653,7 +678,7 @@ void nbh_destroy(nbh_handle* h) {
     h->st[1].release();
     DevBuf* all[] = {&h->sys_off, &h->keys, &h->keys_alt, &h->perm, &h->perm_alt, &h->sort_tmp, &h->tree_tmp,
                      &h->scan_tmp, &h->box, &h->inv, &h->unit, &h->bbox_part, &h->geo, &h->link, &h->nsys_of,
-                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->row_node, &h->row_src,
+                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->row_node, &h->row_src,
                      &h->row_sl_start, &h->row_sl_cnt, &h->sl, &h->m2l_cnt, &h->p2p_cnt, &h->m2l_off,
                      &h->p2p_off, &h->m2l_cur, &h->p2p_cur, &h->m2l_src, &h->p2p_src, &h->counters, &h->diag,
                      &h->tmp_small};

@@ -39,11 +39,15 @@ def _ics(kind, n, seed=0):
     return nb.ics.plummer(n, seed + 1)
 
 
+KERNELS = [("tuned", nb.M2L_GROUPED, nb.P2P_LANE), ("plain", nb.M2L_PLAIN, nb.P2P_LEAF)]
+
+
+@pytest.mark.parametrize("kern,m2l,p2p", KERNELS)
 @pytest.mark.parametrize("name,kind,n,cap,p,th,eps", CASES)
-def test_tree_lists_forces_match_oracle(name, kind, n, cap, p, th, eps):
-    _case(name)
+def test_tree_lists_forces_match_oracle(name, kind, n, cap, p, th, eps, kern, m2l, p2p):
+    _case(f"{name}/{kern}")
     x, v, m = _ics(kind, n)
-    sim = make_sim(x, v, m, leaf_capacity=cap, order=p, theta=th[0] / th[1], eps=eps)
+    sim = make_sim(x, v, m, leaf_capacity=cap, order=p, theta=th[0] / th[1], eps=eps, m2l=m2l, p2p=p2p)
     acc, pot = sim.forces()
     keys, ids, box, xs, qs = _gpu_sorted(sim, x, m)
     # keys, box and the stable particle order
@@ -142,8 +146,10 @@ def test_kdk_trajectory_c1():
     dx = np.max(np.abs(xg - xo))
     dv = rms_rel(vg, vo)
     print(f"  max |dx| {dx:.2e}, rms dv {dv:.2e}", flush=True)
-    assert dx < 2e-8
-    assert dv < 1e-3
+    # the FMM's force error (~1e-4 rms at p=4) bounds the trajectory error:
+    # displacements are ~5e-5 after 10 steps
+    assert dx < 1e-7
+    assert dv < 3e-4
     sim.close()
 
 
