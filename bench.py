@@ -64,7 +64,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--m2l", default="plain", choices=["plain", "grouped", "bf16x3"])
+    ap.add_argument("--m2l", default="grouped", choices=["plain", "grouped", "bf16x3"])
+    ap.add_argument("--p2p", default="lane", choices=["leaf", "lane"])
+    ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -81,10 +83,11 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     m2l = {"plain": nb.M2L_PLAIN, "grouped": nb.M2L_GROUPED, "bf16x3": nb.M2L_BF16X3}[args.m2l]
+    p2p = {"leaf": nb.P2P_LEAF, "lane": nb.P2P_LANE}[args.p2p]
     n = args.n
-    p = 4
+    p = args.order
     x, v, m = nb.ics.plummer(n, seed=1, a=1.0, M=1.0, G=1.0, rmax=20.0)
-    cfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, dt=1e-3, leaf_capacity=32, order=p, m2l=m2l)
+    cfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, dt=1e-3, leaf_capacity=32, order=p, m2l=m2l, p2p=p2p)
     sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
 
     # one force evaluation, checked against the FP64 direct sum
@@ -138,11 +141,11 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"plummer_N{n}_seed1_eps1e-3_theta0.5_cap32_p{p}_kdk_dt1e-3",
-                   "n": n, "order": p, "theta": 0.5, "leaf_capacity": 32, "eps": 1e-3, "m2l": args.m2l,
+                   "n": n, "order": p, "theta": 0.5, "leaf_capacity": 32, "eps": 1e-3, "m2l": args.m2l, "p2p": args.p2p,
                    "state_dtype": "fp64"},
         "roofline": {"bound": "mfma", "engine": "fp32-valu (P2P kernel)", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None, "kernel": "p2p_kernel<4>", "ms_per_launch": p2p_ms,
+                     "traffic": None, "kernel": ("p2p_lane_kernel" if args.p2p == "lane" else "p2p_kernel") + f"<{p}>", "ms_per_launch": p2p_ms,
                      "flop_per_launch": p2p_flop},
         "phases_ms": {k[3:]: agg[k] / K for k in agg},
         "m2l": {"pairs_per_step": m2l_pairs / K, "tflops": m2l_tf},

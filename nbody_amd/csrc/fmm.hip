@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t
         float* out = F.M + (size_t)A * F.ncp;
 #pragma unroll
         for (int k = 0; k < NC; ++k) out[k] = M[k];
+        if (F.leaf_lanes) F.leaf_lanes[wid] = (uint32_t)((lk.y + P2P_TPL - 1) / P2P_TPL);
     }
 }
 
@@ -274,78 +275,115 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// P2P, one lane per target particle: a wavefront takes 64 consecutive
-// (Morton-sorted) particles, usually several neighbouring leaves, and every
-// lane walks its own leaf's P2P list.  No lane idles for small leaves and the
-// per-source-leaf set-up (centre difference, bounds) is shared by all of that
-// leaf's particles.  Same epilogue as p2p_kernel (L2P, force, half kick).
+// P2P with P2P_TPL target particles of one leaf per lane: every source
+// particle loaded is used for P2P_TPL interactions, halving the vector-memory
+// traffic per interaction.  lane_leaf/lane_first map each lane to its leaf and
+// first target (built from the per-leaf lane counts by a scan).
+__global__ void p2p_lane_map_kernel(const uint32_t* leaf_ids, const uint32_t* leaf_lanes_off, uint32_t num_leaves,
+                                    NodeView nodes, uint32_t* lane_leaf, uint32_t* lane_first) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= num_leaves) return;
+    const int A = (int)leaf_ids[w];
+    const int4 lk = nodes.link[A];
+    const uint32_t o = leaf_lanes_off[w];
+    const int nl = (lk.y + P2P_TPL - 1) / P2P_TPL;
+    for (int k = 0; k < nl; ++k) {
+        lane_leaf[o + k] = (uint32_t)A;
+        lane_first[o + k] = (uint32_t)(lk.x + k * P2P_TPL);
+    }
+}
+
 template <int P>
-__global__ __launch_bounds__(256) void p2p_lane_kernel(FmmParams F, State S, uint32_t n, double kick_dt) {
+__global__ __launch_bounds__(256) void p2p_multi_kernel(FmmParams F, State S, const uint32_t* lanes_total,
+                                                        const uint32_t* lane_leaf, const uint32_t* lane_first,
+                                                        double kick_dt) {
     constexpr int NC = ncoef(P);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int A = F.pleaf[i];
+    constexpr int T = P2P_TPL;
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= *lanes_total) return;
+    const int A = (int)lane_leaf[w];
+    const int i0 = (int)lane_first[w];
     const int4 ga = F.nodes.geo[A];
+    const int4 la = F.nodes.link[A];
+    const int nt = min(T, la.x + la.y - i0);
     const float unit = F.unit[F.nodes.sys[A]];
     const float eps2 = F.eps2;
-    const float4 me = F.pf[i];
-    float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+    float4 me[T];
+    float ax[T], ay[T], az[T], ph[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        me[t] = F.pf[i0 + min(t, nt - 1)];
+        ax[t] = ay[t] = az[t] = ph[t] = 0.f;
+    }
     const unsigned long long e0 = F.p2p_off[A], e1 = F.p2p_off[A + 1];
     for (unsigned long long e = e0; e < e1; ++e) {
         const int B = F.p2p_src[e];
         const int4 gb = F.nodes.geo[B];
         const int4 lb = F.nodes.link[B];
         const float3 d = centre_diff(ga, gb, F.bits, unit);
-        const float dx = d.x - me.x, dy = d.y - me.y, dz = d.z - me.z;
+        float dx[T], dy[T], dz[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) { dx[t] = d.x - me[t].x; dy[t] = d.y - me[t].y; dz[t] = d.z - me[t].z; }
         const float4* src = F.pf + lb.x;
         if (B != A) {
-#pragma unroll 4
+#pragma unroll 2
             for (int j = 0; j < lb.y; ++j) {
                 const float4 s = src[j];
-                const float rx = dx + s.x, ry = dy + s.y, rz = dz + s.z;
-                const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
-                const float inv = rsqrtf(r2);
-                const float qi = s.w * inv;
-                const float qi3 = qi * inv * inv;
-                ax = fmaf(rx, qi3, ax);
-                ay = fmaf(ry, qi3, ay);
-                az = fmaf(rz, qi3, az);
-                ph += qi;
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    const float rx = dx[t] + s.x, ry = dy[t] + s.y, rz = dz[t] + s.z;
+                    const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+                    const float inv = rsqrtf(r2);
+                    const float qi = s.w * inv;
+                    const float qi3 = qi * inv * inv;
+                    ax[t] = fmaf(rx, qi3, ax[t]);
+                    ay[t] = fmaf(ry, qi3, ay[t]);
+                    az[t] = fmaf(rz, qi3, az[t]);
+                    ph[t] += qi;
+                }
             }
         } else {
-            const int self = (int)i - lb.x;
+            const int self0 = i0 - lb.x;
             for (int j = 0; j < lb.y; ++j) {
                 const float4 s = src[j];
-                const float rx = dx + s.x, ry = dy + s.y, rz = dz + s.z;
-                const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
-                const float inv = j == self ? 0.f : rsqrtf(r2);
-                const float qi = s.w * inv;
-                const float qi3 = qi * inv * inv;
-                ax = fmaf(rx, qi3, ax);
-                ay = fmaf(ry, qi3, ay);
-                az = fmaf(rz, qi3, az);
-                ph += qi;
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    const float rx = dx[t] + s.x, ry = dy[t] + s.y, rz = dz[t] + s.z;
+                    const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+                    const float inv = j == self0 + min(t, nt - 1) ? 0.f : rsqrtf(r2);
+                    const float qi = s.w * inv;
+                    const float qi3 = qi * inv * inv;
+                    ax[t] = fmaf(rx, qi3, ax[t]);
+                    ay[t] = fmaf(ry, qi3, ay[t]);
+                    az[t] = fmaf(rz, qi3, az[t]);
+                    ph[t] += qi;
+                }
             }
         }
     }
-    float lphi, gx, gy, gz;
+    float Lr[NC];
     {
         const float* Lsrc = F.L + (size_t)A * F.ncp;
-        float Lr[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) Lr[k] = Lsrc[k];
-        l2p_eval<P>(me.x, me.y, me.z, Lr, lphi, gx, gy, gz);
     }
-    const float fac = (float)(S.charge[i] / S.mass[i]) * F.G;
-    const float axo = fac * (ax + gx), ayo = fac * (ay + gy), azo = fac * (az + gz);
-    S.ax[i] = axo;
-    S.ay[i] = ayo;
-    S.az[i] = azo;
-    S.pot[i] = -F.G * (ph + lphi);
-    if (kick_dt != 0.0) {
-        S.vx[i] += (double)axo * kick_dt;
-        S.vy[i] += (double)ayo * kick_dt;
-        S.vz[i] += (double)azo * kick_dt;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        if (t >= nt) break;
+        const int i = i0 + t;
+        float lphi, gx, gy, gz;
+        l2p_eval<P>(me[t].x, me[t].y, me[t].z, Lr, lphi, gx, gy, gz);
+        const float fac = (float)(S.charge[i] / S.mass[i]) * F.G;
+        const float axo = fac * (ax[t] + gx), ayo = fac * (ay[t] + gy), azo = fac * (az[t] + gz);
+        S.ax[i] = axo;
+        S.ay[i] = ayo;
+        S.az[i] = azo;
+        S.pot[i] = -F.G * (ph[t] + lphi);
+        if (kick_dt != 0.0) {
+            S.vx[i] += (double)axo * kick_dt;
+            S.vy[i] += (double)ayo * kick_dt;
+            S.vz[i] += (double)azo * kick_dt;
+        }
     }
 }
 
@@ -358,7 +396,8 @@ __global__ __launch_bounds__(256) void p2p_lane_kernel(FmmParams F, State S, uin
 // cell) build their tensor per pair.  One wavefront per target cell; lanes
 // stride its source list and meet in a wave reduction.
 template <int P>
-__global__ void dtable_kernel(FmmParams F, int level0, int nlev, int K) {
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) void dtable_kernel(FmmParams F, int level0,
+                                                                                                int nlev, int K) {
     constexpr int NC = ncoef(P);
     const int side = 2 * K + 1;
     const int per = side * side * side;
@@ -375,7 +414,8 @@ __global__ void dtable_kernel(FmmParams F, int level0, int nlev, int K) {
 }
 
 template <int P>
-__global__ __launch_bounds__(256) void m2l_grouped_kernel(FmmParams F, uint32_t num_nodes) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_grouped_kernel(
+    FmmParams F, uint32_t num_nodes) {
     constexpr int NC = ncoef(P);
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const uint32_t nw = gridDim.x * blockDim.x / WAVE;
@@ -446,8 +486,10 @@ static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves
 }
 
 template <int P>
-static void launch_p2p_lane_t(const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st) {
-    hipLaunchKernelGGL((p2p_lane_kernel<P>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, S, n, kick_dt);
+static void launch_p2p_multi_t(const FmmParams& F, const State& S, uint32_t max_lanes, const uint32_t* lanes_total,
+                               const uint32_t* lane_leaf, const uint32_t* lane_first, double kick_dt, hipStream_t st) {
+    hipLaunchKernelGGL((p2p_multi_kernel<P>), dim3(ceil_div(max_lanes, 256)), dim3(256), 0, st, F, S, lanes_total,
+                       lane_leaf, lane_first, kick_dt);
 }
 template <int P>
 static void launch_dtable_t(const FmmParams& F, int level0, int nlev, int K, hipStream_t st) {
@@ -490,8 +532,13 @@ void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint
     if (!count) return;
     NBH_DISPATCH_P(p, launch_l2l_t<PP>(F, count, off, st));
 }
-void launch_p2p_lane(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st) {
-    NBH_DISPATCH_P(p, launch_p2p_lane_t<PP>(F, S, n, kick_dt, st));
+void launch_p2p_multi(int p, const FmmParams& F, const State& S, uint32_t num_leaves, uint32_t max_lanes,
+                      const uint32_t* leaf_lanes_off, const uint32_t* lanes_total, uint32_t* lane_leaf,
+                      uint32_t* lane_first, double kick_dt, hipStream_t st) {
+    if (!num_leaves) return;
+    hipLaunchKernelGGL(p2p_lane_map_kernel, dim3(ceil_div(num_leaves, 256)), dim3(256), 0, st, F.leaf_ids,
+                       leaf_lanes_off, num_leaves, F.nodes, lane_leaf, lane_first);
+    NBH_DISPATCH_P(p, launch_p2p_multi_t<PP>(F, S, max_lanes, lanes_total, lane_leaf, lane_first, kick_dt, st));
 }
 void launch_dtable(int p, const FmmParams& F, int level0, int nlev, int K, hipStream_t st) {
     if (nlev <= 0) return;

@@ -86,6 +86,7 @@ struct FmmParams {
     const unsigned long long* p2p_off;
     const int* p2p_src;
     uint32_t* pleaf;       // per particle: its leaf node
+    uint32_t* leaf_lanes;  // per leaf (leaf order): lanes of the multi-target P2P
     float* dtab;           // per-level derivative tensors of same-level offsets (or null)
     int dK, dlevel0, dnlev;
     double eps2_64;
@@ -97,7 +98,10 @@ void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint
 void launch_m2l_plain(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st);
 void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st);
-void launch_p2p_lane(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st);
+constexpr int P2P_TPL = 2;  // target particles per lane in p2p_multi_kernel
+void launch_p2p_multi(int p, const FmmParams& F, const State& S, uint32_t num_leaves, uint32_t max_lanes,
+                      const uint32_t* leaf_lanes_off, const uint32_t* lanes_total, uint32_t* lane_leaf,
+                      uint32_t* lane_first, double kick_dt, hipStream_t st);
 void launch_dtable(int p, const FmmParams& F, int level0, int nlev, int K, hipStream_t st);
 void launch_m2l_grouped(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st);
 

@@ -93,7 +93,8 @@ struct nbh_handle {
     int cur = 0;
     DevBuf sys_off, keys, keys_alt, perm, perm_alt, sort_tmp, tree_tmp, scan_tmp;
     DevBuf box, inv, unit, bbox_part;
-    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab;
+    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab, leaf_lanes, leaf_lanes_off, lane_leaf,
+        lane_first;
     DevBuf row_node, row_src, row_sl_start, row_sl_cnt, sl, m2l_cnt, p2p_cnt, m2l_off, p2p_off, m2l_cur, p2p_cur,
         m2l_src, p2p_src;
     DevBuf counters, diag, tmp_small;
@@ -296,6 +297,7 @@ static FmmParams fmm_params(nbh_handle* h) {
     F.p2p_off = h->p2p_off.as<unsigned long long>();
     F.p2p_src = h->p2p_src.as<int>();
     F.pleaf = h->pleaf.as<uint32_t>();
+    F.leaf_lanes = h->leaf_lanes.as<uint32_t>();
     F.dtab = nullptr;
     F.eps2_64 = h->cfg.eps * h->cfg.eps;
     return F;
@@ -434,8 +436,17 @@ static int force_fmm(nbh_handle* h, double kick_dt) {
     for (uint32_t l = 1; l <= h->depth; ++l)
         launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
     record(h, 5);
-    if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_lane(p, F, S, (uint32_t)h->n, kick_dt, h->stream);
-    else launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
+    if (h->cfg.p2p == NBH_P2P_LANE) {
+        // lanes per leaf -> offsets (total in the last slot)
+        uint32_t* lo = h->leaf_lanes_off.as<uint32_t>();
+        exclusive_scan<uint32_t, uint32_t>(h->leaf_lanes.as<uint32_t>(), lo, nullptr, h->num_leaves,
+                                           lo + h->num_leaves, h->scan_tmp.p, h->stream);
+        const uint32_t max_lanes = (uint32_t)std::min<size_t>(h->n, h->n / P2P_TPL + h->num_leaves);
+        launch_p2p_multi(p, F, S, h->num_leaves, max_lanes, lo, lo + h->num_leaves, h->lane_leaf.as<uint32_t>(),
+                         h->lane_first.as<uint32_t>(), kick_dt, h->stream);
+    } else {
+        launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
+    }
     record(h, 6);
     HIPCHK(h, hipGetLastError());
     // stats (read at the end of the evaluation; one more sync)
@@ -614,6 +625,10 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
     CCHK(h->leaf_ids.ensure(n * 4));
     CCHK(h->pf.ensure(n * 16));
     CCHK(h->pleaf.ensure(n * 4));
+    CCHK(h->leaf_lanes.ensure(n * 4));
+    CCHK(h->leaf_lanes_off.ensure((n + 1) * 4));
+    CCHK(h->lane_leaf.ensure(n * 4));
+    CCHK(h->lane_first.ensure(n * 4));
     CCHK(h->counters.ensure(sizeof(Counters)));
     CCHK(h->diag.ensure(1024 * 8 * 8));
     CCHK(h->tmp_small.ensure(64));
@@ -678,7 +693,7 @@ void nbh_destroy(nbh_handle* h) {
     h->st[1].release();
     DevBuf* all[] = {&h->sys_off, &h->keys, &h->keys_alt, &h->perm, &h->perm_alt, &h->sort_tmp, &h->tree_tmp,
                      &h->scan_tmp, &h->box, &h->inv, &h->unit, &h->bbox_part, &h->geo, &h->link, &h->nsys_of,
-                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->row_node, &h->row_src,
+                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->leaf_lanes, &h->leaf_lanes_off, &h->lane_leaf, &h->lane_first, &h->row_node, &h->row_src,
                      &h->row_sl_start, &h->row_sl_cnt, &h->sl, &h->m2l_cnt, &h->p2p_cnt, &h->m2l_off,
                      &h->p2p_off, &h->m2l_cur, &h->p2p_cur, &h->m2l_src, &h->p2p_src, &h->counters, &h->diag,
                      &h->tmp_small};
