@@ -256,4 +256,60 @@ __device__ __forceinline__ void l2p_eval(float ax, float ay, float az, const flo
     phi = p; gx = x; gy = y; gz = z;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint helpers.
+//
+// Dipole P2M: moments of point dipoles w at offsets b,
+//   M_n = sum_i sum_e w_{i,e} b_i^(n - e_e) / (n - e_e)!     (n_e >= 1)
+// i.e. the derivative of the monopole moments with respect to the source
+// position; through the unchanged M2M/M2L/L2L it yields the local expansion of
+// psi(x) = sum_i w_i . grad g(x_i - x).
+template <int P, int K>
+__device__ __forceinline__ void dip_step(const float* mono, float wx, float wy, float wz, float* M) {
+    constexpr auto& T = Tab<P>::t;
+    if constexpr (T.mx[K] > 0) M[K] = fmaf(wx, mono[T.at(T.mx[K] - 1, T.my[K], T.mz[K])], M[K]);
+    if constexpr (T.my[K] > 0) M[K] = fmaf(wy, mono[T.at(T.mx[K], T.my[K] - 1, T.mz[K])], M[K]);
+    if constexpr (T.mz[K] > 0) M[K] = fmaf(wz, mono[T.at(T.mx[K], T.my[K], T.mz[K] - 1)], M[K]);
+}
+template <int P, size_t... I>
+__device__ __forceinline__ void dip_all(const float* mono, float wx, float wy, float wz, float* M,
+                                        std::index_sequence<I...>) {
+    (dip_step<P, (int)I + 1>(mono, wx, wy, wz, M), ...);
+}
+template <int P>
+__device__ __forceinline__ void dipole_p2m_accum(float bx, float by, float bz, float wx, float wy, float wz,
+                                                 float* M) {
+    float mono[ncoef(P)];
+    monomials<P>(bx, by, bz, mono);
+    dip_all<P>(mono, wx, wy, wz, M, std::make_index_sequence<ncoef(P) - 1>{});
+}
+
+// Hessian of phi_L(a) = sum_k L_k a^k/k!:  H_ab = sum_{|k| <= P-2} L_{k+e_a+e_b} a^k/k!
+template <int P, int K>
+__device__ __forceinline__ void hess_step(const float* mono, const float* L, float* H) {
+    constexpr auto& T = Tab<P>::t;
+    if constexpr (T.deg[K] + 2 <= P) {
+        constexpr int x = T.mx[K], y = T.my[K], z = T.mz[K];
+        H[0] = fmaf(L[T.at(x + 2, y, z)], mono[K], H[0]);
+        H[1] = fmaf(L[T.at(x, y + 2, z)], mono[K], H[1]);
+        H[2] = fmaf(L[T.at(x, y, z + 2)], mono[K], H[2]);
+        H[3] = fmaf(L[T.at(x + 1, y + 1, z)], mono[K], H[3]);
+        H[4] = fmaf(L[T.at(x + 1, y, z + 1)], mono[K], H[4]);
+        H[5] = fmaf(L[T.at(x, y + 1, z + 1)], mono[K], H[5]);
+    }
+}
+template <int P, size_t... I>
+__device__ __forceinline__ void hess_all(const float* mono, const float* L, float* H, std::index_sequence<I...>) {
+    (hess_step<P, (int)I>(mono, L, H), ...);
+}
+// H = {xx, yy, zz, xy, xz, yz}
+template <int P>
+__device__ __forceinline__ void l2p_hessian(float ax, float ay, float az, const float* L, float* H) {
+    float mono[ncoef(P)];
+    monomials<P>(ax, ay, az, mono);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) H[k] = 0.f;
+    if constexpr (P >= 2) hess_all<P>(mono, L, H, std::make_index_sequence<ncoef(P)>{});
+}
+
 }  // namespace nbh

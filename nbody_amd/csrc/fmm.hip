@@ -19,24 +19,6 @@
 
 namespace nbh {
 
-// centre difference c_B - c_A of two cells of one system, FP32
-__device__ __forceinline__ float3 centre_diff(int4 ga, int4 gb, int bits, float unit) {
-    const int sa = bits - (ga.w & 0xff), sb = bits - (gb.w & 0xff);
-    const int dx = ((2 * gb.x + 1) << sb) - ((2 * ga.x + 1) << sa);
-    const int dy = ((2 * gb.y + 1) << sb) - ((2 * ga.y + 1) << sa);
-    const int dz = ((2 * gb.z + 1) << sb) - ((2 * ga.z + 1) << sa);
-    return make_float3((float)dx * unit, (float)dy * unit, (float)dz * unit);
-}
-
-// FP64 centre of a cell
-__device__ __forceinline__ void centre64(int4 g, double4 box, int bits, double c[3]) {
-    const int l = g.w & 0xff;
-    const double h = ldexp(box.w, -l);
-    c[0] = box.x + ((double)g.x + 0.5) * h;
-    c[1] = box.y + ((double)g.y + 0.5) * h;
-    c[2] = box.z + ((double)g.z + 0.5) * h;
-}
-
 // ---------------------------------------------------------------------------
 __global__ void gather_state_kernel(State a, State b, const uint32_t* __restrict__ perm, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

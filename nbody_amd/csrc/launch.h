@@ -118,6 +118,11 @@ void launch_direct_fp64(const State& S, const uint32_t* sys_off, int nsys, const
 void launch_diagnostics(const State& S, uint32_t n, double* partial, uint32_t nblk, hipStream_t st);
 uint32_t diagnostics_blocks(uint32_t n);
 
+// adjoint (vjp.hip): g = [gx | gy | gz | gm] in sorted order, FP64
+void launch_force_vjp(int p, const FmmParams& F, const FmmParams& Fw, const double* w_in, const uint32_t* id,
+                      float4* pw, uint32_t n, uint32_t num_leaves, const uint32_t* level_count,
+                      const uint32_t* level_off, int depth, double* g, hipStream_t st);
+
 void launch_inverse_perm(const uint32_t* id, uint32_t* slot_of, uint32_t n, hipStream_t st);
 
 }  // namespace nbh

@@ -99,6 +99,9 @@ struct nbh_handle {
         m2l_src, p2p_src;
     DevBuf counters, diag, tmp_small;
     Counters* hc = nullptr;  // pinned host copy
+    DevBuf Mw, Lw, pw, wbuf, gbuf;   // adjoint
+    bool gravity = true;             // charge == mass for every particle
+    FmmParams lastF{};               // parameters of the last FMM force evaluation
     uint32_t node_cap = 0, row_cap = 0, sl_cap = 0;
     unsigned long long m2l_cap = 0, p2p_cap = 0;
     uint32_t num_nodes = 0, num_leaves = 0, depth = 0;
@@ -448,6 +451,7 @@ static int force_fmm(nbh_handle* h, double kick_dt) {
         launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
     }
     record(h, 6);
+    h->lastF = F;
     HIPCHK(h, hipGetLastError());
     // stats (read at the end of the evaluation; one more sync)
     CHK(read_counters(h));
@@ -574,12 +578,16 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
         if (!(mass[i] > 0) || !std::isfinite(mass[i])) { g_create_error = "masses must be finite and > 0"; return NBH_E_INVALID; }
         if (charge && !std::isfinite(charge[i])) { g_create_error = "non-finite charge"; return NBH_E_NONFINITE; }
     }
+    bool gravity = true;
+    if (charge)
+        for (size_t i = 0; i < n; ++i) gravity = gravity && charge[i] == mass[i];
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device available"; return NBH_E_HIP; }
     if (cfg->device < 0 || cfg->device >= ndev) { g_create_error = "device ordinal out of range"; return NBH_E_INVALID; }
 
     nbh_handle* h = new nbh_handle;
     h->cfg = *cfg;
+    h->gravity = gravity;
     h->n = n;
     h->nsys = (int)nsys;
     h->device = cfg->device;
@@ -696,7 +704,7 @@ void nbh_destroy(nbh_handle* h) {
                      &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->leaf_lanes, &h->leaf_lanes_off, &h->lane_leaf, &h->lane_first, &h->row_node, &h->row_src,
                      &h->row_sl_start, &h->row_sl_cnt, &h->sl, &h->m2l_cnt, &h->p2p_cnt, &h->m2l_off,
                      &h->p2p_off, &h->m2l_cur, &h->p2p_cur, &h->m2l_src, &h->p2p_src, &h->counters, &h->diag,
-                     &h->tmp_small};
+                     &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf};
     for (DevBuf* b : all) b->release();
     if (h->hc) (void)hipHostFree(h->hc);
     for (auto& e : h->ev)
@@ -859,8 +867,43 @@ int nbh_set_timing(nbh_handle* h, int enable) {
 
 int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
     if (!h) return NBH_E_INVALID;
-    (void)w; (void)gx; (void)gm;
-    return h->fail(NBH_E_INVALID, "nbh_force_vjp is not available in this build");
+    if (!w || !gx || !gm) return h->fail(NBH_E_INVALID, "w, gx and gm are required");
+    if (h->cfg.solver != NBH_SOLVER_FMM) return h->fail(NBH_E_INVALID, "nbh_force_vjp needs the FMM solver");
+    if (!h->gravity) return h->fail(NBH_E_INVALID, "nbh_force_vjp needs charge == mass (gravity)");
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t n = h->n;
+    // tree, lists and mass expansions at the current positions
+    CHK(force_fmm(h, 0.0));
+    h->acc_valid = true;
+    const int p = (int)h->cfg.order;
+    const size_t ncp = (size_t)ncp_of(p);
+    HIPCHK(h, h->Mw.ensure((size_t)h->num_nodes * ncp * 4));
+    HIPCHK(h, h->Lw.ensure((size_t)h->num_nodes * ncp * 4));
+    HIPCHK(h, h->pw.ensure(n * 16));
+    HIPCHK(h, h->wbuf.ensure(n * 24));
+    HIPCHK(h, h->gbuf.ensure(n * 32));
+    HIPCHK(h, hipMemcpyAsync(h->wbuf.p, w, n * 24, hipMemcpyHostToDevice, h->stream));
+    FmmParams F = h->lastF;
+    FmmParams Fw = F;
+    Fw.M = h->Mw.as<float>();
+    Fw.L = h->Lw.as<float>();
+    State S = h->st[h->cur].view();
+    launch_force_vjp(p, F, Fw, h->wbuf.as<double>(), S.id, h->pw.as<float4>(), (uint32_t)n, h->num_leaves,
+                     h->hc->level_count, h->hc->level_off, (int)h->depth, h->gbuf.as<double>(), h->stream);
+    HIPCHK(h, hipGetLastError());
+    std::vector<double> g(4 * n);
+    std::vector<uint32_t> ids(n);
+    HIPCHK(h, hipMemcpyAsync(g.data(), h->gbuf.p, 4 * n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ids.data(), S.id, n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; ++i) {
+        const size_t j = ids[i];
+        gx[3 * j] = g[i];
+        gx[3 * j + 1] = g[n + i];
+        gx[3 * j + 2] = g[2 * n + i];
+        gm[j] = g[3 * n + i];
+    }
+    return NBH_OK;
 }
 
 int nbh_debug_sizes(nbh_handle* h, int64_t* sizes) {

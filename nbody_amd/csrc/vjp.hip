@@ -1,0 +1,160 @@
+// vjp.hip -- adjoint of the FMM force evaluation (no counterpart in the
+// reference, whose pipeline is forward only).
+//
+// For gravity (charge == mass), a_i = G sum_j m_j f(x_j - x_i) with
+// f(r) = r / (|r|^2+eps^2)^(3/2).  Its vector-Jacobian product with a
+// cotangent w is
+//   gx_k = G sum_j T(x_j - x_k) (m_k w_j - m_j w_k),   T(r) = df/dr (tidal tensor)
+//   gm_k = G psi(x_k),   psi(x) = sum_j w_j . grad g(x_j - x),  g = (|r|^2+eps^2)^(-1/2)
+// and gx_k = G [ m_k grad psi(x_k) + Hess(phi)(x_k) w_k ] with phi the mass
+// potential of the forward pass.  The far field is the exact transpose of
+// the forward FMM on the same tree and lists: L2L^T has the form of M2M,
+// M2M^T of L2L, and M2L^T is M2L between the same (symmetric) cell pairs,
+// so the cotangents enter as point dipoles (dipole P2M) and run through the
+// unchanged M2M -> M2L -> L2L kernels; the near field is the exact pair VJP
+// over the P2P lists, and Hess(phi) comes from the forward local expansions.
+#include <algorithm>
+
+#include "common.h"
+#include "expansion.h"
+#include "launch.h"
+
+namespace nbh {
+
+// cotangent in sorted order: pw[i] = (w[id[i]], 0), FP32
+__global__ void vjp_gather_kernel(const double* w_in, const uint32_t* id, float4* pw, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = id[i];
+    pw[i] = make_float4((float)w_in[3 * (size_t)j], (float)w_in[3 * (size_t)j + 1], (float)w_in[3 * (size_t)j + 2],
+                        0.f);
+}
+
+// dipole P2M per leaf (one wavefront), offsets from the forward pass (pf)
+template <int P>
+__global__ __launch_bounds__(256) void dipole_p2m_kernel(FmmParams F, const float4* pw, float* Mw,
+                                                         uint32_t num_leaves) {
+    constexpr int NC = ncoef(P);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (wid >= num_leaves) return;
+    const int A = (int)F.leaf_ids[wid];
+    const int4 lk = F.nodes.link[A];
+    float M[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) M[k] = 0.f;
+    for (int j = lane_id(); j < lk.y; j += WAVE) {
+        const int i = lk.x + j;
+        const float4 b = F.pf[i];
+        const float4 w = pw[i];
+        dipole_p2m_accum<P>(b.x, b.y, b.z, w.x, w.y, w.z, M);
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) M[k] = wave_sum(M[k]);
+    if (lane_id() == 0) {
+        float* out = Mw + (size_t)A * F.ncp;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) out[k] = M[k];
+    }
+}
+
+// near field (pair VJP over the P2P list) + far field (Hessian of the mass
+// local expansion, gradient and value of the dipole local expansion); one
+// lane per particle.
+template <int P>
+__global__ __launch_bounds__(256) void vjp_final_kernel(FmmParams F, const float4* pw, const float* Lw, uint32_t n,
+                                                        double* gx, double* gy, double* gz, double* gm) {
+    constexpr int NC = ncoef(P);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int A = F.pleaf[i];
+    const int4 ga = F.nodes.geo[A];
+    const float unit = F.unit[F.nodes.sys[A]];
+    const float eps2 = F.eps2;
+    const float4 me = F.pf[i];
+    const float4 wi = pw[i];
+    const float mi = me.w;
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f, gmm = 0.f;
+    const unsigned long long e0 = F.p2p_off[A], e1 = F.p2p_off[A + 1];
+    for (unsigned long long e = e0; e < e1; ++e) {
+        const int B = F.p2p_src[e];
+        const int4 gb = F.nodes.geo[B];
+        const int4 lb = F.nodes.link[B];
+        const float3 d = centre_diff(ga, gb, F.bits, unit);
+        const float dx = d.x - me.x, dy = d.y - me.y, dz = d.z - me.z;
+        for (int j = 0; j < lb.y; ++j) {
+            const int src = lb.x + j;
+            const float4 s = F.pf[src];
+            const float4 wj = pw[src];
+            const float rx = dx + s.x, ry = dy + s.y, rz = dz + s.z;
+            const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+            const float inv = src == (int)i ? 0.f : rsqrtf(r2);
+            const float i2 = inv * inv;
+            const float i3 = i2 * inv;
+            const float i5 = i3 * i2;
+            const float ux = mi * wj.x - s.w * wi.x, uy = mi * wj.y - s.w * wi.y, uz = mi * wj.z - s.w * wi.z;
+            const float ru = rx * ux + ry * uy + rz * uz;
+            const float c = 3.f * ru * i5;
+            g0 += ux * i3 - rx * c;
+            g1 += uy * i3 - ry * c;
+            g2 += uz * i3 - rz * c;
+            gmm -= (wj.x * rx + wj.y * ry + wj.z * rz) * i3;
+        }
+    }
+    // far field
+    float L[NC];
+    float H[6];
+    {
+        const float* Ls = F.L + (size_t)A * F.ncp;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = Ls[k];
+        l2p_hessian<P>(me.x, me.y, me.z, L, H);
+    }
+    float psi, px, py, pz;
+    {
+        const float* Ls = Lw + (size_t)A * F.ncp;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = Ls[k];
+        l2p_eval<P>(me.x, me.y, me.z, L, psi, px, py, pz);
+    }
+    g0 += mi * px + H[0] * wi.x + H[3] * wi.y + H[4] * wi.z;
+    g1 += mi * py + H[3] * wi.x + H[1] * wi.y + H[5] * wi.z;
+    g2 += mi * pz + H[4] * wi.x + H[5] * wi.y + H[2] * wi.z;
+    gmm += psi;
+    const double G = (double)F.G;
+    gx[i] = G * g0;
+    gy[i] = G * g1;
+    gz[i] = G * g2;
+    gm[i] = G * gmm;
+}
+
+template <int P>
+static void launch_vjp_t(const FmmParams& F, const FmmParams& Fw, const double* w_in, const uint32_t* id,
+                         float4* pw, uint32_t n, uint32_t num_leaves, const uint32_t* level_count,
+                         const uint32_t* level_off, int depth, double* g, hipStream_t st) {
+    hipLaunchKernelGGL(vjp_gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, w_in, id, pw, n);
+    hipLaunchKernelGGL((dipole_p2m_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, pw, Fw.M,
+                       num_leaves);
+    for (int l = depth - 1; l >= 0; --l) launch_m2m_level(P, Fw, l, level_count[l], level_off[l], st);
+    uint32_t nn = 0;
+    for (int l = 0; l <= depth; ++l) nn += level_count[l];
+    launch_m2l_grouped(P, Fw, nn, st);
+    for (int l = 1; l <= depth; ++l) launch_l2l_level(P, Fw, l, level_count[l], level_off[l], st);
+    hipLaunchKernelGGL((vjp_final_kernel<P>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, pw, Fw.L, n, g, g + n,
+                       g + 2 * (size_t)n, g + 3 * (size_t)n);
+}
+
+void launch_force_vjp(int p, const FmmParams& F, const FmmParams& Fw, const double* w_in, const uint32_t* id,
+                      float4* pw, uint32_t n, uint32_t num_leaves, const uint32_t* level_count,
+                      const uint32_t* level_off, int depth, double* g, hipStream_t st) {
+    switch (p) {
+        case 1: launch_vjp_t<1>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
+        case 2: launch_vjp_t<2>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
+        case 3: launch_vjp_t<3>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
+        case 4: launch_vjp_t<4>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
+        case 5: launch_vjp_t<5>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
+        case 6: launch_vjp_t<6>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
+        default: break;
+    }
+}
+
+}  // namespace nbh

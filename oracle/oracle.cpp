@@ -601,6 +601,111 @@ void orc_fmm(const orc_tree* tr, const orc_lists* li, int p, size_t n, const dou
 }
 
 // ---------------------------------------------------------------------------
+// VJP of orc_fmm (gravity, q == m) with cotangent w, on the same tree and
+// lists: the exact transpose of the discrete FMM.  Near field: pair terms
+// T(r)(m_i w_j - m_j w_i) and -(w_j . r)/s^3.  Far field: Hessian of the mass
+// local expansion applied to w_i, plus m_i grad psi and psi from the dipole
+// moments M^w_n = sum_i sum_e w_ie b^(n-e)/(n-e)! run through M2M, M2L, L2L.
+void orc_fmm_vjp(const orc_tree* tr, const orc_lists* li, int p, size_t n, const double* x, const double* m,
+                 const double* w, const double* box, double G, double eps, double* gx, double* gm) {
+    const Tree& t = tr->t;
+    const Lists& L = li->l;
+    MultiIndex T(p);
+    const int nc = T.size();
+    size_t nn = t.level.size();
+    std::vector<double> M(nn * nc, 0.0), Lc(nn * nc, 0.0), Mw(nn * nc, 0.0), Lw(nn * nc, 0.0), cen(nn * 3);
+    for (size_t i = 0; i < nn; ++i) cell_centre(t, box, (int)i, &cen[3 * i]);
+    std::vector<double> mono;
+    for (size_t a = nn; a-- > 0;) {
+        if (t.nchild[a] == 0) {
+            orc_p2m(p, &cen[3 * a], (size_t)t.count[a], x + 3 * (size_t)t.begin[a], m + t.begin[a], &M[a * nc]);
+            for (int i = t.begin[a]; i < t.begin[a] + t.count[a]; ++i) {
+                double b[3] = {x[3 * i] - cen[3 * a], x[3 * i + 1] - cen[3 * a + 1], x[3 * i + 2] - cen[3 * a + 2]};
+                monomials(T, b, mono);
+                for (int k = 0; k < nc; ++k) {
+                    const auto& nk = T.mi[k];
+                    for (int e = 0; e < 3; ++e) {
+                        int d[3] = {nk[0], nk[1], nk[2]};
+                        if (d[e] == 0) continue;
+                        d[e] -= 1;
+                        Mw[a * nc + k] += w[3 * i + e] * mono[T.at(d[0], d[1], d[2])];
+                    }
+                }
+            }
+        } else {
+            for (int c = 0; c < t.nchild[a]; ++c) {
+                int ch = t.first_child[a] + c;
+                orc_m2m(p, &cen[3 * ch], &cen[3 * a], &M[(size_t)ch * nc], &M[a * nc]);
+                orc_m2m(p, &cen[3 * ch], &cen[3 * a], &Mw[(size_t)ch * nc], &Mw[a * nc]);
+            }
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long long a = 0; a < (long long)nn; ++a)
+        for (int64_t e = L.m2l_off[a]; e < L.m2l_off[a + 1]; ++e) {
+            int b = L.m2l_src[e];
+            orc_m2l(p, &cen[3 * a], &cen[3 * b], eps, &M[(size_t)b * nc], &Lc[(size_t)a * nc]);
+            orc_m2l(p, &cen[3 * a], &cen[3 * b], eps, &Mw[(size_t)b * nc], &Lw[(size_t)a * nc]);
+        }
+    for (size_t a = 0; a < nn; ++a)
+        for (int c = 0; c < t.nchild[a]; ++c) {
+            int ch = t.first_child[a] + c;
+            orc_l2l(p, &cen[3 * a], &cen[3 * ch], &Lc[a * nc], &Lc[(size_t)ch * nc]);
+            orc_l2l(p, &cen[3 * a], &cen[3 * ch], &Lw[a * nc], &Lw[(size_t)ch * nc]);
+        }
+    const double e2 = eps * eps;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long long a = 0; a < (long long)nn; ++a) {
+        if (t.nchild[a]) continue;
+        std::vector<double> mo;
+        for (int i = t.begin[a]; i < t.begin[a] + t.count[a]; ++i) {
+            double g[3] = {0, 0, 0}, gmi = 0;
+            for (int64_t e = L.p2p_off[a]; e < L.p2p_off[a + 1]; ++e) {
+                int b = L.p2p_src[e];
+                for (int j = t.begin[b]; j < t.begin[b] + t.count[b]; ++j) {
+                    if (j == i) continue;
+                    double r[3] = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
+                    double inv = 1.0 / std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + e2);
+                    double i3 = inv * inv * inv, i5 = i3 * inv * inv;
+                    double u[3], ru = 0, wr = 0;
+                    for (int k = 0; k < 3; ++k) {
+                        u[k] = m[i] * w[3 * j + k] - m[j] * w[3 * i + k];
+                        ru += r[k] * u[k];
+                        wr += w[3 * j + k] * r[k];
+                    }
+                    for (int k = 0; k < 3; ++k) g[k] += u[k] * i3 - 3.0 * r[k] * ru * i5;
+                    gmi -= wr * i3;
+                }
+            }
+            double b[3] = {x[3 * i] - cen[3 * a], x[3 * i + 1] - cen[3 * a + 1], x[3 * i + 2] - cen[3 * a + 2]};
+            monomials(T, b, mo);
+            double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, psi = 0, dpsi[3] = {0, 0, 0};
+            for (int k = 0; k < nc; ++k) {
+                const auto& kk = T.mi[k];
+                psi += Lw[(size_t)a * nc + k] * mo[k];
+                for (int u = 0; u < 3; ++u) {
+                    int d1[3] = {kk[0], kk[1], kk[2]};
+                    d1[u] += 1;
+                    int s1 = T.at(d1[0], d1[1], d1[2]);
+                    if (s1 >= 0) dpsi[u] += Lw[(size_t)a * nc + s1] * mo[k];
+                    for (int v = 0; v < 3; ++v) {
+                        int d2[3] = {d1[0], d1[1], d1[2]};
+                        d2[v] += 1;
+                        int s2 = T.at(d2[0], d2[1], d2[2]);
+                        if (s2 >= 0) H[u][v] += Lc[(size_t)a * nc + s2] * mo[k];
+                    }
+                }
+            }
+            for (int u = 0; u < 3; ++u) {
+                double hw = H[u][0] * w[3 * i] + H[u][1] * w[3 * i + 1] + H[u][2] * w[3 * i + 2];
+                gx[3 * i + u] = G * (g[u] + m[i] * dpsi[u] + hw);
+            }
+            gm[i] = G * (gmi + psi);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 static void accel_direct(size_t n, const double* x, const double* m, const double* q,
                          double G, double eps, std::vector<double>& a) {
     a.assign(3 * n, 0.0);

@@ -105,6 +105,11 @@ void orc_fmm(const orc_tree*, const orc_lists*, int p, size_t n, const double* x
              const double* q, const double* box, double G, double eps,
              double* acc, double* pot);
 
+/* VJP of orc_fmm for gravity (q == m): gx[3n], gm[n] of sum_i w_i . acc_i,
+ * with the tree and lists held fixed (exact transpose of the FMM). */
+void orc_fmm_vjp(const orc_tree*, const orc_lists*, int p, size_t n, const double* x, const double* m,
+                 const double* w, const double* box, double G, double eps, double* gx, double* gm);
+
 /* ---- integrators over the FP64 direct sum ------------------------------- */
 /* integrator: 0 = KDK leapfrog, 1 = explicit Euler (open_cl_simulation.cpp:
  * 600-607), 2 = kick-drift (naive_simulation.cpp:28-41).  In-place, nsteps. */

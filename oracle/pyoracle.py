@@ -245,6 +245,21 @@ def fmm(tree: Tree, lists: Lists, p, x_sorted, q_sorted, box, G=1.0, eps=0.0):
     return acc, pot
 
 
+def fmm_vjp(tree: Tree, lists: Lists, p, x_sorted, m_sorted, w_sorted, box, G=1.0, eps=0.0):
+    x = c64(x_sorted).reshape(-1, 3)
+    m = c64(m_sorted)
+    w = c64(w_sorted).reshape(-1, 3)
+    box = c64(box).reshape(-1)
+    gx = np.zeros_like(x)
+    gm = np.zeros(len(x))
+    L = lib()
+    L.orc_fmm_vjp.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t,
+                              _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+    L.orc_fmm_vjp(ctypes.c_void_p(tree.handle), ctypes.c_void_p(lists.handle), p, len(x), _ptr(x, _dp),
+                  _ptr(m, _dp), _ptr(w, _dp), _ptr(box, _dp), G, eps, _ptr(gx, _dp), _ptr(gm, _dp))
+    return gx, gm
+
+
 def integrate_direct(x, v, m, q, G, eps, dt, nsteps, integrator=0):
     x = c64(x).reshape(-1, 3).copy(); v = c64(v).reshape(-1, 3).copy()
     m = c64(m); q = c64(q)

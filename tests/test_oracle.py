@@ -394,3 +394,38 @@ def test_euler_and_kick_drift_one_step_closed_form():
     assert np.allclose(xe, x + v * dt) and np.allclose(ve, v + a0 * dt)
     xk, vk = po.integrate_direct(x, v, m, m, 1.0, 0.0, dt, 1, 2)
     assert np.allclose(vk, v + a0 * dt) and np.allclose(xk, x + (v + a0 * dt) * dt)
+
+
+def test_fmm_vjp_is_the_exact_transpose_of_the_fmm():
+    """orc_fmm_vjp against FP64 central differences of orc_fmm with the tree
+    and lists held fixed, and against the direct-sum adjoint (to the FMM's
+    truncation accuracy)."""
+    rng = np.random.default_rng(13)
+    n = 600
+    x = np.concatenate([rng.random((500, 3)), 0.3 + 0.05 * rng.random((100, 3))])
+    m = rng.random(n) / n + 1e-4
+    keys, o, t, box = _tree(x, cap=8)
+    L = po.walk(t, 1, 2)
+    xs, ms = x[o], m[o]
+    w = rng.standard_normal((n, 3))
+    p, eps = 4, 1e-2
+    gx, gm = po.fmm_vjp(t, L, p, xs, ms, w, box, eps=eps)
+
+    def f(xx, mm):
+        a, _ = po.fmm(t, L, p, xx, mm, box, eps=eps)
+        return float((a * w).sum())
+
+    h = 1e-6
+    for i in rng.choice(n, 12, replace=False):
+        for k in range(3):
+            xp = xs.copy(); xp[i, k] += h
+            xm = xs.copy(); xm[i, k] -= h
+            fd = (f(xp, ms) - f(xm, ms)) / (2 * h)
+            assert abs(fd - gx[i, k]) < 1e-6 * np.abs(gx).max(), (i, k, fd, gx[i, k])
+        mp = ms.copy(); mp[i] += h * 1e-3
+        mm_ = ms.copy(); mm_[i] -= h * 1e-3
+        fd = (f(xs, mp) - f(xs, mm_)) / (2 * h * 1e-3)
+        assert abs(fd - gm[i]) < 1e-6 * np.abs(gm).max(), (i, fd, gm[i])
+    gxd, gmd = po.direct_vjp(xs, ms, w, eps=eps)
+    assert np.sqrt(np.sum((gx - gxd) ** 2) / np.sum(gxd ** 2)) < 2e-3
+    assert np.sqrt(np.sum((gm - gmd) ** 2) / np.sum(gmd ** 2)) < 2e-3
