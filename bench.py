@@ -145,7 +145,7 @@ def main():
                    "state_dtype": "fp64"},
         "roofline": {"bound": "mfma", "engine": "fp32-valu (P2P kernel)", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None, "kernel": ("p2p_lane_kernel" if args.p2p == "lane" else "p2p_kernel") + f"<{p}>", "ms_per_launch": p2p_ms,
+                     "traffic": None, "kernel": ("p2p_staged_kernel" if args.p2p == "lane" else "p2p_kernel") + f"<{p}>", "ms_per_launch": p2p_ms,
                      "flop_per_launch": p2p_flop},
         "phases_ms": {k[3:]: agg[k] / K for k in agg},
         "m2l": {"pairs_per_step": m2l_pairs / K, "tflops": m2l_tf},

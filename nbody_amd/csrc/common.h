@@ -33,7 +33,8 @@ struct Counters {
     uint32_t carried[MAX_LEVELS + 2];  // walk: carried leaf rows per depth
     uint32_t sl_used;                  // walk: split-list entries used
     uint32_t rows_used;
-    uint32_t pad;
+    uint32_t m2l_pool_used;            // walk: phase-1 segment pools used
+    uint32_t p2p_pool_used;
     unsigned long long m2l_total, p2p_total, p2p_inter, split_total;
     uint32_t nonfinite;
 };

@@ -70,7 +70,6 @@ __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t
         float* out = F.M + (size_t)A * F.ncp;
 #pragma unroll
         for (int k = 0; k < NC; ++k) out[k] = M[k];
-        if (F.leaf_lanes) F.leaf_lanes[wid] = (uint32_t)((lk.y + P2P_TPL - 1) / P2P_TPL);
     }
 }
 
@@ -257,114 +256,155 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// P2P with P2P_TPL target particles of one leaf per lane: every source
-// particle loaded is used for P2P_TPL interactions, halving the vector-memory
-// traffic per interaction.  lane_leaf/lane_first map each lane to its leaf and
-// first target (built from the per-leaf lane counts by a scan).
-__global__ void p2p_lane_map_kernel(const uint32_t* leaf_ids, const uint32_t* leaf_lanes_off, uint32_t num_leaves,
-                                    NodeView nodes, uint32_t* lane_leaf, uint32_t* lane_first) {
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= num_leaves) return;
-    const int A = (int)leaf_ids[w];
-    const int4 lk = nodes.link[A];
-    const uint32_t o = leaf_lanes_off[w];
-    const int nl = (lk.y + P2P_TPL - 1) / P2P_TPL;
-    for (int k = 0; k < nl; ++k) {
-        lane_leaf[o + k] = (uint32_t)A;
-        lane_first[o + k] = (uint32_t)(lk.x + k * P2P_TPL);
-    }
+// P2P, staged: one wavefront per target leaf, sources flattened across the
+// leaf's P2P list.  Up to 64 list entries at a time are read by the lanes
+// (one entry each: centre offset, first particle, count) and prefix-summed;
+// then 64 source particles at a time are gathered by a shuffle binary search
+// over that prefix, shifted into the target leaf's frame and staged in LDS.
+// Lanes are (target particle) x (source stride): with nt targets, lane l
+// takes target l % nt and staged sources j == l / nt (mod 64 / nt), so the
+// wave stays full for any leaf size and every lane runs the same trip count.
+constexpr int P2P_WAVES = 4;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int P>
-__global__ __launch_bounds__(256) void p2p_multi_kernel(FmmParams F, State S, const uint32_t* lanes_total,
-                                                        const uint32_t* lane_leaf, const uint32_t* lane_first,
-                                                        double kick_dt) {
+__global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves,
+                                                                     double kick_dt) {
     constexpr int NC = ncoef(P);
-    constexpr int T = P2P_TPL;
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= *lanes_total) return;
-    const int A = (int)lane_leaf[w];
-    const int i0 = (int)lane_first[w];
-    const int4 ga = F.nodes.geo[A];
-    const int4 la = F.nodes.link[A];
-    const int nt = min(T, la.x + la.y - i0);
-    const float unit = F.unit[F.nodes.sys[A]];
-    const float eps2 = F.eps2;
-    float4 me[T];
-    float ax[T], ay[T], az[T], ph[T];
+    __shared__ float4 stage_s[P2P_WAVES][WAVE];
+    __shared__ int stage_i[P2P_WAVES][WAVE];
+    const int wslot = threadIdx.x / WAVE;
+    const int lane = lane_id();
+    float4* stage = stage_s[wslot];
+    int* sidx = stage_i[wslot];
+    const uint32_t nw = gridDim.x * P2P_WAVES;
+    for (uint32_t w = blockIdx.x * P2P_WAVES + wslot; w < num_leaves; w += nw) {
+        const int A = (int)F.leaf_ids[w];
+        const int4 ga = F.nodes.geo[A];
+        const int4 la = F.nodes.link[A];
+        const float unit = F.unit[F.nodes.sys[A]];
+        const float eps2 = F.eps2;
+        const unsigned long long e_begin = F.p2p_off[A], e_end = F.p2p_off[A + 1];
+        for (int t0 = 0; t0 < la.y; t0 += WAVE) {
+            const int nt = min(WAVE, la.y - t0);
+            const int splits = WAVE / nt;
+            const int ti = lane % nt;
+            const int sp = lane / nt;
+            const int i = la.x + t0 + ti;
+            const float4 me = F.pf[i];
+            float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+            for (unsigned long long eb = e_begin; eb < e_end; eb += WAVE) {
+                const int ne = (int)min((unsigned long long)WAVE, e_end - eb);
+                int cnt = 0, beg = 0;
+                float dx = 0.f, dy = 0.f, dz = 0.f;
+                if (lane < ne) {
+                    const int B = F.p2p_src[eb + lane];
+                    const int4 lb = F.nodes.link[B];
+                    const float3 d = centre_diff(ga, F.nodes.geo[B], F.bits, unit);
+                    cnt = lb.y;
+                    beg = lb.x;
+                    dx = d.x; dy = d.y; dz = d.z;
+                }
+                // inclusive prefix of the counts across the wave
+                int incl = cnt;
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
-        me[t] = F.pf[i0 + min(t, nt - 1)];
-        ax[t] = ay[t] = az[t] = ph[t] = 0.f;
-    }
-    const unsigned long long e0 = F.p2p_off[A], e1 = F.p2p_off[A + 1];
-    for (unsigned long long e = e0; e < e1; ++e) {
-        const int B = F.p2p_src[e];
-        const int4 gb = F.nodes.geo[B];
-        const int4 lb = F.nodes.link[B];
-        const float3 d = centre_diff(ga, gb, F.bits, unit);
-        float dx[T], dy[T], dz[T];
+                for (int o = 1; o < WAVE; o <<= 1) {
+                    const int y = __shfl_up(incl, o, WAVE);
+                    if (lane >= o) incl += y;
+                }
+                const int total = __shfl(incl, WAVE - 1, WAVE);
+                const int excl = incl - cnt;
+                for (int f0 = 0; f0 < total; f0 += WAVE) {
+                    const int fidx = f0 + lane;
+                    // entry k: the last lane with excl <= fidx (binary search over lanes)
+                    int lo = 0, hi = ne - 1;
 #pragma unroll
-        for (int t = 0; t < T; ++t) { dx[t] = d.x - me[t].x; dy[t] = d.y - me[t].y; dz[t] = d.z - me[t].z; }
-        const float4* src = F.pf + lb.x;
-        if (B != A) {
-#pragma unroll 2
-            for (int j = 0; j < lb.y; ++j) {
-                const float4 s = src[j];
-#pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    const float rx = dx[t] + s.x, ry = dy[t] + s.y, rz = dz[t] + s.z;
-                    const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
-                    const float inv = rsqrtf(r2);
-                    const float qi = s.w * inv;
-                    const float qi3 = qi * inv * inv;
-                    ax[t] = fmaf(rx, qi3, ax[t]);
-                    ay[t] = fmaf(ry, qi3, ay[t]);
-                    az[t] = fmaf(rz, qi3, az[t]);
-                    ph[t] += qi;
+                    for (int step = 0; step < 6; ++step) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        const int em = __shfl(excl, mid, WAVE);
+                        if (em <= fidx) lo = mid; else hi = mid - 1;
+                    }
+                    const int kb = __shfl(beg, lo, WAVE), ke = __shfl(excl, lo, WAVE);
+                    const float kx = __shfl(dx, lo, WAVE), ky = __shfl(dy, lo, WAVE), kz = __shfl(dz, lo, WAVE);
+                    if (fidx < total) {
+                        const int src = kb + (fidx - ke);
+                        const float4 s = F.pf[src];
+                        stage[lane] = make_float4(kx + s.x, ky + s.y, kz + s.z, s.w);
+                        sidx[lane] = src;
+                    }
+                    wave_lds_sync();
+                    const int nchunk = min(WAVE, total - f0);
+                    if (sp < splits) {
+                        if (eps2 > 0.f) {
+                            // the self pair has r = 0 exactly: no force, potential
+                            // q_i/eps (removed after the loop)
+#pragma unroll 4
+                            for (int j = sp; j < nchunk; j += splits) {
+                                const float4 s = stage[j];
+                                const float rx = s.x - me.x, ry = s.y - me.y, rz = s.z - me.z;
+                                const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+                                const float inv = rsqrtf(r2);
+                                const float qi = s.w * inv;
+                                const float qi3 = qi * inv * inv;
+                                ax = fmaf(rx, qi3, ax);
+                                ay = fmaf(ry, qi3, ay);
+                                az = fmaf(rz, qi3, az);
+                                ph += qi;
+                            }
+                        } else {
+                            for (int j = sp; j < nchunk; j += splits) {
+                                const float4 s = stage[j];
+                                const float rx = s.x - me.x, ry = s.y - me.y, rz = s.z - me.z;
+                                const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+                                const float inv = sidx[j] == i ? 0.f : rsqrtf(r2);
+                                const float qi = s.w * inv;
+                                const float qi3 = qi * inv * inv;
+                                ax = fmaf(rx, qi3, ax);
+                                ay = fmaf(ry, qi3, ay);
+                                az = fmaf(rz, qi3, az);
+                                ph += qi;
+                            }
+                        }
+                    }
+                    wave_lds_sync();
                 }
             }
-        } else {
-            const int self0 = i0 - lb.x;
-            for (int j = 0; j < lb.y; ++j) {
-                const float4 s = src[j];
-#pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    const float rx = dx[t] + s.x, ry = dy[t] + s.y, rz = dz[t] + s.z;
-                    const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
-                    const float inv = j == self0 + min(t, nt - 1) ? 0.f : rsqrtf(r2);
-                    const float qi = s.w * inv;
-                    const float qi3 = qi * inv * inv;
-                    ax[t] = fmaf(rx, qi3, ax[t]);
-                    ay[t] = fmaf(ry, qi3, ay[t]);
-                    az[t] = fmaf(rz, qi3, az[t]);
-                    ph[t] += qi;
+            // the self term q_i/eps sits in exactly one stride's partial sum
+            if (eps2 > 0.f && sp == 0) ph -= me.w * rsqrtf(eps2);
+            // combine the source strides of each target
+            {
+                const float ax0 = ax, ay0 = ay, az0 = az, ph0 = ph;
+                for (int q = 1; q < splits; ++q) {
+                    const int srcl = ti + q * nt;
+                    const float ox = __shfl(ax0, srcl, WAVE), oy = __shfl(ay0, srcl, WAVE);
+                    const float oz = __shfl(az0, srcl, WAVE), ow = __shfl(ph0, srcl, WAVE);
+                    ax += ox; ay += oy; az += oz; ph += ow;
                 }
             }
-        }
-    }
-    float Lr[NC];
-    {
-        const float* Lsrc = F.L + (size_t)A * F.ncp;
+            if (sp == 0) {
+                float lphi, gx, gy, gz;
+                const float* Lsrc = F.L + (size_t)A * F.ncp;
+                float Lr[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) Lr[k] = Lsrc[k];
-    }
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-        if (t >= nt) break;
-        const int i = i0 + t;
-        float lphi, gx, gy, gz;
-        l2p_eval<P>(me[t].x, me[t].y, me[t].z, Lr, lphi, gx, gy, gz);
-        const float fac = (float)(S.charge[i] / S.mass[i]) * F.G;
-        const float axo = fac * (ax[t] + gx), ayo = fac * (ay[t] + gy), azo = fac * (az[t] + gz);
-        S.ax[i] = axo;
-        S.ay[i] = ayo;
-        S.az[i] = azo;
-        S.pot[i] = -F.G * (ph[t] + lphi);
-        if (kick_dt != 0.0) {
-            S.vx[i] += (double)axo * kick_dt;
-            S.vy[i] += (double)ayo * kick_dt;
-            S.vz[i] += (double)azo * kick_dt;
+                for (int k = 0; k < NC; ++k) Lr[k] = Lsrc[k];
+                l2p_eval<P>(me.x, me.y, me.z, Lr, lphi, gx, gy, gz);
+                const float fac = (float)(S.charge[i] / S.mass[i]) * F.G;
+                const float axo = fac * (ax + gx), ayo = fac * (ay + gy), azo = fac * (az + gz);
+                S.ax[i] = axo;
+                S.ay[i] = ayo;
+                S.az[i] = azo;
+                S.pot[i] = -F.G * (ph + lphi);
+                if (kick_dt != 0.0) {
+                    S.vx[i] += (double)axo * kick_dt;
+                    S.vy[i] += (double)ayo * kick_dt;
+                    S.vz[i] += (double)azo * kick_dt;
+                }
+            }
         }
     }
 }
@@ -468,10 +508,10 @@ static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves
 }
 
 template <int P>
-static void launch_p2p_multi_t(const FmmParams& F, const State& S, uint32_t max_lanes, const uint32_t* lanes_total,
-                               const uint32_t* lane_leaf, const uint32_t* lane_first, double kick_dt, hipStream_t st) {
-    hipLaunchKernelGGL((p2p_multi_kernel<P>), dim3(ceil_div(max_lanes, 256)), dim3(256), 0, st, F, S, lanes_total,
-                       lane_leaf, lane_first, kick_dt);
+static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt,
+                                hipStream_t st) {
+    const unsigned grid = std::min<unsigned>(ceil_div(num_leaves, P2P_WAVES), 65536u);
+    hipLaunchKernelGGL((p2p_staged_kernel<P>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S, num_leaves, kick_dt);
 }
 template <int P>
 static void launch_dtable_t(const FmmParams& F, int level0, int nlev, int K, hipStream_t st) {
@@ -514,13 +554,10 @@ void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint
     if (!count) return;
     NBH_DISPATCH_P(p, launch_l2l_t<PP>(F, count, off, st));
 }
-void launch_p2p_multi(int p, const FmmParams& F, const State& S, uint32_t num_leaves, uint32_t max_lanes,
-                      const uint32_t* leaf_lanes_off, const uint32_t* lanes_total, uint32_t* lane_leaf,
-                      uint32_t* lane_first, double kick_dt, hipStream_t st) {
+void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt,
+                       hipStream_t st) {
     if (!num_leaves) return;
-    hipLaunchKernelGGL(p2p_lane_map_kernel, dim3(ceil_div(num_leaves, 256)), dim3(256), 0, st, F.leaf_ids,
-                       leaf_lanes_off, num_leaves, F.nodes, lane_leaf, lane_first);
-    NBH_DISPATCH_P(p, launch_p2p_multi_t<PP>(F, S, max_lanes, lanes_total, lane_leaf, lane_first, kick_dt, st));
+    NBH_DISPATCH_P(p, launch_p2p_staged_t<PP>(F, S, num_leaves, kick_dt, st));
 }
 void launch_dtable(int p, const FmmParams& F, int level0, int nlev, int K, hipStream_t st) {
     if (nlev <= 0) return;

@@ -36,6 +36,12 @@ void launch_tree_link(const uint64_t* keys, uint32_t n, const uint32_t* sys_off,
                       NodeView nodes, hipStream_t st);
 
 // ---- walk (walk.hip)
+// Per-wave chunk sizes of the walk's segment pools (entries).  A wave reserves
+// a chunk with one atomic and carves row segments from it; the host sizes the
+// pools for one unused tail per wave per depth.
+constexpr uint32_t POOL_CHUNK_M = 2048, POOL_CHUNK_P = 512, POOL_CHUNK_S = 512;
+constexpr uint32_t WALK_WAVES = 4;
+
 struct WalkParams {
     NodeView nodes;
     Counters* C;
@@ -56,6 +62,14 @@ struct WalkParams {
     int* m2l_src;
     int* p2p_src;
     unsigned long long m2l_cap, p2p_cap;
+    // phase-1 segment pools and per-row segments
+    int* m2l_pool;
+    int* p2p_pool;
+    unsigned long long m2l_pool_cap, p2p_pool_cap;
+    uint32_t* row_m_start;
+    uint32_t* row_m_cnt;
+    uint32_t* row_p_start;
+    uint32_t* row_p_cnt;
 };
 void launch_walk_phase1(const WalkParams& W, int s, uint32_t grid, hipStream_t st);
 void launch_walk_phase2(const WalkParams& W, int s, uint32_t grid, hipStream_t st);
@@ -86,7 +100,6 @@ struct FmmParams {
     const unsigned long long* p2p_off;
     const int* p2p_src;
     uint32_t* pleaf;       // per particle: its leaf node
-    uint32_t* leaf_lanes;  // per leaf (leaf order): lanes of the multi-target P2P
     float* dtab;           // per-level derivative tensors of same-level offsets (or null)
     int dK, dlevel0, dnlev;
     double eps2_64;
@@ -98,10 +111,8 @@ void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint
 void launch_m2l_plain(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st);
 void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st);
-constexpr int P2P_TPL = 2;  // target particles per lane in p2p_multi_kernel
-void launch_p2p_multi(int p, const FmmParams& F, const State& S, uint32_t num_leaves, uint32_t max_lanes,
-                      const uint32_t* leaf_lanes_off, const uint32_t* lanes_total, uint32_t* lane_leaf,
-                      uint32_t* lane_first, double kick_dt, hipStream_t st);
+void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt,
+                       hipStream_t st);
 void launch_dtable(int p, const FmmParams& F, int level0, int nlev, int K, hipStream_t st);
 void launch_m2l_grouped(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st);
 

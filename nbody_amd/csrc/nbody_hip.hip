@@ -93,10 +93,10 @@ struct nbh_handle {
     int cur = 0;
     DevBuf sys_off, keys, keys_alt, perm, perm_alt, sort_tmp, tree_tmp, scan_tmp;
     DevBuf box, inv, unit, bbox_part;
-    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab, leaf_lanes, leaf_lanes_off, lane_leaf,
-        lane_first;
+    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab;
     DevBuf row_node, row_src, row_sl_start, row_sl_cnt, sl, m2l_cnt, p2p_cnt, m2l_off, p2p_off, m2l_cur, p2p_cur,
-        m2l_src, p2p_src;
+        m2l_src, p2p_src, row_m_start, row_m_cnt, row_p_start, row_p_cnt, m2l_pool, p2p_pool;
+    unsigned long long m2l_pool_cap = 0, p2p_pool_cap = 0;
     DevBuf counters, diag, tmp_small;
     Counters* hc = nullptr;  // pinned host copy
     DevBuf Mw, Lw, pw, wbuf, gbuf;   // adjoint
@@ -300,25 +300,50 @@ static FmmParams fmm_params(nbh_handle* h) {
     F.p2p_off = h->p2p_off.as<unsigned long long>();
     F.p2p_src = h->p2p_src.as<int>();
     F.pleaf = h->pleaf.as<uint32_t>();
-    F.leaf_lanes = h->leaf_lanes.as<uint32_t>();
     F.dtab = nullptr;
     F.eps2_64 = h->cfg.eps * h->cfg.eps;
     return F;
 }
 
+static int grow_rows(nbh_handle* h, uint32_t cap) {
+    h->row_cap = cap;
+    DevBuf* rows[] = {&h->row_node, &h->row_src, &h->row_sl_start, &h->row_sl_cnt,
+                      &h->row_m_start, &h->row_m_cnt, &h->row_p_start, &h->row_p_cnt};
+    for (DevBuf* b : rows) HIPCHK(h, b->ensure((size_t)cap * 4));
+    return NBH_OK;
+}
+
 static int run_walk(nbh_handle* h) {
     Counters* dC = h->counters.as<Counters>();
     const uint32_t nn = h->num_nodes;
-    if (h->row_cap < 2 * nn + 1024) {
-        h->row_cap = 2 * nn + 1024;
-        HIPCHK(h, h->row_node.ensure((size_t)h->row_cap * 4));
-        HIPCHK(h, h->row_src.ensure((size_t)h->row_cap * 4));
-        HIPCHK(h, h->row_sl_start.ensure((size_t)h->row_cap * 4));
-        HIPCHK(h, h->row_sl_cnt.ensure((size_t)h->row_cap * 4));
+    if (h->row_cap < 2 * nn + 1024) CHK(grow_rows(h, 2 * nn + 1024));
+    // segment pools: sized from the last evaluation (or a per-node guess) and
+    // grown on overflow
+    // phase 1 grid per depth: about what stays resident (LDS-limited), so that
+    // the per-wave pool chunks waste little; every wave may leave one unused
+    // chunk tail per depth
+    uint32_t grid1[MAX_LEVELS + 1];
+    unsigned long long waves = 0;
+    for (uint32_t s = 0; s <= h->depth; ++s) {
+        const uint32_t rows = h->hc->level_count[s];
+        grid1[s] = std::max<uint32_t>(std::min<uint32_t>(ceil_div(rows, WALK_WAVES) + 16, 512), 64);
+        waves += std::min<unsigned long long>((unsigned long long)grid1[s] * WALK_WAVES, nn);
     }
-    if (h->sl_cap < 64u * nn + 4096) {
-        h->sl_cap = 64u * nn + 4096;
+    const unsigned long long waste_m = waves * POOL_CHUNK_M, waste_p = waves * POOL_CHUNK_P;
+    const unsigned long long waste_s = waves * POOL_CHUNK_S;
+    if (h->sl_cap < 64ull * nn + waste_s) {
+        h->sl_cap = (uint32_t)std::min<unsigned long long>(64ull * nn + waste_s, 0xffff0000ull);
         HIPCHK(h, h->sl.ensure((size_t)h->sl_cap * 4));
+    }
+    const unsigned long long mguess = std::max<unsigned long long>(h->m2l_cap, 256ull * nn) + waste_m;
+    const unsigned long long pguess = std::max<unsigned long long>(h->p2p_cap, 64ull * nn) + waste_p;
+    if (h->m2l_pool_cap < mguess) {
+        h->m2l_pool_cap = mguess;
+        HIPCHK(h, h->m2l_pool.ensure((size_t)h->m2l_pool_cap * 4));
+    }
+    if (h->p2p_pool_cap < pguess) {
+        h->p2p_pool_cap = pguess;
+        HIPCHK(h, h->p2p_pool.ensure((size_t)h->p2p_pool_cap * 4));
     }
     WalkParams W{};
     W.nodes = node_view(h);
@@ -326,25 +351,29 @@ static int run_walk(nbh_handle* h) {
     W.bits = h->bits;
     W.num4 = 4u * h->theta_num * h->theta_num;
     W.den3 = 3u * h->theta_den * h->theta_den;
-    for (int attempt = 0; attempt < 8; ++attempt) {
+    for (int attempt = 0; attempt < 12; ++attempt) {
         W.row_node = h->row_node.as<int>();
         W.row_src = h->row_src.as<int>();
         W.row_sl_start = h->row_sl_start.as<uint32_t>();
         W.row_sl_cnt = h->row_sl_cnt.as<uint32_t>();
+        W.row_m_start = h->row_m_start.as<uint32_t>();
+        W.row_m_cnt = h->row_m_cnt.as<uint32_t>();
+        W.row_p_start = h->row_p_start.as<uint32_t>();
+        W.row_p_cnt = h->row_p_cnt.as<uint32_t>();
         W.row_cap = h->row_cap;
         W.sl = h->sl.as<int>();
         W.sl_cap = h->sl_cap;
+        W.m2l_pool = h->m2l_pool.as<int>();
+        W.p2p_pool = h->p2p_pool.as<int>();
+        W.m2l_pool_cap = h->m2l_pool_cap;
+        W.p2p_pool_cap = h->p2p_pool_cap;
         W.m2l_cnt = h->m2l_cnt.as<uint32_t>();
         W.p2p_cnt = h->p2p_cnt.as<uint32_t>();
         zero_u32(h->m2l_cnt.p, nn, h->stream);
         zero_u32(h->p2p_cnt.p, nn, h->stream);
         // reset the walk part of the counters
         zero_u32(&dC->carried[0], (offsetof(Counters, nonfinite) - offsetof(Counters, carried)) / 4, h->stream);
-        for (uint32_t s = 0; s <= h->depth; ++s) {
-            const uint32_t rows = h->hc->level_count[s];
-            const uint32_t grid = std::max<uint32_t>(std::min<uint32_t>(ceil_div(rows, 4) + 64, 8192), 256);
-            launch_walk_phase1(W, (int)s, grid, h->stream);
-        }
+        for (uint32_t s = 0; s <= h->depth; ++s) launch_walk_phase1(W, (int)s, grid1[s], h->stream);
         unsigned long long* mo = h->m2l_off.as<unsigned long long>();
         unsigned long long* po = h->p2p_off.as<unsigned long long>();
         exclusive_scan<uint32_t, unsigned long long>(h->m2l_cnt.as<uint32_t>(), mo, nullptr, nn, mo + nn,
@@ -355,22 +384,31 @@ static int run_walk(nbh_handle* h) {
         HIPCHK(h, hipMemcpyAsync(&dC->p2p_total, po + nn, 8, hipMemcpyDeviceToDevice, h->stream));
         CHK(read_counters(h));
         const uint32_t ovf = h->hc->overflow;
+        const uint32_t walk_bits = OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P;
         if (ovf & OVF_SL) {
-            h->sl_cap = std::max<uint32_t>(h->sl_cap * 2, h->hc->sl_used + h->hc->sl_used / 4);
+            h->sl_cap = (uint32_t)std::min<unsigned long long>(
+                std::max<unsigned long long>(2ull * h->sl_cap, h->hc->sl_used + h->hc->sl_used / 4 + waste_s), 0xffff0000ull);
             HIPCHK(h, h->sl.ensure((size_t)h->sl_cap * 4));
         }
         if (ovf & OVF_ROWS) {
             uint64_t need = 0;
             for (uint32_t s = 0; s <= h->depth + 1; ++s) need += h->hc->level_count[s] + h->hc->carried[s];
-            h->row_cap = (uint32_t)std::max<uint64_t>((uint64_t)h->row_cap * 2, need + need / 4);
-            HIPCHK(h, h->row_node.ensure((size_t)h->row_cap * 4));
-            HIPCHK(h, h->row_src.ensure((size_t)h->row_cap * 4));
-            HIPCHK(h, h->row_sl_start.ensure((size_t)h->row_cap * 4));
-            HIPCHK(h, h->row_sl_cnt.ensure((size_t)h->row_cap * 4));
+            CHK(grow_rows(h, (uint32_t)std::max<uint64_t>((uint64_t)h->row_cap * 2, need + need / 4)));
         }
-        if (ovf & (OVF_SL | OVF_ROWS)) {
-            uint32_t keep = ovf & ~(uint32_t)(OVF_SL | OVF_ROWS);
+        if (ovf & OVF_M2L) {
+            h->m2l_pool_cap = std::max<unsigned long long>(h->m2l_pool_cap * 2,
+                                                           h->hc->m2l_total + h->hc->m2l_total / 4 + waste_m);
+            HIPCHK(h, h->m2l_pool.ensure((size_t)h->m2l_pool_cap * 4));
+        }
+        if (ovf & OVF_P2P) {
+            h->p2p_pool_cap = std::max<unsigned long long>(h->p2p_pool_cap * 2,
+                                                           h->hc->p2p_total + h->hc->p2p_total / 4 + waste_p);
+            HIPCHK(h, h->p2p_pool.ensure((size_t)h->p2p_pool_cap * 4));
+        }
+        if (ovf & walk_bits) {
+            uint32_t keep = ovf & ~walk_bits;
             HIPCHK(h, hipMemcpyAsync(&dC->overflow, &keep, 4, hipMemcpyHostToDevice, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
             continue;
         }
         break;
@@ -439,17 +477,8 @@ static int force_fmm(nbh_handle* h, double kick_dt) {
     for (uint32_t l = 1; l <= h->depth; ++l)
         launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
     record(h, 5);
-    if (h->cfg.p2p == NBH_P2P_LANE) {
-        // lanes per leaf -> offsets (total in the last slot)
-        uint32_t* lo = h->leaf_lanes_off.as<uint32_t>();
-        exclusive_scan<uint32_t, uint32_t>(h->leaf_lanes.as<uint32_t>(), lo, nullptr, h->num_leaves,
-                                           lo + h->num_leaves, h->scan_tmp.p, h->stream);
-        const uint32_t max_lanes = (uint32_t)std::min<size_t>(h->n, h->n / P2P_TPL + h->num_leaves);
-        launch_p2p_multi(p, F, S, h->num_leaves, max_lanes, lo, lo + h->num_leaves, h->lane_leaf.as<uint32_t>(),
-                         h->lane_first.as<uint32_t>(), kick_dt, h->stream);
-    } else {
-        launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
-    }
+    if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_staged(p, F, S, h->num_leaves, kick_dt, h->stream);
+    else launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
     record(h, 6);
     h->lastF = F;
     HIPCHK(h, hipGetLastError());
@@ -633,10 +662,6 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
     CCHK(h->leaf_ids.ensure(n * 4));
     CCHK(h->pf.ensure(n * 16));
     CCHK(h->pleaf.ensure(n * 4));
-    CCHK(h->leaf_lanes.ensure(n * 4));
-    CCHK(h->leaf_lanes_off.ensure((n + 1) * 4));
-    CCHK(h->lane_leaf.ensure(n * 4));
-    CCHK(h->lane_first.ensure(n * 4));
     CCHK(h->counters.ensure(sizeof(Counters)));
     CCHK(h->diag.ensure(1024 * 8 * 8));
     CCHK(h->tmp_small.ensure(64));
@@ -701,8 +726,9 @@ void nbh_destroy(nbh_handle* h) {
     h->st[1].release();
     DevBuf* all[] = {&h->sys_off, &h->keys, &h->keys_alt, &h->perm, &h->perm_alt, &h->sort_tmp, &h->tree_tmp,
                      &h->scan_tmp, &h->box, &h->inv, &h->unit, &h->bbox_part, &h->geo, &h->link, &h->nsys_of,
-                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->leaf_lanes, &h->leaf_lanes_off, &h->lane_leaf, &h->lane_first, &h->row_node, &h->row_src,
-                     &h->row_sl_start, &h->row_sl_cnt, &h->sl, &h->m2l_cnt, &h->p2p_cnt, &h->m2l_off,
+                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->row_node, &h->row_src,
+                     &h->row_sl_start, &h->row_sl_cnt, &h->row_m_start, &h->row_m_cnt, &h->row_p_start,
+                     &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->m2l_cnt, &h->p2p_cnt, &h->m2l_off,
                      &h->p2p_off, &h->m2l_cur, &h->p2p_cur, &h->m2l_src, &h->p2p_src, &h->counters, &h->diag,
                      &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf};
     for (DevBuf* b : all) b->release();

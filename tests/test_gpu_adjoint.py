@@ -47,7 +47,7 @@ def test_force_vjp_matches_oracle_transpose(kind, n, cap, p):
 
 
 def test_trajectory_gradient_against_direct_adjoint_and_fd():
-    """C4 scaled down: Plummer N=2048, eps=1e-2, theta=0.5, p=4, 40 KDK steps,
+    """Trajectory gradient, Plummer N=2048, eps=1e-2, theta=0.5, p=4, 40 KDK steps,
     checkpoints every 10.  Loss = mean |x_N - x*|^2 with x* from initial
     velocities perturbed by sigma=1e-3 noise.  GPU FMM adjoint vs the FP64
     direct-sum adjoint; FP64 central differences (h=1e-5) of the direct-sum

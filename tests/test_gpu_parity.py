@@ -134,7 +134,7 @@ def test_fmm_converges_to_direct(p, err):
 
 
 def test_kdk_trajectory_c1():
-    """C1: N=4096 uniform (seed 0), eps=1e-2, theta=0.5, cap 16, 10 KDK steps."""
+    """Uniform cube N=4096 (seed 0), eps=1e-2, theta=0.5, cap 16, 10 KDK steps."""
     _case("c1_kdk_10_steps")
     n = 4096
     x, v, m = nb.ics.uniform(n, 0)

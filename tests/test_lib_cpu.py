@@ -153,3 +153,12 @@ def test_ensemble_ics():
     assert off[0] == 0 and off[-1] == len(x) and np.all(np.diff(off) >= 512)
     for s in range(8):
         assert np.isclose(m[off[s]:off[s + 1]].sum(), 1.0)
+
+
+def test_cli_fails_cleanly_without_device():
+    exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "nbody_cli")
+    assert os.path.exists(exe)
+    if nb.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    r = subprocess.run([exe, "--n", "10"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "no HIP device" in r.stderr
