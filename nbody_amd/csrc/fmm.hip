@@ -115,16 +115,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     float L[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) L[k] = 0.f;
-    const unsigned long long e0 = F.m2l_off[A], e1 = F.m2l_off[A + 1];
-    for (unsigned long long e = e0; e < e1; ++e) {
-        const int B = F.m2l_src[e];
-        const int4 gb = F.nodes.geo[B];
-        const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
-        float D[NC];
-        derivs<P>(d.x, d.y, d.z, F.eps2, D);
-        // source moments read straight from memory; the pair table is ordered
-        // by source coefficient so each M_n is loaded once and used at once
-        m2l_apply<P>(D, F.M + (size_t)B * F.ncp, L);
+    for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
+        const uint32_t s0 = F.lists.m_start[r], n = F.lists.m_cnt[r];
+        for (uint32_t e = 0; e < n; ++e) {
+            const int B = F.lists.m_pool[s0 + e];
+            const int4 gb = F.nodes.geo[B];
+            const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
+            float D[NC];
+            derivs<P>(d.x, d.y, d.z, F.eps2, D);
+            // source moments read straight from memory; the pair table is ordered
+            // by source coefficient so each M_n is loaded once and used at once
+            m2l_apply<P>(D, F.M + (size_t)B * F.ncp, L);
+        }
     }
     float* out = F.L + (size_t)A * F.ncp;
 #pragma unroll
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
     const int4 la = F.nodes.link[A];
     const float unit = F.unit[F.nodes.sys[A]];
     const float eps2 = F.eps2;
-    const unsigned long long e0 = F.p2p_off[A], e1 = F.p2p_off[A + 1];
+    unsigned long long inter = 0;
     for (int t0 = 0; t0 < la.y; t0 += WAVE) {
         const int nt = min(WAVE, la.y - t0);
         const int splits = WAVE / nt;
@@ -183,14 +185,16 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
         const int i = la.x + t0 + ti;
         const float4 me = F.pf[i];
         float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
-        for (unsigned long long e = e0; e < e1; ++e) {
-            const int B = F.p2p_src[e];
+        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r])
+        for (uint32_t e = F.lists.p_start[r], e1 = e + F.lists.p_cnt[r]; e < e1; ++e) {
+            const int B = F.lists.p_pool[e];
             const int4 gb = F.nodes.geo[B];
             const int4 lb = F.nodes.link[B];
             const float3 d = centre_diff(ga, gb, F.bits, unit);
             const float dx = d.x - me.x, dy = d.y - me.y, dz = d.z - me.z;
             const float4* src = F.pf + lb.x;
             const int jn = active ? lb.y : 0;
+            if (t0 == 0 && lane == 0) inter += (unsigned long long)la.y * (unsigned long long)(B == A ? lb.y - 1 : lb.y);
             if (B != A) {
                 for (int j = sp; j < jn; j += splits) {
                     const float4 s = src[j];
@@ -253,6 +257,7 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
             }
         }
     }
+    if (lane == 0 && inter) atomicAdd(F.p2p_inter, inter);
 }
 
 // ---------------------------------------------------------------------------
@@ -283,13 +288,13 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
     float4* stage = stage_s[wslot];
     int* sidx = stage_i[wslot];
     const uint32_t nw = gridDim.x * P2P_WAVES;
+    unsigned long long inter = 0;  // particle pairs, summed over the lanes at the end
     for (uint32_t w = blockIdx.x * P2P_WAVES + wslot; w < num_leaves; w += nw) {
         const int A = (int)F.leaf_ids[w];
         const int4 ga = F.nodes.geo[A];
         const int4 la = F.nodes.link[A];
         const float unit = F.unit[F.nodes.sys[A]];
         const float eps2 = F.eps2;
-        const unsigned long long e_begin = F.p2p_off[A], e_end = F.p2p_off[A + 1];
         for (int t0 = 0; t0 < la.y; t0 += WAVE) {
             const int nt = min(WAVE, la.y - t0);
             const int splits = WAVE / nt;
@@ -298,17 +303,19 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
             const int i = la.x + t0 + ti;
             const float4 me = F.pf[i];
             float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
-            for (unsigned long long eb = e_begin; eb < e_end; eb += WAVE) {
-                const int ne = (int)min((unsigned long long)WAVE, e_end - eb);
+            for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r])
+            for (uint32_t eb = F.lists.p_start[r], e_end = eb + F.lists.p_cnt[r]; eb < e_end; eb += WAVE) {
+                const int ne = (int)min((uint32_t)WAVE, e_end - eb);
                 int cnt = 0, beg = 0;
                 float dx = 0.f, dy = 0.f, dz = 0.f;
                 if (lane < ne) {
-                    const int B = F.p2p_src[eb + lane];
+                    const int B = F.lists.p_pool[eb + lane];
                     const int4 lb = F.nodes.link[B];
                     const float3 d = centre_diff(ga, F.nodes.geo[B], F.bits, unit);
                     cnt = lb.y;
                     beg = lb.x;
                     dx = d.x; dy = d.y; dz = d.z;
+                    if (t0 == 0) inter += (unsigned long long)la.y * (unsigned long long)(B == A ? cnt - 1 : cnt);
                 }
                 // inclusive prefix of the counts across the wave
                 int incl = cnt;
@@ -407,6 +414,8 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
             }
         }
     }
+    inter = wave_sum(inter);
+    if (lane == 0 && inter) atomicAdd(F.p2p_inter, inter);
 }
 
 // ---------------------------------------------------------------------------
@@ -450,9 +459,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
         float L[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = 0.f;
-        const unsigned long long e0 = F.m2l_off[A], e1 = F.m2l_off[A + 1];
-        for (unsigned long long e = e0 + lane; e < e1; e += WAVE) {
-            const int B = F.m2l_src[e];
+        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r])
+        for (uint32_t e = F.lists.m_start[r] + lane, e1 = F.lists.m_start[r] + F.lists.m_cnt[r]; e < e1; e += WAVE) {
+            const int B = F.lists.m_pool[e];
             const int4 gb = F.nodes.geo[B];
             const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
             float D[NC];

@@ -42,37 +42,47 @@ void launch_tree_link(const uint64_t* keys, uint32_t n, const uint32_t* sys_off,
 constexpr uint32_t POOL_CHUNK_M = 2048, POOL_CHUNK_P = 512, POOL_CHUNK_S = 512;
 constexpr uint32_t WALK_WAVES = 4;
 
+// A node's interaction lists are the segments of its rows, one row per walk
+// depth that reached it: row_first[node] is its level row, row_next[row] the
+// row one depth deeper (a carried leaf), NO_ROW ends the chain.
+constexpr uint32_t NO_ROW = 0xffffffffu;
+
 struct WalkParams {
     NodeView nodes;
     Counters* C;
     int bits;
     uint32_t num4;  // 4 num^2
     uint32_t den3;  // 3 den^2
+    uint32_t row_cap;
     int* row_node;
-    int* row_src;
+    uint32_t* row_first;  // [nodes]
+    uint32_t* row_next;
     uint32_t* row_sl_start;
     uint32_t* row_sl_cnt;
-    uint32_t row_cap;
-    int* sl;
-    uint32_t sl_cap;
-    uint32_t* m2l_cnt;
-    uint32_t* p2p_cnt;
-    unsigned long long* m2l_cur;
-    unsigned long long* p2p_cur;
-    int* m2l_src;
-    int* p2p_src;
-    unsigned long long m2l_cap, p2p_cap;
-    // phase-1 segment pools and per-row segments
-    int* m2l_pool;
-    int* p2p_pool;
-    unsigned long long m2l_pool_cap, p2p_pool_cap;
     uint32_t* row_m_start;
     uint32_t* row_m_cnt;
     uint32_t* row_p_start;
     uint32_t* row_p_cnt;
+    int* sl;
+    uint32_t sl_cap;
+    int* m2l_pool;
+    int* p2p_pool;
+    unsigned long long m2l_pool_cap, p2p_pool_cap;
 };
-void launch_walk_phase1(const WalkParams& W, int s, uint32_t grid, hipStream_t st);
-void launch_walk_phase2(const WalkParams& W, int s, uint32_t grid, hipStream_t st);
+void launch_walk_root(const WalkParams& W, hipStream_t st);
+void launch_walk_group(const WalkParams& W, int s, uint32_t grid, hipStream_t st);
+
+// Read-only view of the lists for the force kernels.
+struct ListView {
+    const uint32_t* first;
+    const uint32_t* next;
+    const uint32_t* m_start;
+    const uint32_t* m_cnt;
+    const uint32_t* p_start;
+    const uint32_t* p_cnt;
+    const int* m_pool;
+    const int* p_pool;
+};
 
 // ---- particle state in sorted order (FP64 positions/velocities)
 struct State {
@@ -95,10 +105,8 @@ struct FmmParams {
     int ncp;               // padded coefficient stride
     const uint32_t* leaf_ids;
     float4* pf;            // per particle: offset from its leaf centre, charge
-    const unsigned long long* m2l_off;
-    const int* m2l_src;
-    const unsigned long long* p2p_off;
-    const int* p2p_src;
+    ListView lists;
+    unsigned long long* p2p_inter;  // statistics: particle pairs of the P2P kernels
     uint32_t* pleaf;       // per particle: its leaf node
     float* dtab;           // per-level derivative tensors of same-level offsets (or null)
     int dK, dlevel0, dnlev;

@@ -94,8 +94,8 @@ struct nbh_handle {
     DevBuf sys_off, keys, keys_alt, perm, perm_alt, sort_tmp, tree_tmp, scan_tmp;
     DevBuf box, inv, unit, bbox_part;
     DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab;
-    DevBuf row_node, row_src, row_sl_start, row_sl_cnt, sl, m2l_cnt, p2p_cnt, m2l_off, p2p_off, m2l_cur, p2p_cur,
-        m2l_src, p2p_src, row_m_start, row_m_cnt, row_p_start, row_p_cnt, m2l_pool, p2p_pool;
+    DevBuf row_node, row_first, row_next, row_sl_start, row_sl_cnt, sl, row_m_start, row_m_cnt, row_p_start,
+        row_p_cnt, m2l_pool, p2p_pool;
     unsigned long long m2l_pool_cap = 0, p2p_pool_cap = 0;
     DevBuf counters, diag, tmp_small;
     Counters* hc = nullptr;  // pinned host copy
@@ -103,7 +103,6 @@ struct nbh_handle {
     bool gravity = true;             // charge == mass for every particle
     FmmParams lastF{};               // parameters of the last FMM force evaluation
     uint32_t node_cap = 0, row_cap = 0, sl_cap = 0;
-    unsigned long long m2l_cap = 0, p2p_cap = 0;
     uint32_t num_nodes = 0, num_leaves = 0, depth = 0;
     nbh_stats stats{};
     hipEvent_t ev[8] = {};
@@ -209,12 +208,7 @@ static int alloc_nodes(nbh_handle* h, uint32_t need) {
     HIPCHK(h, h->node_begin.ensure((size_t)cap * 4));
     HIPCHK(h, h->M.ensure((size_t)cap * ncp * 4));
     HIPCHK(h, h->L.ensure((size_t)cap * ncp * 4));
-    HIPCHK(h, h->m2l_cnt.ensure((size_t)cap * 4));
-    HIPCHK(h, h->p2p_cnt.ensure((size_t)cap * 4));
-    HIPCHK(h, h->m2l_off.ensure((size_t)(cap + 1) * 8));
-    HIPCHK(h, h->p2p_off.ensure((size_t)(cap + 1) * 8));
-    HIPCHK(h, h->m2l_cur.ensure((size_t)(cap + 1) * 8));
-    HIPCHK(h, h->p2p_cur.ensure((size_t)(cap + 1) * 8));
+    HIPCHK(h, h->row_first.ensure((size_t)cap * 4));
     HIPCHK(h, h->scan_tmp.ensure(scan_tmp_bytes(cap + 1) + 256));
     h->node_cap = cap;
     return NBH_OK;
@@ -295,10 +289,15 @@ static FmmParams fmm_params(nbh_handle* h) {
     F.ncp = ncp_of((int)h->cfg.order);
     F.leaf_ids = h->leaf_ids.as<uint32_t>();
     F.pf = h->pf.as<float4>();
-    F.m2l_off = h->m2l_off.as<unsigned long long>();
-    F.m2l_src = h->m2l_src.as<int>();
-    F.p2p_off = h->p2p_off.as<unsigned long long>();
-    F.p2p_src = h->p2p_src.as<int>();
+    F.lists.first = h->row_first.as<uint32_t>();
+    F.lists.next = h->row_next.as<uint32_t>();
+    F.lists.m_start = h->row_m_start.as<uint32_t>();
+    F.lists.m_cnt = h->row_m_cnt.as<uint32_t>();
+    F.lists.p_start = h->row_p_start.as<uint32_t>();
+    F.lists.p_cnt = h->row_p_cnt.as<uint32_t>();
+    F.lists.m_pool = h->m2l_pool.as<int>();
+    F.lists.p_pool = h->p2p_pool.as<int>();
+    F.p2p_inter = &h->counters.as<Counters>()->p2p_inter;
     F.pleaf = h->pleaf.as<uint32_t>();
     F.dtab = nullptr;
     F.eps2_64 = h->cfg.eps * h->cfg.eps;
@@ -307,7 +306,7 @@ static FmmParams fmm_params(nbh_handle* h) {
 
 static int grow_rows(nbh_handle* h, uint32_t cap) {
     h->row_cap = cap;
-    DevBuf* rows[] = {&h->row_node, &h->row_src, &h->row_sl_start, &h->row_sl_cnt,
+    DevBuf* rows[] = {&h->row_node, &h->row_next, &h->row_sl_start, &h->row_sl_cnt,
                       &h->row_m_start, &h->row_m_cnt, &h->row_p_start, &h->row_p_cnt};
     for (DevBuf* b : rows) HIPCHK(h, b->ensure((size_t)cap * 4));
     return NBH_OK;
@@ -317,32 +316,31 @@ static int run_walk(nbh_handle* h) {
     Counters* dC = h->counters.as<Counters>();
     const uint32_t nn = h->num_nodes;
     if (h->row_cap < 2 * nn + 1024) CHK(grow_rows(h, 2 * nn + 1024));
-    // segment pools: sized from the last evaluation (or a per-node guess) and
-    // grown on overflow
-    // phase 1 grid per depth: about what stays resident (LDS-limited), so that
-    // the per-wave pool chunks waste little; every wave may leave one unused
+    // group-kernel grid per depth: about what stays resident, so that the
+    // per-wave pool chunks waste little; every wave may leave one unused
     // chunk tail per depth
-    uint32_t grid1[MAX_LEVELS + 1];
+    uint32_t grid[MAX_LEVELS + 1] = {};
     unsigned long long waves = 0;
-    for (uint32_t s = 0; s <= h->depth; ++s) {
-        const uint32_t rows = h->hc->level_count[s];
-        grid1[s] = std::max<uint32_t>(std::min<uint32_t>(ceil_div(rows, WALK_WAVES) + 16, 512), 64);
-        waves += std::min<unsigned long long>((unsigned long long)grid1[s] * WALK_WAVES, nn);
+    for (uint32_t s = 1; s <= h->depth; ++s) {
+        const uint32_t groups = h->hc->level_count[s - 1];
+        grid[s] = std::max<uint32_t>(std::min<uint32_t>(ceil_div(groups, WALK_WAVES) + 16, 1024), 64);
+        waves += std::min<unsigned long long>((unsigned long long)grid[s] * WALK_WAVES, nn);
     }
     const unsigned long long waste_m = waves * POOL_CHUNK_M, waste_p = waves * POOL_CHUNK_P;
     const unsigned long long waste_s = waves * POOL_CHUNK_S;
+    // pools: sized from the last evaluation (or a per-node guess) and grown
+    // on overflow
     if (h->sl_cap < 64ull * nn + waste_s) {
         h->sl_cap = (uint32_t)std::min<unsigned long long>(64ull * nn + waste_s, 0xffff0000ull);
         HIPCHK(h, h->sl.ensure((size_t)h->sl_cap * 4));
     }
-    const unsigned long long mguess = std::max<unsigned long long>(h->m2l_cap, 256ull * nn) + waste_m;
-    const unsigned long long pguess = std::max<unsigned long long>(h->p2p_cap, 64ull * nn) + waste_p;
+    const unsigned long long mguess = 256ull * nn + waste_m, pguess = 64ull * nn + waste_p;
     if (h->m2l_pool_cap < mguess) {
-        h->m2l_pool_cap = mguess;
+        h->m2l_pool_cap = std::min<unsigned long long>(mguess, 0xffff0000ull);
         HIPCHK(h, h->m2l_pool.ensure((size_t)h->m2l_pool_cap * 4));
     }
     if (h->p2p_pool_cap < pguess) {
-        h->p2p_pool_cap = pguess;
+        h->p2p_pool_cap = std::min<unsigned long long>(pguess, 0xffff0000ull);
         HIPCHK(h, h->p2p_pool.ensure((size_t)h->p2p_pool_cap * 4));
     }
     WalkParams W{};
@@ -352,42 +350,33 @@ static int run_walk(nbh_handle* h) {
     W.num4 = 4u * h->theta_num * h->theta_num;
     W.den3 = 3u * h->theta_den * h->theta_den;
     for (int attempt = 0; attempt < 12; ++attempt) {
+        W.row_cap = h->row_cap;
         W.row_node = h->row_node.as<int>();
-        W.row_src = h->row_src.as<int>();
+        W.row_first = h->row_first.as<uint32_t>();
+        W.row_next = h->row_next.as<uint32_t>();
         W.row_sl_start = h->row_sl_start.as<uint32_t>();
         W.row_sl_cnt = h->row_sl_cnt.as<uint32_t>();
         W.row_m_start = h->row_m_start.as<uint32_t>();
         W.row_m_cnt = h->row_m_cnt.as<uint32_t>();
         W.row_p_start = h->row_p_start.as<uint32_t>();
         W.row_p_cnt = h->row_p_cnt.as<uint32_t>();
-        W.row_cap = h->row_cap;
         W.sl = h->sl.as<int>();
         W.sl_cap = h->sl_cap;
         W.m2l_pool = h->m2l_pool.as<int>();
         W.p2p_pool = h->p2p_pool.as<int>();
         W.m2l_pool_cap = h->m2l_pool_cap;
         W.p2p_pool_cap = h->p2p_pool_cap;
-        W.m2l_cnt = h->m2l_cnt.as<uint32_t>();
-        W.p2p_cnt = h->p2p_cnt.as<uint32_t>();
-        zero_u32(h->m2l_cnt.p, nn, h->stream);
-        zero_u32(h->p2p_cnt.p, nn, h->stream);
         // reset the walk part of the counters
         zero_u32(&dC->carried[0], (offsetof(Counters, nonfinite) - offsetof(Counters, carried)) / 4, h->stream);
-        for (uint32_t s = 0; s <= h->depth; ++s) launch_walk_phase1(W, (int)s, grid1[s], h->stream);
-        unsigned long long* mo = h->m2l_off.as<unsigned long long>();
-        unsigned long long* po = h->p2p_off.as<unsigned long long>();
-        exclusive_scan<uint32_t, unsigned long long>(h->m2l_cnt.as<uint32_t>(), mo, nullptr, nn, mo + nn,
-                                                     h->scan_tmp.p, h->stream);
-        exclusive_scan<uint32_t, unsigned long long>(h->p2p_cnt.as<uint32_t>(), po, nullptr, nn, po + nn,
-                                                     h->scan_tmp.p, h->stream);
-        HIPCHK(h, hipMemcpyAsync(&dC->m2l_total, mo + nn, 8, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(&dC->p2p_total, po + nn, 8, hipMemcpyDeviceToDevice, h->stream));
+        launch_walk_root(W, h->stream);
+        for (uint32_t s = 1; s <= h->depth; ++s) launch_walk_group(W, (int)s, grid[s], h->stream);
         CHK(read_counters(h));
         const uint32_t ovf = h->hc->overflow;
         const uint32_t walk_bits = OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P;
         if (ovf & OVF_SL) {
             h->sl_cap = (uint32_t)std::min<unsigned long long>(
-                std::max<unsigned long long>(2ull * h->sl_cap, h->hc->sl_used + h->hc->sl_used / 4 + waste_s), 0xffff0000ull);
+                std::max<unsigned long long>(2ull * h->sl_cap, h->hc->sl_used + h->hc->sl_used / 4 + waste_s),
+                0xffff0000ull);
             HIPCHK(h, h->sl.ensure((size_t)h->sl_cap * 4));
         }
         if (ovf & OVF_ROWS) {
@@ -396,13 +385,13 @@ static int run_walk(nbh_handle* h) {
             CHK(grow_rows(h, (uint32_t)std::max<uint64_t>((uint64_t)h->row_cap * 2, need + need / 4)));
         }
         if (ovf & OVF_M2L) {
-            h->m2l_pool_cap = std::max<unsigned long long>(h->m2l_pool_cap * 2,
-                                                           h->hc->m2l_total + h->hc->m2l_total / 4 + waste_m);
+            h->m2l_pool_cap = std::min<unsigned long long>(
+                std::max<unsigned long long>(h->m2l_pool_cap * 2, h->hc->m2l_pool_used + waste_m), 0xffff0000ull);
             HIPCHK(h, h->m2l_pool.ensure((size_t)h->m2l_pool_cap * 4));
         }
         if (ovf & OVF_P2P) {
-            h->p2p_pool_cap = std::max<unsigned long long>(h->p2p_pool_cap * 2,
-                                                           h->hc->p2p_total + h->hc->p2p_total / 4 + waste_p);
+            h->p2p_pool_cap = std::min<unsigned long long>(
+                std::max<unsigned long long>(h->p2p_pool_cap * 2, h->hc->p2p_pool_used + waste_p), 0xffff0000ull);
             HIPCHK(h, h->p2p_pool.ensure((size_t)h->p2p_pool_cap * 4));
         }
         if (ovf & walk_bits) {
@@ -414,28 +403,6 @@ static int run_walk(nbh_handle* h) {
         break;
     }
     if (h->hc->overflow) return h->fail(NBH_E_INTERNAL, "walk buffers overflow (flags %u)", h->hc->overflow);
-    const unsigned long long nm = h->hc->m2l_total, np = h->hc->p2p_total;
-    if (nm > h->m2l_cap) {
-        h->m2l_cap = nm + nm / 4 + 1024;
-        HIPCHK(h, h->m2l_src.ensure((size_t)h->m2l_cap * 4));
-    }
-    if (np > h->p2p_cap) {
-        h->p2p_cap = np + np / 4 + 1024;
-        HIPCHK(h, h->p2p_src.ensure((size_t)h->p2p_cap * 4));
-    }
-    W.m2l_cur = h->m2l_cur.as<unsigned long long>();
-    W.p2p_cur = h->p2p_cur.as<unsigned long long>();
-    W.m2l_src = h->m2l_src.as<int>();
-    W.p2p_src = h->p2p_src.as<int>();
-    W.m2l_cap = h->m2l_cap;
-    W.p2p_cap = h->p2p_cap;
-    HIPCHK(h, hipMemcpyAsync(h->m2l_cur.p, h->m2l_off.p, (size_t)nn * 8, hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->p2p_cur.p, h->p2p_off.p, (size_t)nn * 8, hipMemcpyDeviceToDevice, h->stream));
-    for (uint32_t s = 0; s <= h->depth; ++s) {
-        const uint32_t rows = h->hc->level_count[s] + h->hc->carried[s];
-        const uint32_t grid = std::max<uint32_t>(std::min<uint32_t>(ceil_div(rows, 4), 8192), 1);
-        launch_walk_phase2(W, (int)s, grid, h->stream);
-    }
     return NBH_OK;
 }
 
@@ -726,10 +693,9 @@ void nbh_destroy(nbh_handle* h) {
     h->st[1].release();
     DevBuf* all[] = {&h->sys_off, &h->keys, &h->keys_alt, &h->perm, &h->perm_alt, &h->sort_tmp, &h->tree_tmp,
                      &h->scan_tmp, &h->box, &h->inv, &h->unit, &h->bbox_part, &h->geo, &h->link, &h->nsys_of,
-                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->row_node, &h->row_src,
-                     &h->row_sl_start, &h->row_sl_cnt, &h->row_m_start, &h->row_m_cnt, &h->row_p_start,
-                     &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->m2l_cnt, &h->p2p_cnt, &h->m2l_off,
-                     &h->p2p_off, &h->m2l_cur, &h->p2p_cur, &h->m2l_src, &h->p2p_src, &h->counters, &h->diag,
+                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->row_node, &h->row_first,
+                     &h->row_next, &h->row_sl_start, &h->row_sl_cnt, &h->row_m_start, &h->row_m_cnt,
+                     &h->row_p_start, &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->counters, &h->diag,
                      &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf};
     for (DevBuf* b : all) b->release();
     if (h->hc) (void)hipHostFree(h->hc);
@@ -982,14 +948,40 @@ int nbh_debug_tree(nbh_handle* h, int32_t* level, int32_t* sys, int32_t* ix, int
 int nbh_debug_lists(nbh_handle* h, int64_t* m2l_off, int32_t* m2l_src, int64_t* p2p_off, int32_t* p2p_src) {
     if (!h) return NBH_E_INVALID;
     HIPCHK(h, hipSetDevice(h->device));
+    // the device keeps each node's lists as a chain of row segments; here they
+    // are concatenated in chain (depth) order into CSR
     const size_t nn = h->num_nodes;
-    if (m2l_off) HIPCHK(h, hipMemcpyAsync(m2l_off, h->m2l_off.p, (nn + 1) * 8, hipMemcpyDeviceToHost, h->stream));
-    if (p2p_off) HIPCHK(h, hipMemcpyAsync(p2p_off, h->p2p_off.p, (nn + 1) * 8, hipMemcpyDeviceToHost, h->stream));
-    if (m2l_src && h->stats.m2l_pairs)
-        HIPCHK(h, hipMemcpyAsync(m2l_src, h->m2l_src.p, h->stats.m2l_pairs * 4, hipMemcpyDeviceToHost, h->stream));
-    if (p2p_src && h->stats.p2p_pairs)
-        HIPCHK(h, hipMemcpyAsync(p2p_src, h->p2p_src.p, h->stats.p2p_pairs * 4, hipMemcpyDeviceToHost, h->stream));
+    size_t rows = 0;
+    for (uint32_t s = 0; s <= h->depth + 1 && s <= (uint32_t)MAX_LEVELS; ++s)
+        rows += h->hc->level_count[s] + h->hc->carried[s];
+    rows = std::min<size_t>(rows, h->row_cap);
+    std::vector<uint32_t> first(nn), next(rows), ms(rows), mc(rows), ps(rows), pc(rows);
+    std::vector<int32_t> mpool(h->hc->m2l_pool_used), ppool(h->hc->p2p_pool_used);
+    struct { void* dst; const void* src; size_t bytes; } cp[] = {
+        {first.data(), h->row_first.p, nn * 4},     {next.data(), h->row_next.p, rows * 4},
+        {ms.data(), h->row_m_start.p, rows * 4},    {mc.data(), h->row_m_cnt.p, rows * 4},
+        {ps.data(), h->row_p_start.p, rows * 4},    {pc.data(), h->row_p_cnt.p, rows * 4},
+        {mpool.data(), h->m2l_pool.p, mpool.size() * 4}, {ppool.data(), h->p2p_pool.p, ppool.size() * 4}};
+    for (auto& c : cp)
+        if (c.bytes) HIPCHK(h, hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    int64_t om = 0, op = 0;
+    for (size_t a = 0; a < nn; ++a) {
+        if (m2l_off) m2l_off[a] = om;
+        if (p2p_off) p2p_off[a] = op;
+        for (uint32_t r = first[a]; r != NO_ROW; r = next[r]) {
+            if (r >= rows) return h->fail(NBH_E_INTERNAL, "row chain of node %zu leaves the row table", a);
+            if ((size_t)ms[r] + mc[r] > mpool.size() || (size_t)ps[r] + pc[r] > ppool.size() ||
+                (uint64_t)om + mc[r] > h->stats.m2l_pairs || (uint64_t)op + pc[r] > h->stats.p2p_pairs)
+                return h->fail(NBH_E_INTERNAL, "row segment of node %zu out of range", a);
+            for (uint32_t k = 0; k < mc[r]; ++k, ++om)
+                if (m2l_src) m2l_src[om] = mpool[ms[r] + k];
+            for (uint32_t k = 0; k < pc[r]; ++k, ++op)
+                if (p2p_src) p2p_src[op] = ppool[ps[r] + k];
+        }
+    }
+    if (m2l_off) m2l_off[nn] = om;
+    if (p2p_off) p2p_off[nn] = op;
     return NBH_OK;
 }
 

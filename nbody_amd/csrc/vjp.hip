@@ -74,9 +74,9 @@ __global__ __launch_bounds__(256) void vjp_final_kernel(FmmParams F, const float
     const float4 wi = pw[i];
     const float mi = me.w;
     float g0 = 0.f, g1 = 0.f, g2 = 0.f, gmm = 0.f;
-    const unsigned long long e0 = F.p2p_off[A], e1 = F.p2p_off[A + 1];
-    for (unsigned long long e = e0; e < e1; ++e) {
-        const int B = F.p2p_src[e];
+    for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r])
+    for (uint32_t e = F.lists.p_start[r], e1 = e + F.lists.p_cnt[r]; e < e1; ++e) {
+        const int B = F.lists.p_pool[e];
         const int4 gb = F.nodes.geo[B];
         const int4 lb = F.nodes.link[B];
         const float3 d = centre_diff(ga, gb, F.bits, unit);

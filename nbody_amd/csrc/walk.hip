@@ -17,11 +17,21 @@
 //   - A at level s:       expand(SL(parent(A)))
 //   - leaf A above s:     expand(SL(A))        (A stays, the source splits)
 // where SL(X) are the sources that X's pairs were split against and
-// expand(B) = children(B) if B has children, else {B}.  The walk runs one
-// depth at a time, one wavefront per target row; rows of depth s are the
-// level-s cells followed by the leaves carried from depth s-1.
+// expand(B) = children(B) if B has children, else {B}.
 //
-// Each target's lists come out in ascending source id without any sort:
+// Rows and groups.  The walk runs one depth at a time.  A row is one target at
+// one depth: rows of depth s are the level-s cells (in id order) followed by
+// the leaves carried from depth s-1.  Every row of depth s-1 is a group: its
+// candidate set expand(SL(row)) is shared by all of its targets at depth s --
+// the cell's children, or the carried leaf itself.  One wavefront takes one
+// group, loads each candidate once and classifies it against every target
+// (up to 8), in two passes: count, reserve, write.  Each row's M2L, P2P and
+// split entries land as contiguous segments carved from per-wave chunks of
+// three pools; a node's rows are chained in depth order (first row = its
+// level row, next = its carried row one depth deeper), and the force kernels
+// follow the chain, so no compaction pass is needed.
+//
+// Each target's entries come out in ascending source id, chain order included:
 // split lists are ascending, kept leaves (coarser levels) precede children,
 // and deeper depths only add deeper (larger-id) sources.
 //
@@ -32,28 +42,11 @@
 
 namespace nbh {
 
-
-
-__device__ __forceinline__ bool mac_approx(int A, int4 ga, int B, int4 gb, int bits, uint32_t num4,
-                                           uint32_t den3) {
-    if (A == B) return false;
-    const int la = ga.w & 0xff, lb = gb.w & 0xff;
-    const int sa = bits - la, sb = bits - lb;
-    const int dx = ((2 * gb.x + 1) << sb) - ((2 * ga.x + 1) << sa);
-    const int dy = ((2 * gb.y + 1) << sb) - ((2 * ga.y + 1) << sa);
-    const int dz = ((2 * gb.z + 1) << sb) - ((2 * ga.z + 1) << sa);
-    const uint64_t dsq = (uint64_t)((int64_t)dx * dx) + (uint64_t)((int64_t)dy * dy) + (uint64_t)((int64_t)dz * dz);
-    const uint64_t side = (1ull << (sa + 1)) + (1ull << (sb + 1));
-    return (uint64_t)den3 * side * side < (uint64_t)num4 * dsq;
-}
-
 __device__ __forceinline__ uint32_t depth_base(const Counters* C, int s) {
     uint32_t b = 0;
     for (int q = 0; q < s; ++q) b += C->level_count[q] + C->carried[q];
     return b;
 }
-
-enum { CAT_M2L = 0, CAT_P2P = 1, CAT_SL = 2, CAT_NONE = 3 };
 
 // LDS writes of one lane made visible to the other lanes of its wavefront
 __device__ __forceinline__ void walk_lds_sync() {
@@ -84,34 +77,20 @@ __device__ __forceinline__ void sl_cache_fill(const WalkParams& W, const SLCache
     walk_lds_sync();
 }
 
-// Enumerate the candidates of one row; f(lane_valid, candidate, geo, category)
-// is called once per chunk of 64 lanes, in ascending candidate order: first
-// the split-list entries without children (kept as themselves), then the
-// children of the others, entry by entry.
+// Enumerate expand(SL) for the split list at sl[st, st+cnt); f(valid, B, geo)
+// is called once per chunk of 64 lanes (uniformly), in ascending candidate
+// order: first the entries without children (kept as themselves), then the
+// children of the others, entry by entry.  `filled`: the cache already holds
+// the whole list (cnt <= SL_CACHE).
 template <typename F>
-__device__ __forceinline__ void enumerate(const WalkParams& W, const SLCache& cache, int A, int4 ga, int src,
-                                          F&& f) {
-    const bool a_leaf = (ga.w >> 8) == 0;
+__device__ __forceinline__ void enumerate_sl(const WalkParams& W, const SLCache& cache, uint32_t st, uint32_t cnt,
+                                             bool filled, F&& f) {
     const int lane = lane_id();
-    auto classify = [&](bool valid, int B, int4 gb) -> int {
-        if (!valid) return CAT_NONE;
-        if (mac_approx(A, ga, B, gb, W.bits, W.num4, W.den3)) return CAT_M2L;
-        const bool b_leaf = (gb.w >> 8) == 0;
-        if (a_leaf && b_leaf) return CAT_P2P;
-        return CAT_SL;
-    };
-    if (src < 0) {
-        const bool valid = lane == 0;
-        f(valid, A, ga, classify(valid, A, ga));
-        return;
-    }
-    const uint32_t st = W.row_sl_start[src];
-    uint32_t cnt = W.row_sl_cnt[src];
-    if ((unsigned long long)st + cnt > W.sl_cap) cnt = 0;  // only after an overflow
+    const bool one_block = cnt <= (uint32_t)SL_CACHE;
     // phase a: sources kept as themselves (childless)
     for (uint32_t p0 = 0; p0 < cnt; p0 += SL_CACHE) {
         const uint32_t pn = min((uint32_t)SL_CACHE, cnt - p0);
-        sl_cache_fill(W, cache, st, p0, pn);
+        if (!(filled && one_block)) sl_cache_fill(W, cache, st, p0, pn);
         for (uint32_t e0 = 0; e0 < pn; e0 += WAVE) {
             const uint32_t e = e0 + lane;
             int B = -1;
@@ -122,15 +101,15 @@ __device__ __forceinline__ void enumerate(const WalkParams& W, const SLCache& ca
             }
             const bool valid = B >= 0 && (gb.w >> 8) == 0;
             if (__ballot(valid) == 0) continue;
-            f(valid, B, gb, classify(valid, B, gb));
+            f(valid, B, gb);
         }
-        walk_lds_sync();
+        if (!one_block) walk_lds_sync();
     }
     // phase b: children of the sources that have children, 8 entries x 8
     // slots per chunk, two chunks in flight
     for (uint32_t p0 = 0; p0 < cnt; p0 += SL_CACHE) {
         const uint32_t pn = min((uint32_t)SL_CACHE, cnt - p0);
-        if (cnt > SL_CACHE) sl_cache_fill(W, cache, st, p0, pn);  // else still cached from phase a
+        if (!one_block) sl_cache_fill(W, cache, st, p0, pn);
         const int c = lane & 7;
         for (uint32_t e0 = 0; e0 < pn; e0 += 16) {
             bool v0 = false, v1 = false;
@@ -141,19 +120,12 @@ __device__ __forceinline__ void enumerate(const WalkParams& W, const SLCache& ca
             if (eb < pn && c < (cache.geo[eb].w >> 8)) { C1 = cache.fc[eb] + c; v1 = true; }
             if (v0) g0 = W.nodes.geo[C0];
             if (v1) g1 = W.nodes.geo[C1];
-            if (__ballot(v0)) f(v0, C0, g0, classify(v0, C0, g0));
-            if (__ballot(v1)) f(v1, C1, g1, classify(v1, C1, g1));
+            if (__ballot(v0)) f(v0, C0, g0);
+            if (__ballot(v1)) f(v1, C1, g1);
         }
-        walk_lds_sync();
+        if (!one_block) walk_lds_sync();
     }
 }
-
-// Phase 1: one enumeration per row.  The row's M2L, P2P and split entries
-// are buffered in LDS (per-wave regions), then written as three contiguous
-// segments carved from per-wave pool chunks; a row that overflows its LDS
-// region is enumerated a second time straight into the pools.  Per-target
-// counts (for the CSR offsets) and leaf carries as before.
-constexpr int CAP_M = 1536, CAP_P = 256, CAP_S = 256;
 
 // Take n entries from a wave's chunk {next, end}, reserving a new chunk of
 // max(n, chunk) entries from the pool when the current one is exhausted.
@@ -174,175 +146,251 @@ __device__ __forceinline__ uint32_t pool_take(uint2& ch, uint32_t n, uint32_t ch
     return r;
 }
 
-__global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_phase1_kernel(WalkParams W, int s) {
-    __shared__ int lbuf[WALK_WAVES][CAP_M + CAP_P + CAP_S];
+// Depth 0: one row per root (one per system), the candidate being the root
+// itself: a childless root is its own P2P source, otherwise (root, root) is
+// split.  One thread per root.
+__global__ __launch_bounds__(64) void walk_root_kernel(WalkParams W) {
+    Counters* C = W.C;
+    const uint32_t n_roots = C->level_count[0];
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_roots; r += gridDim.x * blockDim.x) {
+        if (r >= W.row_cap) {
+            atomicOr(&C->overflow, (uint32_t)OVF_ROWS);
+            continue;
+        }
+        const int A = (int)(C->level_off[0] + r);
+        const bool leaf = (W.nodes.geo[A].w >> 8) == 0;
+        W.row_first[A] = r;
+        W.row_next[r] = NO_ROW;
+        W.row_node[r] = A;
+        W.row_m_start[r] = 0;
+        W.row_m_cnt[r] = 0;
+        uint32_t pc = 0, ps = 0, sc = 0, ss = 0;
+        if (leaf) {
+            ps = atomicAdd(&C->p2p_pool_used, 1u);
+            if (ps < W.p2p_pool_cap) {
+                W.p2p_pool[ps] = A;
+                pc = 1;
+                atomicAdd(&C->p2p_total, 1ull);
+            } else {
+                atomicOr(&C->overflow, (uint32_t)OVF_P2P);
+                ps = 0;
+            }
+        } else {
+            ss = atomicAdd(&C->sl_used, 1u);
+            if (ss < W.sl_cap) {
+                W.sl[ss] = A;
+                sc = 1;
+                atomicAdd(&C->split_total, 1ull);
+            } else {
+                atomicOr(&C->overflow, (uint32_t)OVF_SL);
+                ss = 0;
+            }
+        }
+        W.row_p_start[r] = ps;
+        W.row_p_cnt[r] = pc;
+        W.row_sl_start[r] = ss;
+        W.row_sl_cnt[r] = sc;
+    }
+}
+
+constexpr int MAXT = 8;  // targets per group: the children of one cell
+
+// Depth s >= 1: one wavefront per group (row of depth s-1).
+__global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParams W, int s) {
     __shared__ int4 cgeo[WALK_WAVES][SL_CACHE];
     __shared__ int cnode[WALK_WAVES][SL_CACHE];
     __shared__ int cfc[WALK_WAVES][SL_CACHE];
     Counters* C = W.C;
+    const uint32_t n_prev_level = C->level_count[s - 1];
+    const uint32_t n_groups = n_prev_level + C->carried[s - 1];
+    const uint32_t base_prev = depth_base(C, s - 1);
+    const uint32_t base = base_prev + n_groups;
     const uint32_t n_level = C->level_count[s];
-    const uint32_t n_rows = n_level + C->carried[s];
-    const uint32_t base = depth_base(C, s);
-    const uint32_t base_prev = s > 0 ? base - (C->level_count[s - 1] + C->carried[s - 1]) : 0;
-    const uint32_t base_next = base + n_rows;
+    const uint32_t loff = C->level_off[s], loff_prev = C->level_off[s - 1];
     const int lane = lane_id();
     const int wslot = threadIdx.x / WAVE;
-    int* bm = lbuf[wslot];
-    int* bp = bm + CAP_M;
-    int* bs = bp + CAP_P;
     const SLCache cache{cgeo[wslot], cnode[wslot], cfc[wslot]};
     const uint32_t wid = blockIdx.x * WALK_WAVES + wslot;
     const uint32_t nw = gridDim.x * WALK_WAVES;
     const uint64_t lt = lanemask_lt();
+    const int bits = W.bits;
+    const uint64_t num4 = W.num4, den3 = W.den3;
     uint2 mchunk = make_uint2(0, 0), pchunk = make_uint2(0, 0), schunk = make_uint2(0, 0);  // lane 0 only
-    unsigned long long split_acc = 0;
-    for (uint32_t rl = wid; rl < n_rows; rl += nw) {
-        const uint32_t r = base + rl;
-        if (r >= W.row_cap) {  // rows beyond capacity: flag, the host retries larger
+    unsigned long long acc_m = 0, acc_p = 0, acc_s = 0;                                      // lane 0 only
+    for (uint32_t g = wid; g < n_groups; g += nw) {
+        const uint32_t rg = base_prev + g;
+        if (rg >= W.row_cap) continue;  // flagged when the row was made
+        const int P = g < n_prev_level ? (int)(loff_prev + g) : W.row_node[rg];
+        const int4 gp = W.nodes.geo[P];
+        const int nch = gp.w >> 8;
+        const uint32_t sl_st = W.row_sl_start[rg];
+        uint32_t sl_n = W.row_sl_cnt[rg];
+        if ((unsigned long long)sl_st + sl_n > W.sl_cap) sl_n = 0;  // only after an overflow
+        int nt, A0;
+        uint32_t row0;
+        if (nch > 0) {  // the cell's children: level-s rows in id order
+            nt = nch;
+            A0 = W.nodes.link[P].z;
+            row0 = base + (uint32_t)(A0 - (int)loff);
+        } else {  // a leaf with sources left to split: carried to depth s
+            if (sl_n == 0) continue;
+            nt = 1;
+            A0 = P;
+            uint32_t k = 0;
+            if (lane == 0) k = atomicAdd(&C->carried[s], 1u);
+            k = __shfl(k, 0, WAVE);
+            row0 = base + n_level + k;
+        }
+        if ((unsigned long long)row0 + nt > W.row_cap) {
+            // the host retries with more rows; rows that fit are left empty so
+            // that the next depth reads only rows of this evaluation
             if (lane == 0) atomicOr(&C->overflow, (uint32_t)OVF_ROWS);
+            const uint32_t r = row0 + lane;
+            if (lane < nt && r < W.row_cap) {
+                W.row_node[r] = A0 + lane;
+                W.row_next[r] = NO_ROW;
+                W.row_m_start[r] = W.row_m_cnt[r] = 0;
+                W.row_p_start[r] = W.row_p_cnt[r] = 0;
+                W.row_sl_start[r] = W.row_sl_cnt[r] = 0;
+            }
             continue;
         }
-        int A, src;
-        if (rl < n_level) {
-            A = (int)(C->level_off[s] + rl);
-            src = s == 0 ? -1 : (int)(base_prev + (uint32_t)(W.nodes.link[A].w - (int)C->level_off[s - 1]));
-        } else {
-            A = W.row_node[r];
-            src = W.row_src[r];
+        // targets (all of one level): lane t < nt holds target t
+        int4 gt_l = make_int4(0, 0, 0, 0);
+        if (lane < nt) gt_l = W.nodes.geo[A0 + lane];
+        const int la = gp.w & 0xff;
+        const int lt_level = nch > 0 ? la + 1 : la;
+        const int sa = bits - lt_level;
+        const int side_a = 1 << (sa + 1);
+        int cax[MAXT], cay[MAXT], caz[MAXT];
+        bool leaf_t[MAXT];
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+            const int gx = __builtin_amdgcn_readlane(gt_l.x, t), gy = __builtin_amdgcn_readlane(gt_l.y, t);
+            const int gz = __builtin_amdgcn_readlane(gt_l.z, t), gw = __builtin_amdgcn_readlane(gt_l.w, t);
+            cax[t] = (2 * gx + 1) << sa;
+            cay[t] = (2 * gy + 1) << sa;
+            caz[t] = (2 * gz + 1) << sa;
+            leaf_t[t] = (gw >> 8) == 0;
         }
-        const int4 ga = W.nodes.geo[A];
-        uint32_t nm = 0, np = 0, ns = 0;
-        enumerate(W, cache, A, ga, src, [&](bool valid, int B, int4 gb, int cat) {
-            const uint64_t mm = __ballot(cat == CAT_M2L);
-            const uint64_t mp = __ballot(cat == CAT_P2P);
-            const uint64_t ms = __ballot(cat == CAT_SL);
-            if (cat == CAT_M2L) { const uint32_t q = nm + __popcll(mm & lt); if (q < CAP_M) bm[q] = B; }
-            if (cat == CAT_P2P) { const uint32_t q = np + __popcll(mp & lt); if (q < CAP_P) bp[q] = B; }
-            if (cat == CAT_SL) { const uint32_t q = ns + __popcll(ms & lt); if (q < CAP_S) bs[q] = B; }
-            nm += __popcll(mm);
-            np += __popcll(mp);
-            ns += __popcll(ms);
-        });
-        uint32_t ms0 = 0, ps0 = 0, ss0 = 0;
-        if (lane == 0) {
-            W.m2l_cnt[A] += nm;
-            W.p2p_cnt[A] += np;
-            // segments come from per-wave chunks (one global atomic per chunk,
-            // not per row: same-address atomics serialise in L2)
-            if (nm) {
-                ms0 = pool_take(mchunk, nm, POOL_CHUNK_M, &C->m2l_pool_used, W.m2l_pool_cap);
-                if (ms0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_M2L); nm = 0; ms0 = 0; }
-            }
-            if (np) {
-                ps0 = pool_take(pchunk, np, POOL_CHUNK_P, &C->p2p_pool_used, W.p2p_pool_cap);
-                if (ps0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_P2P); np = 0; ps0 = 0; }
-            }
-            if (ns) {
-                ss0 = pool_take(schunk, ns, POOL_CHUNK_S, &C->sl_used, W.sl_cap);
-                if (ss0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_SL); ns = 0; ss0 = 0; }
-                split_acc += ns;
-            }
-            W.row_m_start[r] = ms0;
-            W.row_m_cnt[r] = nm;
-            W.row_p_start[r] = ps0;
-            W.row_p_cnt[r] = np;
-            W.row_sl_start[r] = ss0;
-            W.row_sl_cnt[r] = ns;
-            const bool a_leaf = (ga.w >> 8) == 0;
-            if (a_leaf && ns) {
-                const uint32_t k = atomicAdd(&C->carried[s + 1], 1u);
-                const uint32_t rr = base_next + C->level_count[s + 1] + k;
-                if (rr < W.row_cap) {
-                    W.row_node[rr] = A;
-                    W.row_src[rr] = (int)r;
-                } else {
-                    atomicOr(&C->overflow, (uint32_t)OVF_ROWS);
+        const bool filled = sl_n <= (uint32_t)SL_CACHE;
+        if (filled && sl_n) sl_cache_fill(W, cache, sl_st, 0, sl_n);
+        // per-candidate part of the MAC, then the per-target categories
+        // (0 = M2L, 1 = P2P, 2 = split, 3 = none)
+        auto classify_all = [&](bool valid, int B, int4 gb, auto&& per_target) {
+            const int sb = bits - (gb.w & 0xff);
+            const int cbx = (2 * gb.x + 1) << sb, cby = (2 * gb.y + 1) << sb, cbz = (2 * gb.z + 1) << sb;
+            const uint64_t side = (uint64_t)side_a + (1ull << (sb + 1));
+            const uint64_t lhs = den3 * side * side;
+            const bool b_leaf = (gb.w >> 8) == 0;
+#pragma unroll
+            for (int t = 0; t < MAXT; ++t) {
+                if (t < nt) {
+                    const int64_t dx = cbx - cax[t], dy = cby - cay[t], dz = cbz - caz[t];
+                    const uint64_t dsq = (uint64_t)(dx * dx + dy * dy + dz * dz);
+                    const bool approx = B != A0 + t && lhs < num4 * dsq;
+                    const int cat = !valid ? 3 : approx ? 0 : (leaf_t[t] && b_leaf) ? 1 : 2;
+                    per_target(t, cat);
                 }
             }
-        }
-        ms0 = __shfl(ms0, 0, WAVE); nm = __shfl(nm, 0, WAVE);
-        ps0 = __shfl(ps0, 0, WAVE); np = __shfl(np, 0, WAVE);
-        ss0 = __shfl(ss0, 0, WAVE); ns = __shfl(ns, 0, WAVE);
-        walk_lds_sync();
-        if (nm <= CAP_M && np <= CAP_P && ns <= CAP_S) {
-            for (uint32_t k = lane; k < nm; k += WAVE) W.m2l_pool[ms0 + k] = bm[k];
-            for (uint32_t k = lane; k < np; k += WAVE) W.p2p_pool[ps0 + k] = bp[k];
-            for (uint32_t k = lane; k < ns; k += WAVE) W.sl[ss0 + k] = bs[k];
-        } else {
-            uint32_t cm = 0, cp = 0, cs = 0;
-            enumerate(W, cache, A, ga, src, [&](bool valid, int B, int4 gb, int cat) {
-                const uint64_t mm = __ballot(cat == CAT_M2L);
-                const uint64_t mp = __ballot(cat == CAT_P2P);
-                const uint64_t ms = __ballot(cat == CAT_SL);
-                if (cat == CAT_M2L) { const uint32_t q = cm + __popcll(mm & lt); if (q < nm) W.m2l_pool[ms0 + q] = B; }
-                if (cat == CAT_P2P) { const uint32_t q = cp + __popcll(mp & lt); if (q < np) W.p2p_pool[ps0 + q] = B; }
-                if (cat == CAT_SL) { const uint32_t q = cs + __popcll(ms & lt); if (q < ns) W.sl[ss0 + q] = B; }
-                cm += __popcll(mm);
-                cp += __popcll(mp);
-                cs += __popcll(ms);
+        };
+        // pass 1: counts per target (uniform)
+        uint32_t nm[MAXT], np[MAXT], ns[MAXT];
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) nm[t] = np[t] = ns[t] = 0;
+        if (sl_n) {
+            enumerate_sl(W, cache, sl_st, sl_n, filled, [&](bool valid, int B, int4 gb) {
+                classify_all(valid, B, gb, [&](int t, int cat) {
+                    nm[t] += __popcll(__ballot(cat == 0));
+                    np[t] += __popcll(__ballot(cat == 1));
+                    ns[t] += __popcll(__ballot(cat == 2));
+                });
             });
         }
-        walk_lds_sync();
-    }
-    if (lane == 0 && split_acc) atomicAdd(&C->split_total, split_acc);
-}
-
-// Phase 2 (per depth, in depth order): copy each row's M2L/P2P segments to
-// its target's CSR cursor.  Depth order makes a leaf's entries from deeper
-// recursion depths follow its earlier ones, so every list stays ascending.
-__global__ __launch_bounds__(256) void walk_phase2_kernel(WalkParams W, int s) {
-    Counters* C = W.C;
-    const uint32_t n_level = C->level_count[s];
-    const uint32_t n_rows = n_level + C->carried[s];
-    const uint32_t base = depth_base(C, s);
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const uint32_t nw = gridDim.x * blockDim.x / WAVE;
-    const int lane = lane_id();
-    unsigned long long inter_acc = 0;
-    for (uint32_t rl = wid; rl < n_rows; rl += nw) {
-        const uint32_t r = base + rl;
-        if (r >= W.row_cap) continue;  // cannot happen after a clean phase 1
-        const int A = rl < n_level ? (int)(C->level_off[s] + rl) : W.row_node[r];
-        const uint32_t nm = W.row_m_cnt[r], np = W.row_p_cnt[r];
-        const uint32_t ms0 = W.row_m_start[r], ps0 = W.row_p_start[r];
-        const unsigned long long cm = W.m2l_cur[A], cp = W.p2p_cur[A];
-        // eight loads in flight per lane before their stores
-        for (uint32_t k0 = 0; k0 < nm; k0 += 8 * WAVE) {
-            int v[8];
+        uint32_t tm = 0, tp = 0, ts = 0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const uint32_t k = k0 + u * WAVE + lane;
-                v[u] = k < nm ? W.m2l_pool[ms0 + k] : 0;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const uint32_t k = k0 + u * WAVE + lane;
-                if (k < nm && cm + k < W.m2l_cap) W.m2l_src[cm + k] = v[u];
-            }
-        }
-        unsigned long long inter = 0;
-        const int na = W.nodes.link[A].y;
-        for (uint32_t k = lane; k < np; k += WAVE) {
-            const int B = W.p2p_pool[ps0 + k];
-            if (cp + k < W.p2p_cap) W.p2p_src[cp + k] = B;
-            const int nb = W.nodes.link[B].y;
-            inter += (unsigned long long)na * (unsigned long long)(B == A ? nb - 1 : nb);
-        }
-        inter_acc += inter;
+        for (int t = 0; t < MAXT; ++t) { tm += nm[t]; tp += np[t]; ts += ns[t]; }
+        // reserve one block per pool for all targets of the group
+        uint32_t ms0 = 0, ps0 = 0, ss0 = 0, fail = 0;
         if (lane == 0) {
-            W.m2l_cur[A] = cm + nm;
-            W.p2p_cur[A] = cp + np;
+            if (tm) {
+                ms0 = pool_take(mchunk, tm, POOL_CHUNK_M, &C->m2l_pool_used, W.m2l_pool_cap);
+                if (ms0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_M2L); fail = 1; }
+            }
+            if (tp) {
+                ps0 = pool_take(pchunk, tp, POOL_CHUNK_P, &C->p2p_pool_used, W.p2p_pool_cap);
+                if (ps0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_P2P); fail = 1; }
+            }
+            if (ts) {
+                ss0 = pool_take(schunk, ts, POOL_CHUNK_S, &C->sl_used, W.sl_cap);
+                if (ss0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_SL); fail = 1; }
+            }
+            if (!fail) { acc_m += tm; acc_p += tp; acc_s += ts; }
         }
+        fail = __shfl(fail, 0, WAVE);
+        ms0 = __shfl(ms0, 0, WAVE);
+        ps0 = __shfl(ps0, 0, WAVE);
+        ss0 = __shfl(ss0, 0, WAVE);
+        if (fail) {  // the host retries with larger pools; leave empty rows
+#pragma unroll
+            for (int t = 0; t < MAXT; ++t) nm[t] = np[t] = ns[t] = 0;
+            tm = tp = ts = 0;
+        }
+        // per-target segment starts (uniform) and the rows (lane t)
+        uint32_t cm[MAXT], cp[MAXT], cs[MAXT];
+        {
+            uint32_t om = ms0, op = ps0, os = ss0;
+            uint32_t my_ms = 0, my_nm = 0, my_ps = 0, my_np = 0, my_ss = 0, my_ns = 0;
+#pragma unroll
+            for (int t = 0; t < MAXT; ++t) {
+                cm[t] = om; cp[t] = op; cs[t] = os;
+                if (lane == t) {
+                    my_ms = om; my_nm = nm[t]; my_ps = op; my_np = np[t]; my_ss = os; my_ns = ns[t];
+                }
+                om += nm[t]; op += np[t]; os += ns[t];
+            }
+            if (lane < nt) {
+                const uint32_t r = row0 + lane;
+                W.row_node[r] = A0 + lane;
+                W.row_next[r] = NO_ROW;
+                W.row_m_start[r] = my_ms;
+                W.row_m_cnt[r] = my_nm;
+                W.row_p_start[r] = my_ps;
+                W.row_p_cnt[r] = my_np;
+                W.row_sl_start[r] = my_ss;
+                W.row_sl_cnt[r] = my_ns;
+                if (nch > 0) W.row_first[A0 + lane] = r;
+            }
+            if (nch == 0 && lane == 0) W.row_next[rg] = row0;
+        }
+        // pass 2: write every target's segments
+        if (tm + tp + ts) {
+            enumerate_sl(W, cache, sl_st, sl_n, filled, [&](bool valid, int B, int4 gb) {
+                classify_all(valid, B, gb, [&](int t, int cat) {
+                    const uint64_t bm = __ballot(cat == 0), bp = __ballot(cat == 1), bs = __ballot(cat == 2);
+                    if (cat == 0) W.m2l_pool[cm[t] + __popcll(bm & lt)] = B;
+                    if (cat == 1) W.p2p_pool[cp[t] + __popcll(bp & lt)] = B;
+                    if (cat == 2) W.sl[cs[t] + __popcll(bs & lt)] = B;
+                    cm[t] += __popcll(bm);
+                    cp[t] += __popcll(bp);
+                    cs[t] += __popcll(bs);
+                });
+            });
+        }
+        walk_lds_sync();  // the cache is refilled by the next group
     }
-    inter_acc = wave_sum(inter_acc);
-    if (lane == 0 && inter_acc) atomicAdd(&C->p2p_inter, inter_acc);
+    if (lane == 0) {
+        if (acc_m) atomicAdd(&C->m2l_total, acc_m);
+        if (acc_p) atomicAdd(&C->p2p_total, acc_p);
+        if (acc_s) atomicAdd(&C->split_total, acc_s);
+    }
 }
 
-void launch_walk_phase1(const WalkParams& W, int s, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL(walk_phase1_kernel, dim3(grid), dim3(WAVE * WALK_WAVES), 0, st, W, s);
+void launch_walk_root(const WalkParams& W, hipStream_t st) {
+    hipLaunchKernelGGL(walk_root_kernel, dim3(1), dim3(64), 0, st, W);
 }
-void launch_walk_phase2(const WalkParams& W, int s, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL(walk_phase2_kernel, dim3(grid), dim3(256), 0, st, W, s);
+void launch_walk_group(const WalkParams& W, int s, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(walk_group_kernel, dim3(grid), dim3(WAVE * WALK_WAVES), 0, st, W, s);
 }
 
 }  // namespace nbh
