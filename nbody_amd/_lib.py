@@ -44,7 +44,7 @@ class Config(ctypes.Structure):
         ("root_lo", ctypes.c_double * 3),
         ("root_size", ctypes.c_double),
         ("p2p", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("m2l_order", ctypes.c_uint32),
     ]
 
 

@@ -36,6 +36,8 @@ struct Counters {
     uint32_t m2l_pool_used;            // walk: phase-1 segment pools used
     uint32_t p2p_pool_used;
     unsigned long long m2l_total, p2p_total, p2p_inter, split_total;
+    unsigned long long m2lx_total;     // walk: M2L entries between different levels
+    uint32_t far_blocks, far_pairs;    // M2L: deferred pairs (level gap >= 2)
     uint32_t nonfinite;
 };
 

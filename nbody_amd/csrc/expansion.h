@@ -8,7 +8,8 @@
 //
 //   P2M  M_n   = sum_j q_j b_j^n / n!                      (b = x_j - z_B)
 //   M2M  M'_n  = sum_{m<=n} M_m d^(n-m)/(n-m)!             (d = z_child - z_parent)
-//   M2L  L_k  += sum_{|k|+|n|<=P} (-1)^|n| D_{k+n}(R) M_n  (R = z_A - z_B)
+//   M2L  L_k  += sum_{|k|,|n|<=P, |k|+|n|<=Q} (-1)^|n| D_{k+n}(R) M_n
+//                                                          (R = z_A - z_B, Q = P or P+1)
 //   L2L  L'_j += sum_m L_{j+m} d^m / m!                    (d = z_child - z_parent)
 //   L2P  phi   = sum_k L_k a^k/k!,  g_e = sum_k L_{k+e} a^k/k!   (a = x - z_A)
 //
@@ -124,6 +125,40 @@ struct Tab {
     static constexpr Tables<P> t{};
 };
 
+// M2L pair table: (k, n, k+n) for |k|, |n| <= P and |k| + |n| <= Q, with k+n
+// indexed in the order-Q coefficient order (orders <= P are a prefix of it);
+// ordered by source coefficient n.  Q = P is the usual total-degree
+// truncation; Q = P + 1 adds the cross terms of degree P + 1.
+constexpr int m2l_npairs(int p, int q) {
+    int c = 0;
+    for (int dk = 0; dk <= p; ++dk)
+        for (int dn = 0; dn <= p; ++dn)
+            if (dk + dn <= q) c += (dk + 1) * (dk + 2) / 2 * ((dn + 1) * (dn + 2) / 2);
+    return c;
+}
+template <int P, int Q>
+struct M2LTables {
+    static constexpr int NP = m2l_npairs(P, Q);
+    int pk[NP], pn[NP], pkn[NP];
+    constexpr M2LTables() : pk(), pn(), pkn() {
+        constexpr auto& TP = Tab<P>::t;
+        constexpr auto& TQ = Tab<Q>::t;
+        int t = 0;
+        for (int b = 0; b < ncoef(P); ++b)
+            for (int a = 0; a < ncoef(P); ++a) {
+                if (TP.deg[a] + TP.deg[b] > Q) continue;
+                pk[t] = a;
+                pn[t] = b;
+                pkn[t] = TQ.at(TP.mx[a] + TP.mx[b], TP.my[a] + TP.my[b], TP.mz[a] + TP.mz[b]);
+                ++t;
+            }
+    }
+};
+template <int P, int Q>
+struct M2LTab {
+    static constexpr M2LTables<P, Q> t{};
+};
+
 // ---------------------------------------------------------------------------
 // Straight-line code generation: each table entry becomes a constexpr
 // template argument via std::index_sequence fold expressions, so every
@@ -177,20 +212,43 @@ __device__ __forceinline__ void derivs(T_ Rx, T_ Ry, T_ Rz, T_ eps2, T_* D) {
     deriv_all<P, T_>(R, inv_s, D, std::make_index_sequence<ncoef(P) - 1>{});
 }
 
-template <int P, int I>
+template <int P, int Q, int I>
 __device__ __forceinline__ void m2l_step(const float* D, const float* M, float* L) {
-    constexpr auto& T = Tab<P>::t;
+    constexpr auto& T = M2LTab<P, Q>::t;
     constexpr int k = T.pk[I], n = T.pn[I], kn = T.pkn[I];
-    if constexpr (T.deg[n] & 1) L[k] = fmaf(-D[kn], M[n], L[k]);
+    if constexpr (Tab<P>::t.deg[n] & 1) L[k] = fmaf(-D[kn], M[n], L[k]);
     else L[k] = fmaf(D[kn], M[n], L[k]);
 }
-template <int P, size_t... I>
+template <int P, int Q, size_t... I>
 __device__ __forceinline__ void m2l_all(const float* D, const float* M, float* L, std::index_sequence<I...>) {
-    (m2l_step<P, (int)I>(D, M, L), ...);
+    (m2l_step<P, Q, (int)I>(D, M, L), ...);
 }
-template <int P>
+// D: derivatives to order Q; M, L: order P
+template <int P, int Q>
 __device__ __forceinline__ void m2l_apply(const float* D, const float* M, float* L) {
-    m2l_all<P>(D, M, L, std::make_index_sequence<npairs(P)>{});
+    m2l_all<P, Q>(D, M, L, std::make_index_sequence<m2l_npairs(P, Q)>{});
+}
+
+// Octant symmetry of the derivative tensor: with S = diag(s) a sign flip of
+// the axes, D_m(S R) = s^m D_m(R), s^m = prod_i s_i^{m_i}.  neg holds bit i
+// when s_i = -1; coefficients whose odd-exponent axes meet neg an odd number
+// of times change sign.
+template <int Q, int K>
+__device__ __forceinline__ void octant_step(const uint32_t* fl, float* D) {
+    constexpr auto& T = Tab<Q>::t;
+    constexpr int par = (T.mx[K] & 1) | ((T.my[K] & 1) << 1) | ((T.mz[K] & 1) << 2);
+    if constexpr (par != 0) D[K] = __uint_as_float(__float_as_uint(D[K]) ^ fl[par]);
+}
+template <int Q, size_t... I>
+__device__ __forceinline__ void octant_all(const uint32_t* fl, float* D, std::index_sequence<I...>) {
+    (octant_step<Q, (int)I>(fl, D), ...);
+}
+template <int Q>
+__device__ __forceinline__ void octant_signs(uint32_t neg, float* D) {
+    uint32_t fl[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) fl[c] = (uint32_t)(__popc(neg & (uint32_t)c) & 1) << 31;
+    octant_all<Q>(fl, D, std::make_index_sequence<ncoef(Q)>{});
 }
 
 template <int P, int I>

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 
 // M2L, plain: one thread per target cell, the derivative tensor rebuilt per
 // pair (replaces field.cl:153-211's order-0 cell field).
-template <int P>
+template <int P, int Q>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_plain_kernel(FmmParams F,
                                                                                                   uint32_t num_nodes) {
     constexpr int NC = ncoef(P);
@@ -121,11 +121,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
             const int B = F.lists.m_pool[s0 + e];
             const int4 gb = F.nodes.geo[B];
             const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
-            float D[NC];
-            derivs<P>(d.x, d.y, d.z, F.eps2, D);
+            float D[ncoef(Q)];
+            derivs<Q>(d.x, d.y, d.z, F.eps2, D);
             // source moments read straight from memory; the pair table is ordered
             // by source coefficient so each M_n is loaded once and used at once
-            m2l_apply<P>(D, F.M + (size_t)B * F.ncp, L);
+            m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
         }
     }
     float* out = F.L + (size_t)A * F.ncp;
@@ -420,70 +420,80 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
 
 // ---------------------------------------------------------------------------
 // M2L grouped by translation offset.  Same-level pairs of one system share
-// their derivative tensor whenever they share the offset o = i_B - i_A
-// (cells at level l, R = -o h_l): the tensor of every offset in [-K, K]^3 is
-// built once per level and step (FP64, stored FP32) and the M2L kernel only
-// contracts it.  Cross-level pairs (a leaf kept against a finer or coarser
-// cell) build their tensor per pair.  One wavefront per target cell; lanes
-// stride its source list and meet in a wave reduction.
-template <int P>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) void dtable_kernel(FmmParams F, int level0,
-                                                                                                int nlev, int K) {
-    constexpr int NC = ncoef(P);
-    const int side = 2 * K + 1;
-    const int per = side * side * side;
+// their derivative tensor whenever they share the offset o = i_B - i_A (cells
+// at level l, R = -o h_l).  Same-level pairs always have |o_i| <= K =
+// 2 floor(sqrt(3)/theta) + 1 (their parents were split), and by the octant
+// symmetry D_m(S R) = s^m D_m(R) only offsets with o_i >= 0 are tabled:
+// (K+1)^3 tensors per level, built once per step in FP64 and stored FP32.
+// For theta = 1/2 and Q <= 5 a level's table fits in LDS (8^3 x 56 floats =
+// 112 KiB): the grouped kernel runs one launch per level, every block loads
+// that level's table once, and the only global traffic per pair is the source
+// moment row.  (Per-lane 144-byte gathers of both the tensor and the moment
+// rows made this contraction L2-bandwidth-bound.)
+template <int Q>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) void dtable_kernel(
+    FmmParams F, float* out, int level0, int nlev, int E, int a, int b, int shift) {
+    constexpr int NCQ = ncoef(Q);
+    const int per = E * E * E;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= per * nlev) return;
     const int lv = t / per, o = t % per;
-    const int ox = o / (side * side) - K, oy = (o / side) % side - K, oz = o % side - K;
-    const double h = ldexp(F.box[0].w, -(level0 + lv));
-    double D[NC];
-    derivs<P, double>(-ox * h, -oy * h, -oz * h, (double)F.eps2_64, D);
-    float* out = F.dtab + (size_t)t * F.ncp;
+    const int ex = o / (E * E), ey = (o / E) % E, ez = o % E;
+    const double h = ldexp(F.box[0].w, -(level0 + lv + shift));
+    double D[NCQ];
+    derivs<Q, double>(-(a + b * ex) * h, -(a + b * ey) * h, -(a + b * ez) * h, (double)F.eps2_64, D);
+    float* row = out + (size_t)t * F.dcp;
 #pragma unroll
-    for (int k = 0; k < NC; ++k) out[k] = (float)D[k];
+    for (int k = 0; k < NCQ; ++k) row[k] = (float)D[k];
+    for (int k = NCQ; k < F.dcp; ++k) row[k] = 0.f;
 }
 
-template <int P>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_grouped_kernel(
-    FmmParams F, uint32_t num_nodes) {
-    constexpr int NC = ncoef(P);
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const uint32_t nw = gridDim.x * blockDim.x / WAVE;
+constexpr int M2L_LDS_THREADS = 1024;
+
+// Same-level part of the rows of the level's targets [off, off+count): one
+// wavefront per target, lanes over its entries; writes L.
+template <int P, int Q>
+__global__ __launch_bounds__(M2L_LDS_THREADS) void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count,
+                                                                  int lvl) {
+    constexpr int NC = ncoef(P), NCQ = ncoef(Q);
+    constexpr int NQD = (NCQ + 3) / 4;  // float4 per tensor row (the host sets dcp = 4 NQD)
+    extern __shared__ float4 dtab_lds[];
+    const int K = F.dK, K1 = K + 1;
+    {
+        const float4* src = reinterpret_cast<const float4*>(F.dtab + (size_t)lvl * F.dsz * F.dcp);
+        const int tot = F.dsz * NQD;
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) dtab_lds[i] = src[i];
+        __syncthreads();
+    }
     const int lane = lane_id();
-    const int K = F.dK, side = 2 * K + 1;
-    for (uint32_t A = wid; A < num_nodes; A += nw) {
+    const uint32_t wpb = blockDim.x / WAVE;
+    for (uint32_t t = blockIdx.x * wpb + threadIdx.x / WAVE; t < count; t += gridDim.x * wpb) {
+        const uint32_t A = off + t;
         const int4 ga = F.nodes.geo[A];
-        const int la = ga.w & 0xff;
-        const float unit = F.unit[F.nodes.sys[A]];
         float L[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = 0.f;
-        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r])
-        for (uint32_t e = F.lists.m_start[r] + lane, e1 = F.lists.m_start[r] + F.lists.m_cnt[r]; e < e1; e += WAVE) {
-            const int B = F.lists.m_pool[e];
-            const int4 gb = F.nodes.geo[B];
-            const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
-            float D[NC];
-            const bool tab = F.dtab && (gb.w & 0xff) == la && la >= F.dlevel0 && la < F.dlevel0 + F.dnlev &&
-                             abs(ox) <= K && abs(oy) <= K && abs(oz) <= K;
-            if (tab) {
-                const int o = ((ox + K) * side + (oy + K)) * side + (oz + K);
-                const float4* src = reinterpret_cast<const float4*>(
-                    F.dtab + ((size_t)(la - F.dlevel0) * side * side * side + o) * F.ncp);
+        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
+            const uint32_t s0 = F.lists.m_start[r];
+            const uint32_t e1 = s0 + F.lists.m_cnt[r] - F.lists.mx_cnt[r];
+            for (uint32_t e = s0 + lane; e < e1; e += WAVE) {
+                const int B = F.lists.m_pool[e];
+                const int4 gb = F.nodes.geo[B];
+                const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
+                const uint32_t neg = (uint32_t)(ox < 0) | ((uint32_t)(oy < 0) << 1) | ((uint32_t)(oz < 0) << 2);
+                const float4* row = dtab_lds + ((abs(ox) * K1 + abs(oy)) * K1 + abs(oz)) * NQD;
+                float D[NCQ];
 #pragma unroll
-                for (int q = 0; q < (NC + 3) / 4; ++q) {
-                    const float4 v = src[q];
-                    if (4 * q + 0 < NC) D[4 * q + 0] = v.x;
-                    if (4 * q + 1 < NC) D[4 * q + 1] = v.y;
-                    if (4 * q + 2 < NC) D[4 * q + 2] = v.z;
-                    if (4 * q + 3 < NC) D[4 * q + 3] = v.w;
+                for (int q = 0; q < NQD; ++q) {
+                    const float4 v = row[q];
+                    if (4 * q + 0 < NCQ) D[4 * q + 0] = v.x;
+                    if (4 * q + 1 < NCQ) D[4 * q + 1] = v.y;
+                    if (4 * q + 2 < NCQ) D[4 * q + 2] = v.z;
+                    if (4 * q + 3 < NCQ) D[4 * q + 3] = v.w;
                 }
-            } else {
-                const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
-                derivs<P>(d.x, d.y, d.z, F.eps2, D);
+                octant_signs<Q>(neg, D);
+                m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
             }
-            m2l_apply<P>(D, F.M + (size_t)B * F.ncp, L);
         }
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
@@ -495,6 +505,239 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     }
 }
 
+// Loads a tensor row (float4 x NQD) and applies the octant signs.
+template <int Q>
+__device__ __forceinline__ void load_tensor(const float4* row, uint32_t neg, float* D) {
+    constexpr int NCQ = ncoef(Q), NQD = (NCQ + 3) / 4;
+#pragma unroll
+    for (int q = 0; q < NQD; ++q) {
+        const float4 v = row[q];
+        if (4 * q + 0 < NCQ) D[4 * q + 0] = v.x;
+        if (4 * q + 1 < NCQ) D[4 * q + 1] = v.y;
+        if (4 * q + 2 < NCQ) D[4 * q + 2] = v.z;
+        if (4 * q + 3 < NCQ) D[4 * q + 3] = v.w;
+    }
+    octant_signs<Q>(neg, D);
+}
+
+// M2L with the tensor built per pair, every entry (no tables: ensembles of
+// systems with different root sizes); writes L.  One wavefront per target,
+// lanes over its entries.
+template <int P, int Q>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_pairs_kernel(
+    FmmParams F, uint32_t num_nodes) {
+    constexpr int NC = ncoef(P);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / WAVE;
+    const int lane = lane_id();
+    for (uint32_t A = wid; A < num_nodes; A += nw) {
+        const int4 ga = F.nodes.geo[A];
+        const float unit = F.unit[F.nodes.sys[A]];
+        float L[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = 0.f;
+        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
+            const uint32_t e0 = F.lists.m_start[r], e1 = e0 + F.lists.m_cnt[r];
+            for (uint32_t e = e0 + lane; e < e1; e += WAVE) {
+                const int B = F.lists.m_pool[e];
+                const int4 gb = F.nodes.geo[B];
+                const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
+                float D[ncoef(Q)];
+                derivs<Q>(d.x, d.y, d.z, F.eps2, D);
+                m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
+        if (lane == 0) {
+            float* out = F.L + (size_t)A * F.ncp;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) out[k] = L[k];
+        }
+    }
+}
+
+// Cross-level entry: its tensor row in F.xtab and octant signs, or -1 when
+// the levels differ by more than one (or the offset lies beyond the table).
+// Centres in units of half the finer cell: 2 i + 1 at the finer level,
+// 4 i + 2 at the coarser; the difference is odd.
+__device__ __forceinline__ int xtab_index(const FmmParams& F, int4 ga, int4 gb, uint32_t& neg) {
+    const int la = ga.w & 0xff, lb = gb.w & 0xff;
+    if (abs(la - lb) != 1) return -1;
+    const int lf = la > lb ? la : lb;
+    const int sa = la == lf ? 1 : 2, sb = lb == lf ? 1 : 2;  // centre = sx (2 i + 1)
+    const int dx = sb * (2 * gb.x + 1) - sa * (2 * ga.x + 1);
+    const int dy = sb * (2 * gb.y + 1) - sa * (2 * ga.y + 1);
+    const int dz = sb * (2 * gb.z + 1) - sa * (2 * ga.z + 1);
+    const int ex = (abs(dx) - 1) >> 1, ey = (abs(dy) - 1) >> 1, ez = (abs(dz) - 1) >> 1;
+    const int E = F.xE;
+    if (ex >= E || ey >= E || ez >= E || lf < F.xlevel0 || lf >= F.xlevel0 + F.xnlev) return -1;
+    neg = (uint32_t)(dx < 0) | ((uint32_t)(dy < 0) << 1) | ((uint32_t)(dz < 0) << 2);
+    return ((lf - F.xlevel0) * E + ex) * E * E + ey * E + ez;
+}
+
+// The other-level tails of the rows (after the same-level kernel): pairs one
+// level apart read their tensor from F.xtab; the rest are deferred to
+// m2l_far_kernel as one contiguous block per target.  Adds to L.
+template <int P, int Q>
+__global__ __launch_bounds__(256) void m2l_cross_kernel(FmmParams F, uint32_t num_nodes) {
+    constexpr int NC = ncoef(P), NCQ = ncoef(Q), NQD = (NCQ + 3) / 4;
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / WAVE;
+    const int lane = lane_id();
+    const float4* X4 = reinterpret_cast<const float4*>(F.xtab);
+    for (uint32_t A = wid; A < num_nodes; A += nw) {
+        uint32_t any = 0;
+        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) any |= F.lists.mx_cnt[r];
+        if (!any) continue;
+        const int4 ga = F.nodes.geo[A];
+        float L[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = 0.f;
+        uint32_t nfar = 0;
+        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
+            const uint32_t e1 = F.lists.m_start[r] + F.lists.m_cnt[r];
+            for (uint32_t e = e1 - F.lists.mx_cnt[r] + lane; e < e1; e += WAVE) {
+                const int B = F.lists.m_pool[e];
+                uint32_t neg = 0;
+                const int t = xtab_index(F, ga, F.nodes.geo[B], neg);
+                if (t < 0) {
+                    ++nfar;
+                    continue;
+                }
+                float D[NCQ];
+                load_tensor<Q>(X4 + (size_t)t * NQD, neg, D);
+                m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
+        if (lane == 0) {
+            float* out = F.L + (size_t)A * F.ncp;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) out[k] += L[k];
+        }
+        nfar = wave_sum(nfar);
+        if (nfar) {
+            uint32_t b = 0, s0 = 0;
+            if (lane == 0) {
+                b = atomicAdd(&F.far_cnt[0], 1u);
+                s0 = atomicAdd(&F.far_cnt[1], nfar);
+            }
+            b = __shfl(b, 0, WAVE);
+            s0 = __shfl(s0, 0, WAVE);
+            // capacities are sized from the walk's count of other-level entries
+            if (b >= F.far_blk_cap || (unsigned long long)s0 + nfar > F.far_cap) continue;
+            uint32_t pos = s0;
+            const uint64_t lt = lanemask_lt();
+            for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
+                const uint32_t e1 = F.lists.m_start[r] + F.lists.m_cnt[r];
+                const uint32_t e0 = e1 - F.lists.mx_cnt[r];
+                for (uint32_t c0 = e0; c0 < e1; c0 += WAVE) {
+                    const uint32_t e = c0 + lane;
+                    int B = -1;
+                    bool far = false;
+                    if (e < e1) {
+                        B = F.lists.m_pool[e];
+                        uint32_t neg = 0;
+                        far = xtab_index(F, ga, F.nodes.geo[B], neg) < 0;
+                    }
+                    const uint64_t bal = __ballot(far);
+                    if (far) F.far_src[pos + __popcll(bal & lt)] = B;
+                    pos += __popcll(bal);
+                }
+            }
+            if (lane == 0) F.far_blk[b] = make_uint4(A, s0, nfar, 0);
+        }
+    }
+}
+
+// Deferred pairs (levels two or more apart): tensor built per pair; one
+// wavefront per target block; adds to L.
+template <int P, int Q>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_far_kernel(FmmParams F) {
+    constexpr int NC = ncoef(P);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / WAVE;
+    const int lane = lane_id();
+    const uint32_t nblk = min(F.far_cnt[0], F.far_blk_cap);
+    for (uint32_t b = wid; b < nblk; b += nw) {
+        const uint4 blk = F.far_blk[b];
+        if ((unsigned long long)blk.y + blk.z > F.far_cap) continue;
+        const int A = (int)blk.x;
+        const int4 ga = F.nodes.geo[A];
+        const float unit = F.unit[F.nodes.sys[A]];
+        float L[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = 0.f;
+        for (uint32_t e = lane; e < blk.z; e += WAVE) {
+            const int B = F.far_src[blk.y + e];
+            const int4 gb = F.nodes.geo[B];
+            const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
+            float D[ncoef(Q)];
+            derivs<Q>(d.x, d.y, d.z, F.eps2, D);
+            m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
+        if (lane == 0) {
+            float* out = F.L + (size_t)A * F.ncp;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) out[k] += L[k];
+        }
+    }
+}
+
+__global__ void zero_u32x2_kernel(uint32_t* p) {
+    if (threadIdx.x < 2) p[threadIdx.x] = 0u;
+}
+
+__global__ void zero_f32_kernel(float* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+template <int P, int Q>
+static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, hipStream_t st) {
+    const uint32_t nn = plan.num_nodes;
+    const unsigned grid = std::min<unsigned>(ceil_div(nn, 4), 65536u);
+    if (plan.mode == 0) {
+        hipLaunchKernelGGL((m2l_plain_kernel<P, Q>), dim3(ceil_div(nn, 256)), dim3(256), 0, st, F, nn);
+        return;
+    }
+    if (!F.dtab) {
+        hipLaunchKernelGGL((m2l_pairs_kernel<P, Q>), dim3(grid), dim3(256), 0, st, F, nn);
+        return;
+    }
+    hipLaunchKernelGGL(zero_u32x2_kernel, dim3(1), dim3(64), 0, st, F.far_cnt);
+    // cells above the tabled levels have no same-level pairs: L starts at 0
+    const uint32_t top = plan.level_off[F.dlevel0];
+    if (top) {
+        const size_t nz = (size_t)top * F.ncp;
+        hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)ceil_div(nz, (size_t)256)), dim3(256), 0, st, F.L, nz);
+    }
+    const size_t lds = (size_t)F.dsz * F.dcp * 4;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    for (int l = F.dlevel0; l < F.dlevel0 + F.dnlev; ++l) {
+        const uint32_t cnt = plan.level_count[l];
+        if (!cnt) continue;
+        const unsigned g = std::min<unsigned>(ceil_div(cnt, M2L_LDS_THREADS / WAVE), 2048u);
+        hipLaunchKernelGGL((m2l_lds_kernel<P, Q>), dim3(g), dim3(M2L_LDS_THREADS), lds, st, F, plan.level_off[l], cnt,
+                           l - F.dlevel0);
+    }
+    hipLaunchKernelGGL((m2l_cross_kernel<P, Q>), dim3(grid), dim3(256), 0, st, F, nn);
+    hipLaunchKernelGGL((m2l_far_kernel<P, Q>), dim3(1024), dim3(256), 0, st, F);
+}
+
+template <int Q>
+static void launch_dtable_t(const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
+                            hipStream_t st) {
+    const unsigned total = (unsigned)(E * E * E * nlev);
+    hipLaunchKernelGGL((dtable_kernel<Q>), dim3(ceil_div(total, 128)), dim3(128), 0, st, F, out, level0, nlev, E, a,
+                       b, shift);
+}
+
 template <int P>
 static void launch_p2m_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     hipLaunchKernelGGL((p2m_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves);
@@ -502,10 +745,6 @@ static void launch_p2m_t(const FmmParams& F, const State& S, uint32_t num_leaves
 template <int P>
 static void launch_m2m_t(const FmmParams& F, uint32_t count, uint32_t off, hipStream_t st) {
     hipLaunchKernelGGL((m2m_kernel<P>), dim3(ceil_div(count, 64)), dim3(64), 0, st, F, count, off);
-}
-template <int P>
-static void launch_m2l_t(const FmmParams& F, uint32_t num_nodes, hipStream_t st) {
-    hipLaunchKernelGGL((m2l_plain_kernel<P>), dim3(ceil_div(num_nodes, 256)), dim3(256), 0, st, F, num_nodes);
 }
 template <int P>
 static void launch_l2l_t(const FmmParams& F, uint32_t count, uint32_t off, hipStream_t st) {
@@ -522,18 +761,6 @@ static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num
     const unsigned grid = std::min<unsigned>(ceil_div(num_leaves, P2P_WAVES), 65536u);
     hipLaunchKernelGGL((p2p_staged_kernel<P>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S, num_leaves, kick_dt);
 }
-template <int P>
-static void launch_dtable_t(const FmmParams& F, int level0, int nlev, int K, hipStream_t st) {
-    const int side = 2 * K + 1;
-    const unsigned total = (unsigned)(side * side * side * nlev);
-    hipLaunchKernelGGL((dtable_kernel<P>), dim3(ceil_div(total, 128)), dim3(128), 0, st, F, level0, nlev, K);
-}
-template <int P>
-static void launch_m2l_grouped_t(const FmmParams& F, uint32_t num_nodes, hipStream_t st) {
-    const unsigned grid = std::min<unsigned>(ceil_div(num_nodes, 4), 65536u);
-    hipLaunchKernelGGL((m2l_grouped_kernel<P>), dim3(grid), dim3(256), 0, st, F, num_nodes);
-}
-
 #define NBH_DISPATCH_P(p, call)          \
     switch (p) {                         \
         case 1: { constexpr int PP = 1; call; } break; \
@@ -554,10 +781,6 @@ void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint
     if (!count) return;
     NBH_DISPATCH_P(p, launch_m2m_t<PP>(F, count, off, st));
 }
-void launch_m2l_plain(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st) {
-    if (!num_nodes) return;
-    NBH_DISPATCH_P(p, launch_m2l_t<PP>(F, num_nodes, st));
-}
 void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st) {
     (void)level;
     if (!count) return;
@@ -568,13 +791,33 @@ void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_l
     if (!num_leaves) return;
     NBH_DISPATCH_P(p, launch_p2p_staged_t<PP>(F, S, num_leaves, kick_dt, st));
 }
-void launch_dtable(int p, const FmmParams& F, int level0, int nlev, int K, hipStream_t st) {
-    if (nlev <= 0) return;
-    NBH_DISPATCH_P(p, launch_dtable_t<PP>(F, level0, nlev, K, st));
+void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
+                   hipStream_t st) {
+#define NBH_DT(QQ) launch_dtable_t<QQ>(F, out, level0, nlev, E, a, b, shift, st)
+    switch (q) {
+        case 1: NBH_DT(1); break;
+        case 2: NBH_DT(2); break;
+        case 3: NBH_DT(3); break;
+        case 4: NBH_DT(4); break;
+        case 5: NBH_DT(5); break;
+        case 6: NBH_DT(6); break;
+        case 7: NBH_DT(7); break;
+        default: break;
+    }
+#undef NBH_DT
 }
-void launch_m2l_grouped(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st) {
-    if (!num_nodes) return;
-    NBH_DISPATCH_P(p, launch_m2l_grouped_t<PP>(F, num_nodes, st));
+void launch_m2l(int p, int q, const FmmParams& F, const M2LPlan& plan, hipStream_t st) {
+    if (!plan.num_nodes) return;
+    const bool ext = q == p + 1;
+    switch (p) {
+        case 1: ext ? launch_m2l_pq<1, 2>(F, plan, st) : launch_m2l_pq<1, 1>(F, plan, st); break;
+        case 2: ext ? launch_m2l_pq<2, 3>(F, plan, st) : launch_m2l_pq<2, 2>(F, plan, st); break;
+        case 3: ext ? launch_m2l_pq<3, 4>(F, plan, st) : launch_m2l_pq<3, 3>(F, plan, st); break;
+        case 4: ext ? launch_m2l_pq<4, 5>(F, plan, st) : launch_m2l_pq<4, 4>(F, plan, st); break;
+        case 5: ext ? launch_m2l_pq<5, 6>(F, plan, st) : launch_m2l_pq<5, 5>(F, plan, st); break;
+        case 6: ext ? launch_m2l_pq<6, 7>(F, plan, st) : launch_m2l_pq<6, 6>(F, plan, st); break;
+        default: break;
+    }
 }
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st) {
     if (!num_leaves) return;

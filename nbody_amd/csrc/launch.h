@@ -60,7 +60,8 @@ struct WalkParams {
     uint32_t* row_sl_start;
     uint32_t* row_sl_cnt;
     uint32_t* row_m_start;
-    uint32_t* row_m_cnt;
+    uint32_t* row_m_cnt;   // M2L entries of the row: same level first, then
+    uint32_t* row_mx_cnt;  // the last row_mx_cnt at other levels
     uint32_t* row_p_start;
     uint32_t* row_p_cnt;
     int* sl;
@@ -78,6 +79,7 @@ struct ListView {
     const uint32_t* next;
     const uint32_t* m_start;
     const uint32_t* m_cnt;
+    const uint32_t* mx_cnt;  // trailing M2L entries at another level than the target
     const uint32_t* p_start;
     const uint32_t* p_cnt;
     const int* m_pool;
@@ -108,21 +110,42 @@ struct FmmParams {
     ListView lists;
     unsigned long long* p2p_inter;  // statistics: particle pairs of the P2P kernels
     uint32_t* pleaf;       // per particle: its leaf node
-    float* dtab;           // per-level derivative tensors of same-level offsets (or null)
+    float* dtab;           // per level, derivative tensors of the offsets [0, K]^3 (or null)
     int dK, dlevel0, dnlev;
+    int dcp;               // tensor row stride (floats): ncoef(Q) padded to 4
+    int dsz;               // tensors per level: (K+1)^3
+    // pairs one level apart: per finer level lf, tensors of the centre offsets
+    // d (units of half the finer cell, odd) with |d_i| = 1, 3, .., 2 xE - 1
+    float* xtab;
+    int xE, xlevel0, xnlev;
+    // pairs further apart in level: deferred per target, (target, first, count)
+    uint4* far_blk;
+    int* far_src;
+    uint32_t far_cap, far_blk_cap;
+    uint32_t* far_cnt;     // [blocks, pairs]
     double eps2_64;
 };
 
 void launch_gather_state(const State& src, const State& dst, const uint32_t* perm, uint32_t n, hipStream_t st);
 void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
 void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
-void launch_m2l_plain(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st);
 void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st);
 void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt,
                        hipStream_t st);
-void launch_dtable(int p, const FmmParams& F, int level0, int nlev, int K, hipStream_t st);
-void launch_m2l_grouped(int p, const FmmParams& F, uint32_t num_nodes, hipStream_t st);
+// M2L of order (p, q): q = p keeps |k| + |n| <= p, q = p + 1 also the cross
+// terms of degree p + 1 (the expansions themselves stay of order p).
+struct M2LPlan {
+    int mode;  // 0: plain (one thread per target, tensor per pair); 1: grouped
+    uint32_t num_nodes;
+    uint32_t level_off[MAX_LEVELS + 2];
+    uint32_t level_count[MAX_LEVELS + 1];
+};
+// tensors at R = -(a + b e) h_l / 2^shift for e in [0, E)^3, levels
+// level0 .. level0+nlev-1, into out (rows of F.dcp floats)
+void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
+                   hipStream_t st);
+void launch_m2l(int p, int q, const FmmParams& F, const M2LPlan& plan, hipStream_t st);
 
 // integrator updates: mode 0 = KDK first half (kick dt_k, drift dt_d),
 // 1 = Euler (x += v dt; v += a dt), 2 = kick-drift (v += a dt; x += v dt)
@@ -138,7 +161,8 @@ void launch_diagnostics(const State& S, uint32_t n, double* partial, uint32_t nb
 uint32_t diagnostics_blocks(uint32_t n);
 
 // adjoint (vjp.hip): g = [gx | gy | gz | gm] in sorted order, FP64
-void launch_force_vjp(int p, const FmmParams& F, const FmmParams& Fw, const double* w_in, const uint32_t* id,
+void launch_force_vjp(int p, int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const double* w_in,
+                      const uint32_t* id,
                       float4* pw, uint32_t n, uint32_t num_leaves, const uint32_t* level_count,
                       const uint32_t* level_off, int depth, double* g, hipStream_t st);
 

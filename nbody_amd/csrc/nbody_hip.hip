@@ -87,15 +87,16 @@ struct nbh_handle {
     double time = 0.0;
     bool acc_valid = false;
     bool timing = false;
+    int max_lds = 65536;  // LDS bytes per workgroup on this device
     std::vector<uint32_t> sys_off_h;
 
     StateBufs st[2];
     int cur = 0;
     DevBuf sys_off, keys, keys_alt, perm, perm_alt, sort_tmp, tree_tmp, scan_tmp;
     DevBuf box, inv, unit, bbox_part;
-    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab;
-    DevBuf row_node, row_first, row_next, row_sl_start, row_sl_cnt, sl, row_m_start, row_m_cnt, row_p_start,
-        row_p_cnt, m2l_pool, p2p_pool;
+    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab, xtab, far_src, far_blk;
+    DevBuf row_node, row_first, row_next, row_sl_start, row_sl_cnt, sl, row_m_start, row_m_cnt, row_mx_cnt,
+        row_p_start, row_p_cnt, m2l_pool, p2p_pool;
     unsigned long long m2l_pool_cap = 0, p2p_pool_cap = 0;
     DevBuf counters, diag, tmp_small;
     Counters* hc = nullptr;  // pinned host copy
@@ -187,6 +188,10 @@ int validate_config(const nbh_config* c, std::string& why) {
         }
         if (c->leaf_capacity < 1 || c->leaf_capacity > 2048) { why = "leaf_capacity must be in [1, 2048]"; return NBH_E_INVALID; }
         if (c->order < 1 || c->order > 6) { why = "order must be in [1, 6]"; return NBH_E_INVALID; }
+        if (c->m2l_order != 0 && c->m2l_order != c->order && c->m2l_order != c->order + 1) {
+            why = "m2l_order must be 0 (= order + 1), order or order + 1";
+            return NBH_E_INVALID;
+        }
         if (c->max_depth > 21) { why = "max_depth must be <= 21"; return NBH_E_INVALID; }
         if (c->m2l > NBH_M2L_BF16X3) { why = "unknown m2l kernel"; return NBH_E_INVALID; }
         if (c->p2p > NBH_P2P_LANE) { why = "unknown p2p kernel"; return NBH_E_INVALID; }
@@ -293,6 +298,7 @@ static FmmParams fmm_params(nbh_handle* h) {
     F.lists.next = h->row_next.as<uint32_t>();
     F.lists.m_start = h->row_m_start.as<uint32_t>();
     F.lists.m_cnt = h->row_m_cnt.as<uint32_t>();
+    F.lists.mx_cnt = h->row_mx_cnt.as<uint32_t>();
     F.lists.p_start = h->row_p_start.as<uint32_t>();
     F.lists.p_cnt = h->row_p_cnt.as<uint32_t>();
     F.lists.m_pool = h->m2l_pool.as<int>();
@@ -307,7 +313,7 @@ static FmmParams fmm_params(nbh_handle* h) {
 static int grow_rows(nbh_handle* h, uint32_t cap) {
     h->row_cap = cap;
     DevBuf* rows[] = {&h->row_node, &h->row_next, &h->row_sl_start, &h->row_sl_cnt,
-                      &h->row_m_start, &h->row_m_cnt, &h->row_p_start, &h->row_p_cnt};
+                      &h->row_m_start, &h->row_m_cnt, &h->row_mx_cnt, &h->row_p_start, &h->row_p_cnt};
     for (DevBuf* b : rows) HIPCHK(h, b->ensure((size_t)cap * 4));
     return NBH_OK;
 }
@@ -358,6 +364,7 @@ static int run_walk(nbh_handle* h) {
         W.row_sl_cnt = h->row_sl_cnt.as<uint32_t>();
         W.row_m_start = h->row_m_start.as<uint32_t>();
         W.row_m_cnt = h->row_m_cnt.as<uint32_t>();
+        W.row_mx_cnt = h->row_mx_cnt.as<uint32_t>();
         W.row_p_start = h->row_p_start.as<uint32_t>();
         W.row_p_cnt = h->row_p_cnt.as<uint32_t>();
         W.sl = h->sl.as<int>();
@@ -406,6 +413,20 @@ static int run_walk(nbh_handle* h) {
     return NBH_OK;
 }
 
+// M2L truncation |k| + |n| <= q
+static int m2l_q(const nbh_handle* h) {
+    return h->cfg.m2l_order ? (int)h->cfg.m2l_order : (int)h->cfg.order + 1;
+}
+
+static M2LPlan m2l_plan(const nbh_handle* h) {
+    M2LPlan pl{};
+    pl.mode = h->cfg.m2l == NBH_M2L_PLAIN ? 0 : 1;
+    pl.num_nodes = h->num_nodes;
+    std::copy(h->hc->level_off, h->hc->level_off + MAX_LEVELS + 2, pl.level_off);
+    std::copy(h->hc->level_count, h->hc->level_count + MAX_LEVELS + 1, pl.level_count);
+    return pl;
+}
+
 // One force evaluation; kick_dt != 0 applies v += a * kick_dt afterwards.
 static int force_fmm(nbh_handle* h, double kick_dt) {
     const int p = (int)h->cfg.order;
@@ -421,25 +442,48 @@ static int force_fmm(nbh_handle* h, double kick_dt) {
     CHK(run_walk(h));
     record(h, 3);
     F = fmm_params(h);
-    if (h->cfg.m2l == NBH_M2L_PLAIN) {
-        launch_m2l_plain(p, F, h->num_nodes, h->stream);
-    } else {
-        // derivative tensors of every same-level offset |o_i| <= K, levels
+    const int q = m2l_q(h);
+    if (h->cfg.m2l != NBH_M2L_PLAIN && h->nsys == 1 && h->depth >= 2) {
+        // derivative tensors of the same-level offsets [0, K]^3 of levels
         // 2..depth (cells of levels 0 and 1 are never well separated); one
-        // system only, since the tensors depend on the root size
-        if (h->nsys == 1 && h->depth >= 2) {
-            const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
-            const int K = 2 * Kp + 1, side = 2 * K + 1;
-            const int nlev = (int)h->depth - 1;
-            HIPCHK(h, h->dtab.ensure((size_t)side * side * side * nlev * F.ncp * 4));
+        // system only, since the tensors depend on the root size; used when
+        // a level's table fits in one workgroup's LDS
+        const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
+        const int K = 2 * Kp + 1;
+        const int dsz = (K + 1) * (K + 1) * (K + 1), dcp = ncp_of(q);
+        const int nlev = (int)h->depth - 1;
+        if ((size_t)dsz * dcp * 4 <= (size_t)h->max_lds) {
+            HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4));
             F.dtab = h->dtab.as<float>();
             F.dK = K;
             F.dlevel0 = 2;
             F.dnlev = nlev;
-            launch_dtable(p, F, 2, nlev, K, h->stream);
+            F.dcp = dcp;
+            F.dsz = dsz;
+            launch_dtable(q, F, F.dtab, 2, nlev, K + 1, 0, 1, 0, h->stream);
+            // pairs one level apart, by finer level 1..depth: offsets |d_i| <=
+            // 4 Kp + 1 half finer cells (the coarser side's parent pair was
+            // split), tabled at d_i = 1, 3, .., 4 Kp + 1
+            const int E = 2 * Kp + 1;
+            const int xn = (int)h->depth;
+            HIPCHK(h, h->xtab.ensure((size_t)E * E * E * xn * dcp * 4));
+            F.xtab = h->xtab.as<float>();
+            F.xE = E;
+            F.xlevel0 = 1;
+            F.xnlev = xn;
+            launch_dtable(q, F, F.xtab, 1, xn, E, 1, 2, 1, h->stream);
+            // deferred pairs: at most every other-level entry, one block per target
+            const unsigned long long nx = std::max<unsigned long long>(h->hc->m2lx_total, 1);
+            HIPCHK(h, h->far_src.ensure((size_t)nx * 4));
+            HIPCHK(h, h->far_blk.ensure((size_t)h->num_nodes * 16));
+            F.far_src = h->far_src.as<int>();
+            F.far_blk = h->far_blk.as<uint4>();
+            F.far_cap = (uint32_t)std::min<unsigned long long>(nx, 0xffffffffull);
+            F.far_blk_cap = h->num_nodes;
+            F.far_cnt = &h->counters.as<Counters>()->far_blocks;
         }
-        launch_m2l_grouped(p, F, h->num_nodes, h->stream);
     }
+    launch_m2l(p, q, F, m2l_plan(h), h->stream);
     record(h, 4);
     for (uint32_t l = 1; l <= h->depth; ++l)
         launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
@@ -611,6 +655,7 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
         }                                                                                                   \
     } while (0)
     CCHK(hipSetDevice(h->device));
+    CCHK(hipDeviceGetAttribute(&h->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
     CCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     for (auto& e : h->ev) CCHK(hipEventCreate(&e));
     CCHK(h->st[0].alloc(n));
@@ -693,9 +738,9 @@ void nbh_destroy(nbh_handle* h) {
     h->st[1].release();
     DevBuf* all[] = {&h->sys_off, &h->keys, &h->keys_alt, &h->perm, &h->perm_alt, &h->sort_tmp, &h->tree_tmp,
                      &h->scan_tmp, &h->box, &h->inv, &h->unit, &h->bbox_part, &h->geo, &h->link, &h->nsys_of,
-                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->row_node, &h->row_first,
+                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->xtab, &h->far_src, &h->far_blk, &h->row_node, &h->row_first,
                      &h->row_next, &h->row_sl_start, &h->row_sl_cnt, &h->row_m_start, &h->row_m_cnt,
-                     &h->row_p_start, &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->counters, &h->diag,
+                     &h->row_mx_cnt, &h->row_p_start, &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->counters, &h->diag,
                      &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf};
     for (DevBuf* b : all) b->release();
     if (h->hc) (void)hipHostFree(h->hc);
@@ -880,7 +925,7 @@ int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
     Fw.M = h->Mw.as<float>();
     Fw.L = h->Lw.as<float>();
     State S = h->st[h->cur].view();
-    launch_force_vjp(p, F, Fw, h->wbuf.as<double>(), S.id, h->pw.as<float4>(), (uint32_t)n, h->num_leaves,
+    launch_force_vjp(p, m2l_q(h), m2l_plan(h), F, Fw, h->wbuf.as<double>(), S.id, h->pw.as<float4>(), (uint32_t)n, h->num_leaves,
                      h->hc->level_count, h->hc->level_off, (int)h->depth, h->gbuf.as<double>(), h->stream);
     HIPCHK(h, hipGetLastError());
     std::vector<double> g(4 * n);
@@ -979,6 +1024,9 @@ int nbh_debug_lists(nbh_handle* h, int64_t* m2l_off, int32_t* m2l_src, int64_t* 
             for (uint32_t k = 0; k < pc[r]; ++k, ++op)
                 if (p2p_src) p2p_src[op] = ppool[ps[r] + k];
         }
+        // rows keep same-level M2L sources ahead of the others; report each
+        // list in ascending source order
+        if (m2l_src && m2l_off) std::sort(m2l_src + m2l_off[a], m2l_src + om);
     }
     if (m2l_off) m2l_off[nn] = om;
     if (p2p_off) p2p_off[nn] = op;

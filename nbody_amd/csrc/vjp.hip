@@ -128,33 +128,34 @@ __global__ __launch_bounds__(256) void vjp_final_kernel(FmmParams F, const float
 }
 
 template <int P>
-static void launch_vjp_t(const FmmParams& F, const FmmParams& Fw, const double* w_in, const uint32_t* id,
-                         float4* pw, uint32_t n, uint32_t num_leaves, const uint32_t* level_count,
-                         const uint32_t* level_off, int depth, double* g, hipStream_t st) {
+static void launch_vjp_t(int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const double* w_in,
+                         const uint32_t* id, float4* pw, uint32_t n, uint32_t num_leaves,
+                         const uint32_t* level_count, const uint32_t* level_off, int depth, double* g,
+                         hipStream_t st) {
     hipLaunchKernelGGL(vjp_gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, w_in, id, pw, n);
     hipLaunchKernelGGL((dipole_p2m_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, pw, Fw.M,
                        num_leaves);
     for (int l = depth - 1; l >= 0; --l) launch_m2m_level(P, Fw, l, level_count[l], level_off[l], st);
-    uint32_t nn = 0;
-    for (int l = 0; l <= depth; ++l) nn += level_count[l];
-    launch_m2l_grouped(P, Fw, nn, st);
+    launch_m2l(P, q, Fw, plan, st);
     for (int l = 1; l <= depth; ++l) launch_l2l_level(P, Fw, l, level_count[l], level_off[l], st);
     hipLaunchKernelGGL((vjp_final_kernel<P>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, pw, Fw.L, n, g, g + n,
                        g + 2 * (size_t)n, g + 3 * (size_t)n);
 }
 
-void launch_force_vjp(int p, const FmmParams& F, const FmmParams& Fw, const double* w_in, const uint32_t* id,
-                      float4* pw, uint32_t n, uint32_t num_leaves, const uint32_t* level_count,
+void launch_force_vjp(int p, int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const double* w_in,
+                      const uint32_t* id, float4* pw, uint32_t n, uint32_t num_leaves, const uint32_t* level_count,
                       const uint32_t* level_off, int depth, double* g, hipStream_t st) {
+#define NBH_VJP(PP) launch_vjp_t<PP>(q, plan, F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st)
     switch (p) {
-        case 1: launch_vjp_t<1>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
-        case 2: launch_vjp_t<2>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
-        case 3: launch_vjp_t<3>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
-        case 4: launch_vjp_t<4>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
-        case 5: launch_vjp_t<5>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
-        case 6: launch_vjp_t<6>(F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st); break;
+        case 1: NBH_VJP(1); break;
+        case 2: NBH_VJP(2); break;
+        case 3: NBH_VJP(3); break;
+        case 4: NBH_VJP(4); break;
+        case 5: NBH_VJP(5); break;
+        case 6: NBH_VJP(6); break;
         default: break;
     }
+#undef NBH_VJP
 }
 
 }  // namespace nbh

@@ -164,6 +164,7 @@ __global__ __launch_bounds__(64) void walk_root_kernel(WalkParams W) {
         W.row_node[r] = A;
         W.row_m_start[r] = 0;
         W.row_m_cnt[r] = 0;
+        W.row_mx_cnt[r] = 0;
         uint32_t pc = 0, ps = 0, sc = 0, ss = 0;
         if (leaf) {
             ps = atomicAdd(&C->p2p_pool_used, 1u);
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
     const int bits = W.bits;
     const uint64_t num4 = W.num4, den3 = W.den3;
     uint2 mchunk = make_uint2(0, 0), pchunk = make_uint2(0, 0), schunk = make_uint2(0, 0);  // lane 0 only
-    unsigned long long acc_m = 0, acc_p = 0, acc_s = 0;                                      // lane 0 only
+    unsigned long long acc_m = 0, acc_p = 0, acc_s = 0, acc_x = 0;                           // lane 0 only
     for (uint32_t g = wid; g < n_groups; g += nw) {
         const uint32_t rg = base_prev + g;
         if (rg >= W.row_cap) continue;  // flagged when the row was made
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
             if (lane < nt && r < W.row_cap) {
                 W.row_node[r] = A0 + lane;
                 W.row_next[r] = NO_ROW;
-                W.row_m_start[r] = W.row_m_cnt[r] = 0;
+                W.row_m_start[r] = W.row_m_cnt[r] = W.row_mx_cnt[r] = 0;
                 W.row_p_start[r] = W.row_p_cnt[r] = 0;
                 W.row_sl_start[r] = W.row_sl_cnt[r] = 0;
             }
@@ -276,9 +277,10 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
         const bool filled = sl_n <= (uint32_t)SL_CACHE;
         if (filled && sl_n) sl_cache_fill(W, cache, sl_st, 0, sl_n);
         // per-candidate part of the MAC, then the per-target categories
-        // (0 = M2L, 1 = P2P, 2 = split, 3 = none)
+        // (0 = M2L same level, 4 = M2L other level, 1 = P2P, 2 = split, 3 = none)
         auto classify_all = [&](bool valid, int B, int4 gb, auto&& per_target) {
-            const int sb = bits - (gb.w & 0xff);
+            const int lb = gb.w & 0xff;
+            const int sb = bits - lb;
             const int cbx = (2 * gb.x + 1) << sb, cby = (2 * gb.y + 1) << sb, cbz = (2 * gb.z + 1) << sb;
             const uint64_t side = (uint64_t)side_a + (1ull << (sb + 1));
             const uint64_t lhs = den3 * side * side;
@@ -289,19 +291,20 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
                     const int64_t dx = cbx - cax[t], dy = cby - cay[t], dz = cbz - caz[t];
                     const uint64_t dsq = (uint64_t)(dx * dx + dy * dy + dz * dz);
                     const bool approx = B != A0 + t && lhs < num4 * dsq;
-                    const int cat = !valid ? 3 : approx ? 0 : (leaf_t[t] && b_leaf) ? 1 : 2;
+                    const int cat = !valid ? 3 : approx ? (lb == lt_level ? 0 : 4) : (leaf_t[t] && b_leaf) ? 1 : 2;
                     per_target(t, cat);
                 }
             }
         };
         // pass 1: counts per target (uniform)
-        uint32_t nm[MAXT], np[MAXT], ns[MAXT];
+        uint32_t nm[MAXT], nx[MAXT], np[MAXT], ns[MAXT];
 #pragma unroll
-        for (int t = 0; t < MAXT; ++t) nm[t] = np[t] = ns[t] = 0;
+        for (int t = 0; t < MAXT; ++t) nm[t] = nx[t] = np[t] = ns[t] = 0;
         if (sl_n) {
             enumerate_sl(W, cache, sl_st, sl_n, filled, [&](bool valid, int B, int4 gb) {
                 classify_all(valid, B, gb, [&](int t, int cat) {
                     nm[t] += __popcll(__ballot(cat == 0));
+                    nx[t] += __popcll(__ballot(cat == 4));
                     np[t] += __popcll(__ballot(cat == 1));
                     ns[t] += __popcll(__ballot(cat == 2));
                 });
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
         }
         uint32_t tm = 0, tp = 0, ts = 0;
 #pragma unroll
-        for (int t = 0; t < MAXT; ++t) { tm += nm[t]; tp += np[t]; ts += ns[t]; }
+        for (int t = 0; t < MAXT; ++t) { tm += nm[t] + nx[t]; tp += np[t]; ts += ns[t]; }
         // reserve one block per pool for all targets of the group
         uint32_t ms0 = 0, ps0 = 0, ss0 = 0, fail = 0;
         if (lane == 0) {
@@ -325,7 +328,11 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
                 ss0 = pool_take(schunk, ts, POOL_CHUNK_S, &C->sl_used, W.sl_cap);
                 if (ss0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_SL); fail = 1; }
             }
-            if (!fail) { acc_m += tm; acc_p += tp; acc_s += ts; }
+            if (!fail) {
+                acc_m += tm; acc_p += tp; acc_s += ts;
+#pragma unroll
+                for (int t = 0; t < MAXT; ++t) acc_x += nx[t];
+            }
         }
         fail = __shfl(fail, 0, WAVE);
         ms0 = __shfl(ms0, 0, WAVE);
@@ -333,21 +340,23 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
         ss0 = __shfl(ss0, 0, WAVE);
         if (fail) {  // the host retries with larger pools; leave empty rows
 #pragma unroll
-            for (int t = 0; t < MAXT; ++t) nm[t] = np[t] = ns[t] = 0;
+            for (int t = 0; t < MAXT; ++t) nm[t] = nx[t] = np[t] = ns[t] = 0;
             tm = tp = ts = 0;
         }
-        // per-target segment starts (uniform) and the rows (lane t)
-        uint32_t cm[MAXT], cp[MAXT], cs[MAXT];
+        // per-target segment starts (uniform) and the rows (lane t); a row's
+        // M2L segment holds its same-level sources, then the others
+        uint32_t cm[MAXT], cx[MAXT], cp[MAXT], cs[MAXT];
         {
             uint32_t om = ms0, op = ps0, os = ss0;
-            uint32_t my_ms = 0, my_nm = 0, my_ps = 0, my_np = 0, my_ss = 0, my_ns = 0;
+            uint32_t my_ms = 0, my_nm = 0, my_nx = 0, my_ps = 0, my_np = 0, my_ss = 0, my_ns = 0;
 #pragma unroll
             for (int t = 0; t < MAXT; ++t) {
-                cm[t] = om; cp[t] = op; cs[t] = os;
+                cm[t] = om; cx[t] = om + nm[t]; cp[t] = op; cs[t] = os;
                 if (lane == t) {
-                    my_ms = om; my_nm = nm[t]; my_ps = op; my_np = np[t]; my_ss = os; my_ns = ns[t];
+                    my_ms = om; my_nm = nm[t] + nx[t]; my_nx = nx[t];
+                    my_ps = op; my_np = np[t]; my_ss = os; my_ns = ns[t];
                 }
-                om += nm[t]; op += np[t]; os += ns[t];
+                om += nm[t] + nx[t]; op += np[t]; os += ns[t];
             }
             if (lane < nt) {
                 const uint32_t r = row0 + lane;
@@ -355,6 +364,7 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
                 W.row_next[r] = NO_ROW;
                 W.row_m_start[r] = my_ms;
                 W.row_m_cnt[r] = my_nm;
+                W.row_mx_cnt[r] = my_nx;
                 W.row_p_start[r] = my_ps;
                 W.row_p_cnt[r] = my_np;
                 W.row_sl_start[r] = my_ss;
@@ -368,7 +378,10 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
             enumerate_sl(W, cache, sl_st, sl_n, filled, [&](bool valid, int B, int4 gb) {
                 classify_all(valid, B, gb, [&](int t, int cat) {
                     const uint64_t bm = __ballot(cat == 0), bp = __ballot(cat == 1), bs = __ballot(cat == 2);
+                    const uint64_t bx = __ballot(cat == 4);
                     if (cat == 0) W.m2l_pool[cm[t] + __popcll(bm & lt)] = B;
+                    if (cat == 4) W.m2l_pool[cx[t] + __popcll(bx & lt)] = B;
+                    cx[t] += __popcll(bx);
                     if (cat == 1) W.p2p_pool[cp[t] + __popcll(bp & lt)] = B;
                     if (cat == 2) W.sl[cs[t] + __popcll(bs & lt)] = B;
                     cm[t] += __popcll(bm);
@@ -383,6 +396,7 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
         if (acc_m) atomicAdd(&C->m2l_total, acc_m);
         if (acc_p) atomicAdd(&C->p2p_total, acc_p);
         if (acc_s) atomicAdd(&C->split_total, acc_s);
+        if (acc_x) atomicAdd(&C->m2lx_total, acc_x);
     }
 }
 

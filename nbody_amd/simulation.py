@@ -90,6 +90,8 @@ class HipSimulation(Simulation):
         mass = _c64(mass)
         charge = None if charge is None else _c64(charge)
         self.config = cfg
+        # M2L truncation |k| + |n| <= m2l_order (0 selects order + 1)
+        self.m2l_order = int(cfg.m2l_order) or int(cfg.order) + 1
         h = ctypes.c_void_p()
         L = lib()
         if sys_off is None:

@@ -470,17 +470,17 @@ void orc_m2m(int p, const double* zc, const double* zp, const double* Mc, double
     }
 }
 
-void orc_m2l(int p, const double* zA, const double* zB, double eps, const double* M, double* L) {
-    MultiIndex T(p);
+void orc_m2l(int p, int q, const double* zA, const double* zB, double eps, const double* M, double* L) {
+    MultiIndex T(p), TQ(q);
     double R[3] = {zA[0] - zB[0], zA[1] - zB[1], zA[2] - zB[2]};
     std::vector<double> D;
-    derivs(T, R, eps, D);
+    derivs(TQ, R, eps, D);
     for (int k = 0; k < T.size(); ++k) {
         const auto& kk = T.mi[k];
         double acc = 0;
         for (int n = 0; n < T.size(); ++n) {
             const auto& nn = T.mi[n];
-            int s = T.at(kk[0] + nn[0], kk[1] + nn[1], kk[2] + nn[2]);
+            int s = TQ.at(kk[0] + nn[0], kk[1] + nn[1], kk[2] + nn[2]);
             if (s < 0) continue;
             double sign = ((nn[0] + nn[1] + nn[2]) & 1) ? -1.0 : 1.0;
             acc += sign * D[s] * M[n];
@@ -536,7 +536,7 @@ static void cell_centre(const Tree& t, const double* box, int node, double* c) {
     c[2] = box[4 * s + 2] + (t.iz[node] + 0.5) * h;
 }
 
-void orc_fmm(const orc_tree* tr, const orc_lists* li, int p, size_t n, const double* x,
+void orc_fmm(const orc_tree* tr, const orc_lists* li, int p, int qm2l, size_t n, const double* x,
              const double* q, const double* box, double G, double eps,
              double* acc, double* pot) {
     const Tree& t = tr->t;
@@ -563,7 +563,7 @@ void orc_fmm(const orc_tree* tr, const orc_lists* li, int p, size_t n, const dou
     for (long long a = 0; a < (long long)nn; ++a)
         for (int64_t e = L.m2l_off[a]; e < L.m2l_off[a + 1]; ++e) {
             int b = L.m2l_src[e];
-            orc_m2l(p, &cen[3 * a], &cen[3 * b], eps, &M[(size_t)b * nc], &Lc[(size_t)a * nc]);
+            orc_m2l(p, qm2l, &cen[3 * a], &cen[3 * b], eps, &M[(size_t)b * nc], &Lc[(size_t)a * nc]);
         }
     // L2L downwards (BFS order: parents before children)
     for (size_t i = 0; i < nn; ++i)
@@ -606,7 +606,7 @@ void orc_fmm(const orc_tree* tr, const orc_lists* li, int p, size_t n, const dou
 // T(r)(m_i w_j - m_j w_i) and -(w_j . r)/s^3.  Far field: Hessian of the mass
 // local expansion applied to w_i, plus m_i grad psi and psi from the dipole
 // moments M^w_n = sum_i sum_e w_ie b^(n-e)/(n-e)! run through M2M, M2L, L2L.
-void orc_fmm_vjp(const orc_tree* tr, const orc_lists* li, int p, size_t n, const double* x, const double* m,
+void orc_fmm_vjp(const orc_tree* tr, const orc_lists* li, int p, int qm2l, size_t n, const double* x, const double* m,
                  const double* w, const double* box, double G, double eps, double* gx, double* gm) {
     const Tree& t = tr->t;
     const Lists& L = li->l;
@@ -644,8 +644,8 @@ void orc_fmm_vjp(const orc_tree* tr, const orc_lists* li, int p, size_t n, const
     for (long long a = 0; a < (long long)nn; ++a)
         for (int64_t e = L.m2l_off[a]; e < L.m2l_off[a + 1]; ++e) {
             int b = L.m2l_src[e];
-            orc_m2l(p, &cen[3 * a], &cen[3 * b], eps, &M[(size_t)b * nc], &Lc[(size_t)a * nc]);
-            orc_m2l(p, &cen[3 * a], &cen[3 * b], eps, &Mw[(size_t)b * nc], &Lw[(size_t)a * nc]);
+            orc_m2l(p, qm2l, &cen[3 * a], &cen[3 * b], eps, &M[(size_t)b * nc], &Lc[(size_t)a * nc]);
+            orc_m2l(p, qm2l, &cen[3 * a], &cen[3 * b], eps, &Mw[(size_t)b * nc], &Lw[(size_t)a * nc]);
         }
     for (size_t a = 0; a < nn; ++a)
         for (int c = 0; c < t.nchild[a]; ++c) {

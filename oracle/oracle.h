@@ -91,8 +91,9 @@ void orc_derivs(int p, const double* R, double eps, double* D);
 void orc_p2m(int p, const double* z, size_t n, const double* x, const double* q, double* M);
 /* shift multipole from child centre zc to parent centre zp */
 void orc_m2m(int p, const double* zc, const double* zp, const double* Mc, double* Mp);
-/* L_k += sum_{|k|+|n|<=p} (-1)^|n| D_{k+n}(zA - zB) M_n */
-void orc_m2l(int p, const double* zA, const double* zB, double eps, const double* M, double* L);
+/* L_k += sum_{|k|,|n|<=p, |k|+|n|<=q} (-1)^|n| D_{k+n}(zA - zB) M_n  (q >= p;
+ * q == p is the usual total-degree truncation) */
+void orc_m2l(int p, int q, const double* zA, const double* zB, double eps, const double* M, double* L);
 /* shift local expansion from parent zp to child zc (accumulates into Lc) */
 void orc_l2l(int p, const double* zp, const double* zc, const double* Lp, double* Lc);
 /* evaluate: phi_L(x) = sum_k L_k a^k/k!, g = sum_k L_{k+e} a^k/k! (a = x - z) */
@@ -101,13 +102,13 @@ void orc_l2p(int p, const double* z, const double* L, const double* x, double* p
 /* ---- full FMM force evaluation (FP64), same tree/lists as the product ---- */
 /* x[3n] in sorted order, q[n]; returns acc (without the q_i/m_i factor) and
  * potential per particle, in the same order.  box[4*nsys] from orc_morton. */
-void orc_fmm(const orc_tree*, const orc_lists*, int p, size_t n, const double* x,
+void orc_fmm(const orc_tree*, const orc_lists*, int p, int qm2l, size_t n, const double* x,
              const double* q, const double* box, double G, double eps,
              double* acc, double* pot);
 
 /* VJP of orc_fmm for gravity (q == m): gx[3n], gm[n] of sum_i w_i . acc_i,
  * with the tree and lists held fixed (exact transpose of the FMM). */
-void orc_fmm_vjp(const orc_tree*, const orc_lists*, int p, size_t n, const double* x, const double* m,
+void orc_fmm_vjp(const orc_tree*, const orc_lists*, int p, int qm2l, size_t n, const double* x, const double* m,
                  const double* w, const double* box, double G, double eps, double* gx, double* gm);
 
 /* ---- integrators over the FP64 direct sum ------------------------------- */

@@ -48,7 +48,7 @@ def lib():
         L.orc_lists_free.argtypes = [ctypes.c_void_p]
         L.orc_lists_sizes.argtypes = [ctypes.c_void_p, _i64p]
         L.orc_lists_get.argtypes = [ctypes.c_void_p, _i64p, _i32p, _i64p, _i32p]
-        L.orc_fmm.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t,
+        L.orc_fmm.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
                               _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
         _LIB = L
     return _LIB
@@ -212,10 +212,12 @@ def m2m(p, zc, zp, Mc):
     return Mp
 
 
-def m2l(p, zA, zB, M, eps=0.0):
+def m2l(p, zA, zB, M, eps=0.0, m2l_order=None):
+    """m2l_order q: keep |k| + |n| <= q (default p)."""
     zA = c64(zA); zB = c64(zB); M = c64(M)
     L = np.zeros(ncoef(p))
-    lib().orc_m2l(ctypes.c_int(p), _ptr(zA, _dp), _ptr(zB, _dp), ctypes.c_double(eps), _ptr(M, _dp), _ptr(L, _dp))
+    q = p if m2l_order is None else int(m2l_order)
+    lib().orc_m2l(ctypes.c_int(p), ctypes.c_int(q), _ptr(zA, _dp), _ptr(zB, _dp), ctypes.c_double(eps), _ptr(M, _dp), _ptr(L, _dp))
     return L
 
 
@@ -234,18 +236,19 @@ def l2p(p, z, L, x):
     return phi.value, g
 
 
-def fmm(tree: Tree, lists: Lists, p, x_sorted, q_sorted, box, G=1.0, eps=0.0):
+def fmm(tree: Tree, lists: Lists, p, x_sorted, q_sorted, box, G=1.0, eps=0.0, m2l_order=None):
     x = c64(x_sorted).reshape(-1, 3)
     q = c64(q_sorted)
     box = c64(box).reshape(-1)
     acc = np.zeros_like(x)
     pot = np.zeros(len(x))
-    lib().orc_fmm(ctypes.c_void_p(tree.handle), ctypes.c_void_p(lists.handle), p, len(x),
+    lib().orc_fmm(ctypes.c_void_p(tree.handle), ctypes.c_void_p(lists.handle), p,
+                  p if m2l_order is None else int(m2l_order), len(x),
                   _ptr(x, _dp), _ptr(q, _dp), _ptr(box, _dp), G, eps, _ptr(acc, _dp), _ptr(pot, _dp))
     return acc, pot
 
 
-def fmm_vjp(tree: Tree, lists: Lists, p, x_sorted, m_sorted, w_sorted, box, G=1.0, eps=0.0):
+def fmm_vjp(tree: Tree, lists: Lists, p, x_sorted, m_sorted, w_sorted, box, G=1.0, eps=0.0, m2l_order=None):
     x = c64(x_sorted).reshape(-1, 3)
     m = c64(m_sorted)
     w = c64(w_sorted).reshape(-1, 3)
@@ -253,9 +256,10 @@ def fmm_vjp(tree: Tree, lists: Lists, p, x_sorted, m_sorted, w_sorted, box, G=1.
     gx = np.zeros_like(x)
     gm = np.zeros(len(x))
     L = lib()
-    L.orc_fmm_vjp.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t,
+    L.orc_fmm_vjp.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
                               _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
-    L.orc_fmm_vjp(ctypes.c_void_p(tree.handle), ctypes.c_void_p(lists.handle), p, len(x), _ptr(x, _dp),
+    L.orc_fmm_vjp(ctypes.c_void_p(tree.handle), ctypes.c_void_p(lists.handle), p,
+                  p if m2l_order is None else int(m2l_order), len(x), _ptr(x, _dp),
                   _ptr(m, _dp), _ptr(w, _dp), _ptr(box, _dp), G, eps, _ptr(gx, _dp), _ptr(gm, _dp))
     return gx, gm
 

@@ -25,11 +25,11 @@ def max_rel(a, ref):
 
 
 def oracle_pipeline(x_sorted, q_sorted, box, cap, max_depth, theta_num, theta_den, p, eps, G=1.0,
-                    sys_off=None, bits=21):
+                    sys_off=None, bits=21, m2l_order=None):
     """Oracle tree + lists + FP64 FMM on particles already in the GPU's order."""
     keys, box_o = po.morton(x_sorted, sys_off=sys_off, bits=bits)
     assert np.all(np.diff(keys.astype(np.float64)) >= 0) or np.all(keys[1:] >= keys[:-1])
     tree = po.build_tree(keys, sys_off=sys_off, bits=bits, cap=cap, max_depth=max_depth)
     lists = po.walk(tree, theta_num, theta_den)
-    acc, pot = po.fmm(tree, lists, p, x_sorted, q_sorted, box_o, G=G, eps=eps)
+    acc, pot = po.fmm(tree, lists, p, x_sorted, q_sorted, box_o, G=G, eps=eps, m2l_order=m2l_order)
     return keys, box_o, tree, lists, acc, pot

@@ -34,7 +34,7 @@ def test_force_vjp_matches_oracle_transpose(kind, n, cap, p):
     keys, ids, box = sim.debug_keys()
     tree = po.build_tree(keys, cap=cap)
     lists = po.walk(tree, 1, 2)
-    gxo, gmo = po.fmm_vjp(tree, lists, p, x[ids], m[ids], w[ids], box, eps=eps)
+    gxo, gmo = po.fmm_vjp(tree, lists, p, x[ids], m[ids], w[ids], box, eps=eps, m2l_order=sim.m2l_order)
     inv = np.empty(n, dtype=np.int64)
     inv[ids] = np.arange(n)
     e_x = rms_rel(gx, gxo[inv])

@@ -40,7 +40,7 @@ def test_ensemble_matches_oracle_and_direct():
     # no pair crosses systems
     tsys = np.repeat(tree.sys, np.diff(lists.m2l_off))
     assert np.array_equal(tsys, tree.sys[lists.m2l_src])
-    acc_o, _ = po.fmm(tree, lists, 4, x[ids], m[ids], box_o, eps=1e-2)
+    acc_o, _ = po.fmm(tree, lists, 4, x[ids], m[ids], box_o, eps=1e-2, m2l_order=sim.m2l_order)
     assert rms_rel(acc[ids], acc_o) < 2e-5
     # per-system error vs the FP64 direct sum on up to 256 targets each
     worst = 0.0

@@ -39,15 +39,18 @@ def _ics(kind, n, seed=0):
     return nb.ics.plummer(n, seed + 1)
 
 
-KERNELS = [("tuned", nb.M2L_GROUPED, nb.P2P_LANE), ("plain", nb.M2L_PLAIN, nb.P2P_LEAF)]
+# name, M2L kernel, P2P kernel, M2L truncation (0: |k|+|n| <= p+1, the default; -1: <= p)
+KERNELS = [("tuned", nb.M2L_GROUPED, nb.P2P_LANE, 0), ("tuned_qp", nb.M2L_GROUPED, nb.P2P_LANE, -1),
+           ("plain", nb.M2L_PLAIN, nb.P2P_LEAF, 0)]
 
 
-@pytest.mark.parametrize("kern,m2l,p2p", KERNELS)
+@pytest.mark.parametrize("kern,m2l,p2p,qsel", KERNELS)
 @pytest.mark.parametrize("name,kind,n,cap,p,th,eps", CASES)
-def test_tree_lists_forces_match_oracle(name, kind, n, cap, p, th, eps, kern, m2l, p2p):
+def test_tree_lists_forces_match_oracle(name, kind, n, cap, p, th, eps, kern, m2l, p2p, qsel):
     _case(f"{name}/{kern}")
     x, v, m = _ics(kind, n)
-    sim = make_sim(x, v, m, leaf_capacity=cap, order=p, theta=th[0] / th[1], eps=eps, m2l=m2l, p2p=p2p)
+    sim = make_sim(x, v, m, leaf_capacity=cap, order=p, theta=th[0] / th[1], eps=eps, m2l=m2l, p2p=p2p,
+                   m2l_order=p if qsel < 0 else 0)
     acc, pot = sim.forces()
     keys, ids, box, xs, qs = _gpu_sorted(sim, x, m)
     # keys, box and the stable particle order
@@ -69,7 +72,7 @@ def test_tree_lists_forces_match_oracle(name, kind, n, cap, p, th, eps, kern, m2
     assert np.array_equal(pofs, lists.p2p_off)
     assert np.array_equal(ps, lists.p2p_src)
     # forces: FP32 kernels vs FP64 oracle FMM on the same tree and lists
-    acc_o, pot_o = po.fmm(tree, lists, p, xs, qs, box_o, G=1.0, eps=eps)
+    acc_o, pot_o = po.fmm(tree, lists, p, xs, qs, box_o, G=1.0, eps=eps, m2l_order=sim.m2l_order)
     inv = np.empty_like(ids)
     inv[ids] = np.arange(n, dtype=np.uint32)
     acc_s = acc[ids]
@@ -133,9 +136,9 @@ def test_fmm_converges_to_direct(p, err):
     sim.close()
 
 
-def test_kdk_trajectory_c1():
+def test_kdk_trajectory_uniform4096():
     """Uniform cube N=4096 (seed 0), eps=1e-2, theta=0.5, cap 16, 10 KDK steps."""
-    _case("c1_kdk_10_steps")
+    _case("uniform4096_kdk_10_steps")
     n = 4096
     x, v, m = nb.ics.uniform(n, 0)
     sim = make_sim(x, v, m, leaf_capacity=16, order=4, eps=1e-2, dt=1e-3)

@@ -396,10 +396,47 @@ def test_euler_and_kick_drift_one_step_closed_form():
     assert np.allclose(vk, v + a0 * dt) and np.allclose(xk, x + (v + a0 * dt) * dt)
 
 
-def test_fmm_vjp_is_the_exact_transpose_of_the_fmm():
+def test_m2l_extended_truncation_is_the_higher_order_operator_restricted():
+    """orc_m2l(p, q=p+1) keeps |k|, |n| <= p with |k|+|n| <= p+1: it must equal
+    the order-(p+1) M2L applied to M padded with zeros above order p, read at
+    |k| <= p (the order-(p+1) operator is pinned by the sympy derivative test
+    and the convergence tests above)."""
+    rng = np.random.default_rng(21)
+    zA, zB = np.array([0.3, -0.2, 0.1]), np.array([2.1, 0.7, -1.4])
+    for p in [1, 2, 3, 4, 5]:
+        M = rng.standard_normal(po.ncoef(p))
+        Mpad = np.zeros(po.ncoef(p + 1))
+        Mpad[:po.ncoef(p)] = M
+        ext = po.m2l(p, zA, zB, M, eps=0.03, m2l_order=p + 1)
+        ref = po.m2l(p + 1, zA, zB, Mpad, eps=0.03)[:po.ncoef(p)]
+        assert np.allclose(ext, ref, rtol=1e-13, atol=1e-15 * np.abs(ref).max())
+        # and it differs from the total-degree truncation exactly by the
+        # degree-(p+1) cross terms (nonzero whenever p >= 1)
+        assert not np.allclose(ext, po.m2l(p, zA, zB, M, eps=0.03))
+
+
+def test_fmm_extended_m2l_is_more_accurate():
+    rng = np.random.default_rng(12)
+    x = rng.random((1500, 3))
+    q = np.full(1500, 1 / 1500)
+    keys, o, t, box = _tree(x, cap=12)
+    L = po.walk(t, 1, 2)
+    ad, _ = po.direct(x[o], q, eps=1e-2)
+
+    def err(p, qq):
+        a, _ = po.fmm(t, L, p, x[o], q, box, eps=1e-2, m2l_order=qq)
+        return np.sqrt(np.mean(np.sum((a - ad) ** 2, 1)) / np.mean(np.sum(ad ** 2, 1)))
+
+    for p in [2, 3, 4]:
+        e0, e1, e2 = err(p, p), err(p, p + 1), err(p + 1, p + 1)
+        assert e2 < e1 < e0, (p, e0, e1, e2)
+
+
+@pytest.mark.parametrize("ext", [0, 1])
+def test_fmm_vjp_is_the_exact_transpose_of_the_fmm(ext):
     """orc_fmm_vjp against FP64 central differences of orc_fmm with the tree
     and lists held fixed, and against the direct-sum adjoint (to the FMM's
-    truncation accuracy)."""
+    truncation accuracy); M2L truncated at |k|+|n| <= p + ext."""
     rng = np.random.default_rng(13)
     n = 600
     x = np.concatenate([rng.random((500, 3)), 0.3 + 0.05 * rng.random((100, 3))])
@@ -409,10 +446,10 @@ def test_fmm_vjp_is_the_exact_transpose_of_the_fmm():
     xs, ms = x[o], m[o]
     w = rng.standard_normal((n, 3))
     p, eps = 4, 1e-2
-    gx, gm = po.fmm_vjp(t, L, p, xs, ms, w, box, eps=eps)
+    gx, gm = po.fmm_vjp(t, L, p, xs, ms, w, box, eps=eps, m2l_order=p + ext)
 
     def f(xx, mm):
-        a, _ = po.fmm(t, L, p, xx, mm, box, eps=eps)
+        a, _ = po.fmm(t, L, p, xx, mm, box, eps=eps, m2l_order=p + ext)
         return float((a * w).sum())
 
     h = 1e-6
