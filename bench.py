@@ -128,6 +128,166 @@ def time_steps(sim, warmup, steps):
     return agg, wall, inter, m2l_pairs, split_pairs, p2p_pairs, sim.stats()
 
 
+def force_error(sim, n, ntargets, seed):
+    """RMS (sqrt(sum|da|^2 / sum|a|^2)) and max relative error of one force
+    evaluation against the library's FP64 direct sum on random targets."""
+    acc, _ = sim.forces()
+    tg = np.sort(np.random.default_rng(seed).choice(n, size=min(ntargets, n), replace=False))
+    ad, _ = sim.direct_fp64(tg)
+    d = np.linalg.norm(acc[tg].astype(np.float64) - ad, axis=1)
+    ref = np.linalg.norm(ad, axis=1)
+    return {"rms": float(np.sqrt(np.mean(d ** 2) / np.mean(ref ** 2))), "max": float(np.max(d / ref)),
+            "targets": int(len(tg))}
+
+
+def workload_uniform4096(nb, args):
+    """Uniform cube N=4096 (seed 0), eps=1e-2, theta=0.5, cap 16, p=4, 10 KDK
+    steps at dt=1e-3: the tuned and the plain FP32 port, and the reference CPU
+    path (NaiveSimulation, its own force law) on one host core."""
+    n, K = 4096, 10
+    x, v, m = nb.ics.uniform(n, 0)
+    res = {}
+    for label, m2l, p2p in [("tuned", nb.M2L_GROUPED, nb.P2P_LANE), ("plain_port", nb.M2L_PLAIN, nb.P2P_LEAF)]:
+        cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=16, order=4, m2l=m2l, p2p=p2p)
+        sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
+        err = force_error(sim, n, n, 0)
+        agg, wall, *_ = time_steps(sim, 2, K)
+        sim.close()
+        res[label] = {"particle_steps_per_s": n * K / wall, "ms_per_step": wall / K * 1e3, "force_err": err}
+    out = {"metric": "particle-steps/s, N=4096 uniform, 10 KDK steps; force error vs FP64 direct",
+           "value": res["tuned"]["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
+           "warmup": 2, "ms_per_step": res["tuned"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": "uniform_N4096_seed0_eps1e-2_theta0.5_cap16_p4_kdk_dt1e-3"}, "results": res}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        P = np.zeros((n, 12), dtype=np.float32)
+        P[:, 0:3] = x
+        P[:, 8] = m
+        P[:, 9] = m
+        kind = "reference" if po.ref_lib() is not None else "port"
+        run = po.naive_reference if kind == "reference" else po.naive_compat
+        P1, _ = run(P, 1.0, 1e-3, 1)
+        a_ref = (P1[:, 4:7].astype(np.float64) - 0.0) / 1e-3
+        cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, leaf_capacity=16)
+        sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
+        ad, _ = sim.direct_fp64(np.arange(n))
+        sim.close()
+        # the reference path's force law differs (separation (dx,dx,dx), no
+        # softening): its error against the softened FP64 sum is O(1)
+        d = np.linalg.norm(a_ref - ad, axis=1)
+        t0 = time.perf_counter()
+        run(P, 1.0, 1e-3, K)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * K / dt, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+                               "sample": f"NaiveSimulation::step, N=4096 uniform, {K} steps in {dt:.2f} s",
+                               "force_err": {"rms": float(np.sqrt(np.mean(d ** 2) / np.mean(np.sum(ad ** 2, 1)))),
+                                             "max": float(np.max(d / np.linalg.norm(ad, axis=1)))}}
+    return out
+
+
+def workload_clustered(nb, args):
+    """Two Plummer spheres of 4,194,304 particles (seeds 2, 3) at (+-5, 0, 0)
+    approaching at 0.3 each; eps=5e-4, theta=0.4, p=6 (M2L |k|+|n| <= 6),
+    leaf capacity 32, depth <= 21; 1 force evaluation (error on 8192 targets)
+    plus 5 KDK steps at dt=5e-4."""
+    n_each = args.n // 2 if args.n != 1 << 20 else 4194304
+    x, v, m = nb.ics.clustered_pair(n_each)
+    n = len(m)
+    m = m / m.sum()
+    cfg = nb.default_config(G=1.0, eps=5e-4, theta=0.4, dt=5e-4, leaf_capacity=32, order=6, m2l_order=6,
+                            max_depth=21)
+    sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
+    err = force_error(sim, n, 8192, 8192)
+    K = 5
+    agg, wall, inter, m2l_pairs, split_pairs, p2p_pairs, st = time_steps(sim, 1, K)
+    sim.close()
+    return {"metric": "particle-steps/s, N=8,388,608 clustered pair, theta=0.4, p=6; force error vs FP64 direct",
+            "value": n * K / wall, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": 1,
+            "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": f"clustered_pair_2x{n_each}_seeds2_3_eps5e-4_theta0.4_cap32_p6_q6_kdk_dt5e-4"},
+            "force_err": {"fp32": err}, "phases_ms": {k[3:]: agg[k] / K for k in agg},
+            "tree": {"nodes": st["num_nodes"], "leaves": st["num_leaves"], "depth": st["depth"]},
+            "m2l_pairs_per_step": m2l_pairs / K, "p2p_interactions_per_step": inter / K}
+
+
+def workload_ensemble(nb, args):
+    """256 independent Plummer systems (seeds 100-355), N log-uniform in
+    [512, 65536], eps=1e-2, theta=0.5, p=4, one handle; the leaf capacity is
+    picked by timing 16, 32 and 64 (2 steps each); then 20 timed steps and the
+    per-system force error on 256 targets each."""
+    x, v, m, off = nb.ics.ensemble(256, 100, 512, 65536)
+    n = len(m)
+    tune = {}
+    for cap in (16, 32, 64):
+        cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=cap, order=4)
+        sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg, sys_off=off)
+        _, wall, *_ = time_steps(sim, 1, 2)
+        sim.close()
+        tune[cap] = wall / 2 * 1e3
+    cap = min(tune, key=tune.get)
+    cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=cap, order=4)
+    sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg, sys_off=off)
+    acc, _ = sim.forces()
+    rng = np.random.default_rng(256)
+    errs = []
+    tg_all = []
+    for s in range(256):
+        b, e = int(off[s]), int(off[s + 1])
+        tg_all.append(np.sort(rng.choice(np.arange(b, e), min(256, e - b), replace=False)))
+    tg = np.concatenate(tg_all)
+    ad, _ = sim.direct_fp64(tg)
+    pos = 0
+    for t in tg_all:
+        a, r = acc[t].astype(np.float64), ad[pos:pos + len(t)]
+        pos += len(t)
+        errs.append(float(np.sqrt(np.sum((a - r) ** 2) / np.sum(r ** 2))))
+    K = 20
+    agg, wall, *_ = time_steps(sim, 2, K)
+    sim.close()
+    return {"metric": "aggregate particle-steps/s, 256-system Plummer ensemble; per-system force error",
+            "value": n * K / wall, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": 2,
+            "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": f"ensemble_256_plummer_seeds100_355_N512_65536_eps1e-2_theta0.5_p4_cap{cap}",
+                       "n_total": n},
+            "autotune_ms_per_step": tune, "leaf_capacity": cap,
+            "force_err": {"per_system_rms_median": float(np.median(errs)), "per_system_rms_max": float(np.max(errs))},
+            "phases_ms": {k[3:]: agg[k] / K for k in agg}}
+
+
+def workload_adjoint(nb, args):
+    """Plummer N=16384 (seed 4), eps=1e-2, theta=0.5, p=4, 200 KDK steps at
+    dt=1e-3, checkpoints every 20: forward trajectory vs adjoint (reverse)
+    time for the loss mean |x_N - x*|^2, x* evolved from velocities perturbed
+    by sigma=1e-3 noise (seed 5)."""
+    from nbody_amd.adjoint import HipForceProvider, TrajectoryAdjoint
+    n, steps, dt = 16384, 200, 1e-3
+    x0, v0, m = nb.ics.plummer(n, 4)
+    cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=dt, leaf_capacity=16, order=4)
+    rng = np.random.default_rng(5)
+    prov = HipForceProvider(x0, v0, m, cfg)
+    adj = TrajectoryAdjoint(prov, dt, steps, ckpt_every=20)
+    xt, _ = adj.forward(x0, v0 + 1e-3 * rng.standard_normal((n, 3)))
+    t0 = time.perf_counter()
+    xN, vN = adj.forward(x0, v0)
+    t_fwd = time.perf_counter() - t0
+    loss = float(np.sum((xN - xt) ** 2) / (3 * n))
+    t0 = time.perf_counter()
+    gx, gv, gm = adj.backward(2.0 * (xN - xt) / (3 * n), np.zeros_like(vN))
+    t_bwd = time.perf_counter() - t0
+    prov.close()
+    return {"metric": "adjoint cost: reverse (gradient) time / forward trajectory time, N=16384, 200 steps",
+            "value": t_bwd / t_fwd, "unit": "ratio", "n_gpus": 1, "steps": steps, "warmup": 0,
+            "ms_per_step": t_fwd / steps * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "adjoint_plummer_N16384_seed4_eps1e-2_theta0.5_p4_cap16_200steps_ckpt20"},
+            "forward_s": t_fwd, "backward_s": t_bwd, "backward_parts_s": adj.seconds, "loss": loss,
+            "grad_norms": {"x0": float(np.linalg.norm(gx)), "v0": float(np.linalg.norm(gv)), "m": float(np.linalg.norm(gm))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +301,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
+    ap.add_argument("--workload", default="plummer1m",
+                    choices=["plummer1m", "uniform4096", "clustered", "ensemble", "adjoint"],
+                    help="plummer1m: the headline; the others print the line of another configuration")
     ap.add_argument("--no-plain-compare", action="store_true",
                     help="skip timing the plain FP32 port (per-target M2L threads, per-leaf P2P) beside the tuned path")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -159,6 +322,11 @@ def main():
         sys.exit(1)
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib << 20)
+    others = {"uniform4096": workload_uniform4096, "clustered": workload_clustered,
+              "ensemble": workload_ensemble, "adjoint": workload_adjoint}
+    if args.workload in others:
+        print(json.dumps(others[args.workload](nb, args)), flush=True)
+        return
     m2l = {"plain": nb.M2L_PLAIN, "grouped": nb.M2L_GROUPED, "bf16x3": nb.M2L_BF16X3}[args.m2l]
     p2p = {"leaf": nb.P2P_LEAF, "lane": nb.P2P_LANE}[args.p2p]
     n = args.n

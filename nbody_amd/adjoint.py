@@ -17,6 +17,7 @@ recomputes each segment from its checkpoint before reversing it.
 """
 from __future__ import annotations
 
+import time
 from typing import Callable, Optional
 
 import numpy as np
@@ -32,6 +33,7 @@ class HipForceProvider:
         cfg = config if config is not None else default_config()
         self.sim = HipSimulation.from_arrays(x0, v0, m, m, config=cfg)
         self.dt = float(cfg.dt)
+        self._zeros = np.zeros((len(m), 3))
 
     def run(self, x, v, nsteps, record=False):
         self.sim.set_state(x, v)
@@ -39,12 +41,12 @@ class HipForceProvider:
         for _ in range(nsteps):
             self.sim.step(1)
             if record:
-                xs.append(self.sim.state()[0])
+                xs.append(self.sim.positions())
         xf, vf, _, _, _ = self.sim.state()
         return xf, vf, xs
 
     def vjp(self, x, w):
-        self.sim.set_state(x, np.zeros_like(x))
+        self.sim.set_state(x, self._zeros if self._zeros.shape == np.shape(x) else np.zeros_like(x))
         return self.sim.force_vjp(w)
 
     def close(self):
@@ -61,6 +63,7 @@ class TrajectoryAdjoint:
         self.k = max(1, int(ckpt_every))
         self.ckpts = None
         self.vjp_calls = 0
+        self.seconds = {"recompute": 0.0, "vjp": 0.0}
 
     def forward(self, x0, v0):
         x = np.array(x0, dtype=np.float64)
@@ -88,12 +91,16 @@ class TrajectoryAdjoint:
         mbar = gm.copy()
         for start, xc, vc in reversed(self.ckpts):
             seg = min(self.k, self.nsteps - start)
+            t0 = time.perf_counter()
             _, _, xs = self.p.run(xc, vc, seg, record=True)  # xs[0..seg]
+            self.seconds["recompute"] += time.perf_counter() - t0
             for j in range(seg - 1, -1, -1):
                 vh = vbar + dt * xbar
                 n = start + j
                 c = (0.5 if n == 0 else 1.0) * dt
+                t0 = time.perf_counter()
                 gx, gm = self.p.vjp(xs[j], c * vh)
+                self.seconds["vjp"] += time.perf_counter() - t0
                 self.vjp_calls += 1
                 xbar = xbar + gx
                 mbar += gm

@@ -45,6 +45,13 @@ enum : uint32_t { OVF_NODES = 1, OVF_SL = 2, OVF_ROWS = 4, OVF_M2L = 8, OVF_P2P 
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// LDS writes of one lane made visible to the other lanes of its wavefront
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int l = __lane_id();
     return l == 0 ? 0ull : (~0ull >> (64 - l));

@@ -271,11 +271,6 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
 // wave stays full for any leaf size and every lane runs the same trip count.
 constexpr int P2P_WAVES = 4;
 
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <int P>
 __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves,
@@ -448,23 +443,29 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     for (int k = NCQ; k < F.dcp; ++k) row[k] = 0.f;
 }
 
-constexpr int M2L_LDS_THREADS = 1024;
+// 1024 threads (16 waves at <= 128 VGPRs, one workgroup per CU beside a
+// 112 KiB table) while expansion and tensor fit that register budget (up to
+// p = 4, q = 5); 256 above, with the table read from global memory
+constexpr int m2l_table_threads(int p, int q) { return ncoef(p) + ncoef(q) <= 91 ? 1024 : 256; }
 
 // Same-level part of the rows of the level's targets [off, off+count): one
-// wavefront per target, lanes over its entries; writes L.
-template <int P, int Q>
-__global__ __launch_bounds__(M2L_LDS_THREADS) void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count,
+// wavefront per target, lanes over its entries; writes L.  IN_LDS: the
+// level's table is first copied to LDS (when it fits one workgroup's LDS),
+// else it is read from global memory.
+template <int P, int Q, bool IN_LDS>
+__global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count,
                                                                   int lvl) {
     constexpr int NC = ncoef(P), NCQ = ncoef(Q);
     constexpr int NQD = (NCQ + 3) / 4;  // float4 per tensor row (the host sets dcp = 4 NQD)
     extern __shared__ float4 dtab_lds[];
     const int K = F.dK, K1 = K + 1;
-    {
-        const float4* src = reinterpret_cast<const float4*>(F.dtab + (size_t)lvl * F.dsz * F.dcp);
+    const float4* src = reinterpret_cast<const float4*>(F.dtab + (size_t)lvl * F.dsz * F.dcp);
+    if (IN_LDS) {
         const int tot = F.dsz * NQD;
         for (int i = threadIdx.x; i < tot; i += blockDim.x) dtab_lds[i] = src[i];
         __syncthreads();
     }
+    const float4* tab = IN_LDS ? dtab_lds : src;
     const int lane = lane_id();
     const uint32_t wpb = blockDim.x / WAVE;
     for (uint32_t t = blockIdx.x * wpb + threadIdx.x / WAVE; t < count; t += gridDim.x * wpb) {
@@ -481,7 +482,7 @@ __global__ __launch_bounds__(M2L_LDS_THREADS) void m2l_lds_kernel(FmmParams F, u
                 const int4 gb = F.nodes.geo[B];
                 const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
                 const uint32_t neg = (uint32_t)(ox < 0) | ((uint32_t)(oy < 0) << 1) | ((uint32_t)(oz < 0) << 2);
-                const float4* row = dtab_lds + ((abs(ox) * K1 + abs(oy)) * K1 + abs(oz)) * NQD;
+                const float4* row = tab + ((abs(ox) * K1 + abs(oy)) * K1 + abs(oz)) * NQD;
                 float D[NCQ];
 #pragma unroll
                 for (int q = 0; q < NQD; ++q) {
@@ -717,14 +718,21 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, hipStream_t s
         hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)ceil_div(nz, (size_t)256)), dim3(256), 0, st, F.L, nz);
     }
     const size_t lds = (size_t)F.dsz * F.dcp * 4;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const bool in_lds = lds <= plan.max_lds && m2l_table_threads(P, Q) == 1024;
+    if (in_lds)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     for (int l = F.dlevel0; l < F.dlevel0 + F.dnlev; ++l) {
         const uint32_t cnt = plan.level_count[l];
         if (!cnt) continue;
-        const unsigned g = std::min<unsigned>(ceil_div(cnt, M2L_LDS_THREADS / WAVE), 2048u);
-        hipLaunchKernelGGL((m2l_lds_kernel<P, Q>), dim3(g), dim3(M2L_LDS_THREADS), lds, st, F, plan.level_off[l], cnt,
-                           l - F.dlevel0);
+        constexpr int T = m2l_table_threads(P, Q);
+        const unsigned g = std::min<unsigned>(ceil_div(cnt, T / WAVE), 2048u);
+        if (in_lds)
+            hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true>), dim3(g), dim3(T), lds, st, F, plan.level_off[l], cnt,
+                               l - F.dlevel0);
+        else
+            hipLaunchKernelGGL((m2l_lds_kernel<P, Q, false>), dim3(g), dim3(T), 0, st, F, plan.level_off[l], cnt,
+                               l - F.dlevel0);
     }
     hipLaunchKernelGGL((m2l_cross_kernel<P, Q>), dim3(grid), dim3(256), 0, st, F, nn);
     hipLaunchKernelGGL((m2l_far_kernel<P, Q>), dim3(1024), dim3(256), 0, st, F);

@@ -140,6 +140,7 @@ struct M2LPlan {
     uint32_t num_nodes;
     uint32_t level_off[MAX_LEVELS + 2];
     uint32_t level_count[MAX_LEVELS + 1];
+    size_t max_lds;  // LDS bytes per workgroup: the same-level table is held in LDS if it fits
 };
 // tensors at R = -(a + b e) h_l / 2^shift for e in [0, E)^3, levels
 // level0 .. level0+nlev-1, into out (rows of F.dcp floats)

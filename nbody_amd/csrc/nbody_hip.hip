@@ -422,13 +422,16 @@ static M2LPlan m2l_plan(const nbh_handle* h) {
     M2LPlan pl{};
     pl.mode = h->cfg.m2l == NBH_M2L_PLAIN ? 0 : 1;
     pl.num_nodes = h->num_nodes;
+    pl.max_lds = (size_t)h->max_lds;
     std::copy(h->hc->level_off, h->hc->level_off + MAX_LEVELS + 2, pl.level_off);
     std::copy(h->hc->level_count, h->hc->level_count + MAX_LEVELS + 1, pl.level_count);
     return pl;
 }
 
 // One force evaluation; kick_dt != 0 applies v += a * kick_dt afterwards.
-static int force_fmm(nbh_handle* h, double kick_dt) {
+// for_vjp: only the tree, lists and mass expansions are needed (the adjoint
+// kernels follow on the same stream): no near field, no closing read-back.
+static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     const int p = (int)h->cfg.order;
     record(h, 0);
     CHK(build_tree(h));
@@ -446,13 +449,12 @@ static int force_fmm(nbh_handle* h, double kick_dt) {
     if (h->cfg.m2l != NBH_M2L_PLAIN && h->nsys == 1 && h->depth >= 2) {
         // derivative tensors of the same-level offsets [0, K]^3 of levels
         // 2..depth (cells of levels 0 and 1 are never well separated); one
-        // system only, since the tensors depend on the root size; used when
-        // a level's table fits in one workgroup's LDS
+        // system only, since the tensors depend on the root size
         const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
         const int K = 2 * Kp + 1;
         const int dsz = (K + 1) * (K + 1) * (K + 1), dcp = ncp_of(q);
         const int nlev = (int)h->depth - 1;
-        if ((size_t)dsz * dcp * 4 <= (size_t)h->max_lds) {
+        {
             HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4));
             F.dtab = h->dtab.as<float>();
             F.dK = K;
@@ -488,10 +490,14 @@ static int force_fmm(nbh_handle* h, double kick_dt) {
     for (uint32_t l = 1; l <= h->depth; ++l)
         launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
     record(h, 5);
+    h->lastF = F;
+    if (for_vjp) {
+        HIPCHK(h, hipGetLastError());
+        return NBH_OK;
+    }
     if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_staged(p, F, S, h->num_leaves, kick_dt, h->stream);
     else launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
     record(h, 6);
-    h->lastF = F;
     HIPCHK(h, hipGetLastError());
     // stats (read at the end of the evaluation; one more sync)
     CHK(read_counters(h));
@@ -766,19 +772,21 @@ int nbh_get_state(nbh_handle* h, double* pos, double* vel, double* mass, double*
     const size_t n = h->n;
     State S = h->st[h->cur].view();
     std::vector<uint32_t> ids(n);
-    std::vector<double> buf(n);
+    std::vector<double> buf(8 * n);
     HIPCHK(h, hipMemcpyAsync(ids.data(), S.id, n * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
     const double* src[8] = {S.x, S.y, S.z, S.vx, S.vy, S.vz, S.mass, S.charge};
+    double* dsts[8] = {pos, pos, pos, vel, vel, vel, mass, charge};
+    for (int k = 0; k < 8; ++k)
+        if (dsts[k]) HIPCHK(h, hipMemcpyAsync(&buf[k * n], src[k], n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     for (int k = 0; k < 8; ++k) {
-        double* dst = k < 3 ? pos : (k < 6 ? vel : (k == 6 ? mass : charge));
+        double* dst = dsts[k];
         if (!dst) continue;
-        HIPCHK(h, hipMemcpyAsync(buf.data(), src[k], n * 8, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
+        const double* b = &buf[k * n];
         if (k < 6)
-            for (size_t i = 0; i < n; ++i) dst[3 * (size_t)ids[i] + (k % 3)] = buf[i];
+            for (size_t i = 0; i < n; ++i) dst[3 * (size_t)ids[i] + (k % 3)] = b[i];
         else
-            for (size_t i = 0; i < n; ++i) dst[ids[i]] = buf[i];
+            for (size_t i = 0; i < n; ++i) dst[ids[i]] = b[i];
     }
     if (t) *t = h->time;
     return NBH_OK;
@@ -800,11 +808,12 @@ int nbh_set_state(nbh_handle* h, const double* pos, const double* vel, double t)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     for (size_t i = 0; i < n; ++i) { mi[ids[i]] = m[i]; qi[ids[i]] = q[i]; }
     double* dst[6] = {S.x, S.y, S.z, S.vx, S.vy, S.vz};
+    buf.resize(6 * n);
     for (int k = 0; k < 6; ++k) {
         const double* srcp = k < 3 ? pos : vel;
-        for (size_t i = 0; i < n; ++i) buf[i] = srcp[3 * i + (k % 3)];
-        HIPCHK(h, hipMemcpyAsync(dst[k], buf.data(), n * 8, hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
+        double* b = &buf[k * n];
+        for (size_t i = 0; i < n; ++i) b[i] = srcp[3 * i + (k % 3)];
+        HIPCHK(h, hipMemcpyAsync(dst[k], b, n * 8, hipMemcpyHostToDevice, h->stream));
     }
     HIPCHK(h, hipMemcpyAsync(S.mass, mi.data(), n * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(S.charge, qi.data(), n * 8, hipMemcpyHostToDevice, h->stream));
@@ -910,8 +919,8 @@ int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
     HIPCHK(h, hipSetDevice(h->device));
     const size_t n = h->n;
     // tree, lists and mass expansions at the current positions
-    CHK(force_fmm(h, 0.0));
-    h->acc_valid = true;
+    CHK(force_fmm(h, 0.0, true));
+    h->acc_valid = false;
     const int p = (int)h->cfg.order;
     const size_t ncp = (size_t)ncp_of(p);
     HIPCHK(h, h->Mw.ensure((size_t)h->num_nodes * ncp * 4));

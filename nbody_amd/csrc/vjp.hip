@@ -57,74 +57,147 @@ __global__ __launch_bounds__(256) void dipole_p2m_kernel(FmmParams F, const floa
     }
 }
 
-// near field (pair VJP over the P2P list) + far field (Hessian of the mass
-// local expansion, gradient and value of the dipole local expansion); one
-// lane per particle.
+// near field (pair VJP over the P2P lists) + far field (Hessian of the mass
+// local expansion, gradient and value of the dipole local expansion).  One
+// wavefront per target leaf, organised like the forward P2P kernel: the
+// leaf's P2P sources are flattened over its row segments, 64 at a time
+// staged in LDS (offset in the target leaf's frame, mass, cotangent), lanes
+// = (target particle) x (source stride), partial sums combined by shuffles.
+constexpr int VJP_WAVES = 4;
+
 template <int P>
-__global__ __launch_bounds__(256) void vjp_final_kernel(FmmParams F, const float4* pw, const float* Lw, uint32_t n,
-                                                        double* gx, double* gy, double* gz, double* gm) {
+__global__ __launch_bounds__(WAVE * VJP_WAVES) void vjp_staged_kernel(FmmParams F, const float4* pw, const float* Lw,
+                                                                      uint32_t num_leaves, double* gx, double* gy,
+                                                                      double* gz, double* gm) {
     constexpr int NC = ncoef(P);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int A = F.pleaf[i];
-    const int4 ga = F.nodes.geo[A];
-    const float unit = F.unit[F.nodes.sys[A]];
+    __shared__ float4 stage_p[VJP_WAVES][WAVE];
+    __shared__ float4 stage_w[VJP_WAVES][WAVE];
+    __shared__ int stage_i[VJP_WAVES][WAVE];
+    const int wslot = threadIdx.x / WAVE;
+    const int lane = lane_id();
+    float4* sp4 = stage_p[wslot];
+    float4* sw4 = stage_w[wslot];
+    int* sidx = stage_i[wslot];
+    const uint32_t nw = gridDim.x * VJP_WAVES;
     const float eps2 = F.eps2;
-    const float4 me = F.pf[i];
-    const float4 wi = pw[i];
-    const float mi = me.w;
-    float g0 = 0.f, g1 = 0.f, g2 = 0.f, gmm = 0.f;
-    for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r])
-    for (uint32_t e = F.lists.p_start[r], e1 = e + F.lists.p_cnt[r]; e < e1; ++e) {
-        const int B = F.lists.p_pool[e];
-        const int4 gb = F.nodes.geo[B];
-        const int4 lb = F.nodes.link[B];
-        const float3 d = centre_diff(ga, gb, F.bits, unit);
-        const float dx = d.x - me.x, dy = d.y - me.y, dz = d.z - me.z;
-        for (int j = 0; j < lb.y; ++j) {
-            const int src = lb.x + j;
-            const float4 s = F.pf[src];
-            const float4 wj = pw[src];
-            const float rx = dx + s.x, ry = dy + s.y, rz = dz + s.z;
-            const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
-            const float inv = src == (int)i ? 0.f : rsqrtf(r2);
-            const float i2 = inv * inv;
-            const float i3 = i2 * inv;
-            const float i5 = i3 * i2;
-            const float ux = mi * wj.x - s.w * wi.x, uy = mi * wj.y - s.w * wi.y, uz = mi * wj.z - s.w * wi.z;
-            const float ru = rx * ux + ry * uy + rz * uz;
-            const float c = 3.f * ru * i5;
-            g0 += ux * i3 - rx * c;
-            g1 += uy * i3 - ry * c;
-            g2 += uz * i3 - rz * c;
-            gmm -= (wj.x * rx + wj.y * ry + wj.z * rz) * i3;
+    for (uint32_t w = blockIdx.x * VJP_WAVES + wslot; w < num_leaves; w += nw) {
+        const int A = (int)F.leaf_ids[w];
+        const int4 ga = F.nodes.geo[A];
+        const int4 la = F.nodes.link[A];
+        const float unit = F.unit[F.nodes.sys[A]];
+        for (int t0 = 0; t0 < la.y; t0 += WAVE) {
+            const int nt = min(WAVE, la.y - t0);
+            const int splits = WAVE / nt;
+            const int ti = lane % nt;
+            const int sp = lane / nt;
+            const int i = la.x + t0 + ti;
+            const float4 me = F.pf[i];
+            const float4 wi = pw[i];
+            const float mi = me.w;
+            float g0 = 0.f, g1 = 0.f, g2 = 0.f, gmm = 0.f;
+            for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r])
+            for (uint32_t eb = F.lists.p_start[r], e_end = eb + F.lists.p_cnt[r]; eb < e_end; eb += WAVE) {
+                const int ne = (int)min((uint32_t)WAVE, e_end - eb);
+                int cnt = 0, beg = 0;
+                float dx = 0.f, dy = 0.f, dz = 0.f;
+                if (lane < ne) {
+                    const int B = F.lists.p_pool[eb + lane];
+                    const int4 lb = F.nodes.link[B];
+                    const float3 d = centre_diff(ga, F.nodes.geo[B], F.bits, unit);
+                    cnt = lb.y;
+                    beg = lb.x;
+                    dx = d.x; dy = d.y; dz = d.z;
+                }
+                int incl = cnt;
+#pragma unroll
+                for (int o = 1; o < WAVE; o <<= 1) {
+                    const int y = __shfl_up(incl, o, WAVE);
+                    if (lane >= o) incl += y;
+                }
+                const int total = __shfl(incl, WAVE - 1, WAVE);
+                const int excl = incl - cnt;
+                for (int f0 = 0; f0 < total; f0 += WAVE) {
+                    const int fidx = f0 + lane;
+                    int lo = 0, hi = ne - 1;
+#pragma unroll
+                    for (int step = 0; step < 6; ++step) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        const int em = __shfl(excl, mid, WAVE);
+                        if (em <= fidx) lo = mid; else hi = mid - 1;
+                    }
+                    const int kb = __shfl(beg, lo, WAVE), ke = __shfl(excl, lo, WAVE);
+                    const float kx = __shfl(dx, lo, WAVE), ky = __shfl(dy, lo, WAVE), kz = __shfl(dz, lo, WAVE);
+                    if (fidx < total) {
+                        const int src = kb + (fidx - ke);
+                        const float4 s = F.pf[src];
+                        sp4[lane] = make_float4(kx + s.x, ky + s.y, kz + s.z, s.w);
+                        sw4[lane] = pw[src];
+                        sidx[lane] = src;
+                    }
+                    wave_lds_sync();
+                    const int nchunk = min(WAVE, total - f0);
+                    if (sp < splits) {
+                        for (int j = sp; j < nchunk; j += splits) {
+                            const float4 s = sp4[j];
+                            const float4 wj = sw4[j];
+                            const float rx = s.x - me.x, ry = s.y - me.y, rz = s.z - me.z;
+                            const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
+                            const float inv = sidx[j] == i ? 0.f : rsqrtf(r2);
+                            const float i2 = inv * inv;
+                            const float i3 = i2 * inv;
+                            const float i5 = i3 * i2;
+                            const float ux = mi * wj.x - s.w * wi.x, uy = mi * wj.y - s.w * wi.y,
+                                        uz = mi * wj.z - s.w * wi.z;
+                            const float ru = rx * ux + ry * uy + rz * uz;
+                            const float c = 3.f * ru * i5;
+                            g0 += ux * i3 - rx * c;
+                            g1 += uy * i3 - ry * c;
+                            g2 += uz * i3 - rz * c;
+                            gmm -= (wj.x * rx + wj.y * ry + wj.z * rz) * i3;
+                        }
+                    }
+                    wave_lds_sync();
+                }
+            }
+            {
+                const float a0 = g0, a1 = g1, a2 = g2, a3 = gmm;
+                for (int q = 1; q < splits; ++q) {
+                    const int srcl = ti + q * nt;
+                    g0 += __shfl(a0, srcl, WAVE);
+                    g1 += __shfl(a1, srcl, WAVE);
+                    g2 += __shfl(a2, srcl, WAVE);
+                    gmm += __shfl(a3, srcl, WAVE);
+                }
+            }
+            if (sp == 0) {
+                // far field
+                float L[NC];
+                float H[6];
+                {
+                    const float* Ls = F.L + (size_t)A * F.ncp;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) L[k] = Ls[k];
+                    l2p_hessian<P>(me.x, me.y, me.z, L, H);
+                }
+                float psi, px, py, pz;
+                {
+                    const float* Ls = Lw + (size_t)A * F.ncp;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) L[k] = Ls[k];
+                    l2p_eval<P>(me.x, me.y, me.z, L, psi, px, py, pz);
+                }
+                g0 += mi * px + H[0] * wi.x + H[3] * wi.y + H[4] * wi.z;
+                g1 += mi * py + H[3] * wi.x + H[1] * wi.y + H[5] * wi.z;
+                g2 += mi * pz + H[4] * wi.x + H[5] * wi.y + H[2] * wi.z;
+                gmm += psi;
+                const double G = (double)F.G;
+                gx[i] = G * g0;
+                gy[i] = G * g1;
+                gz[i] = G * g2;
+                gm[i] = G * gmm;
+            }
         }
     }
-    // far field
-    float L[NC];
-    float H[6];
-    {
-        const float* Ls = F.L + (size_t)A * F.ncp;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) L[k] = Ls[k];
-        l2p_hessian<P>(me.x, me.y, me.z, L, H);
-    }
-    float psi, px, py, pz;
-    {
-        const float* Ls = Lw + (size_t)A * F.ncp;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) L[k] = Ls[k];
-        l2p_eval<P>(me.x, me.y, me.z, L, psi, px, py, pz);
-    }
-    g0 += mi * px + H[0] * wi.x + H[3] * wi.y + H[4] * wi.z;
-    g1 += mi * py + H[3] * wi.x + H[1] * wi.y + H[5] * wi.z;
-    g2 += mi * pz + H[4] * wi.x + H[5] * wi.y + H[2] * wi.z;
-    gmm += psi;
-    const double G = (double)F.G;
-    gx[i] = G * g0;
-    gy[i] = G * g1;
-    gz[i] = G * g2;
-    gm[i] = G * gmm;
 }
 
 template <int P>
@@ -138,8 +211,9 @@ static void launch_vjp_t(int q, const M2LPlan& plan, const FmmParams& F, const F
     for (int l = depth - 1; l >= 0; --l) launch_m2m_level(P, Fw, l, level_count[l], level_off[l], st);
     launch_m2l(P, q, Fw, plan, st);
     for (int l = 1; l <= depth; ++l) launch_l2l_level(P, Fw, l, level_count[l], level_off[l], st);
-    hipLaunchKernelGGL((vjp_final_kernel<P>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, pw, Fw.L, n, g, g + n,
-                       g + 2 * (size_t)n, g + 3 * (size_t)n);
+    const unsigned grid = std::min<unsigned>(ceil_div(num_leaves, VJP_WAVES), 65536u);
+    hipLaunchKernelGGL((vjp_staged_kernel<P>), dim3(grid), dim3(WAVE * VJP_WAVES), 0, st, F, pw, Fw.L, num_leaves, g,
+                       g + n, g + 2 * (size_t)n, g + 3 * (size_t)n);
 }
 
 void launch_force_vjp(int p, int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const double* w_in,

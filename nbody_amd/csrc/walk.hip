@@ -48,12 +48,6 @@ __device__ __forceinline__ uint32_t depth_base(const Counters* C, int s) {
     return b;
 }
 
-// LDS writes of one lane made visible to the other lanes of its wavefront
-__device__ __forceinline__ void walk_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Per-wave LDS cache of one split list (SL_CACHE entries at a time): the
 // entry's node id, geometry and first child.  Loading it once, coalesced,
@@ -74,7 +68,7 @@ __device__ __forceinline__ void sl_cache_fill(const WalkParams& W, const SLCache
         c.geo[k] = W.nodes.geo[B];
         c.fc[k] = W.nodes.link[B].z;
     }
-    walk_lds_sync();
+    wave_lds_sync();
 }
 
 // Enumerate expand(SL) for the split list at sl[st, st+cnt); f(valid, B, geo)
@@ -103,7 +97,7 @@ __device__ __forceinline__ void enumerate_sl(const WalkParams& W, const SLCache&
             if (__ballot(valid) == 0) continue;
             f(valid, B, gb);
         }
-        if (!one_block) walk_lds_sync();
+        if (!one_block) wave_lds_sync();
     }
     // phase b: children of the sources that have children, 8 entries x 8
     // slots per chunk, two chunks in flight
@@ -123,7 +117,7 @@ __device__ __forceinline__ void enumerate_sl(const WalkParams& W, const SLCache&
             if (__ballot(v0)) f(v0, C0, g0);
             if (__ballot(v1)) f(v1, C1, g1);
         }
-        if (!one_block) walk_lds_sync();
+        if (!one_block) wave_lds_sync();
     }
 }
 
@@ -390,7 +384,7 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
                 });
             });
         }
-        walk_lds_sync();  // the cache is refilled by the next group
+        wave_lds_sync();  // the cache is refilled by the next group
     }
     if (lane == 0) {
         if (acc_m) atomicAdd(&C->m2l_total, acc_m);

@@ -154,6 +154,12 @@ class HipSimulation(Simulation):
                                             ctypes.byref(t)), "nbh_get_state")
         return x, v, m, q, t.value
 
+    def positions(self):
+        """Positions only (FP64, input order): one device read instead of four."""
+        x = np.zeros((self.n, 3))
+        _check(self._h, lib().nbh_get_state(self._h, ptr(x, _dp), None, None, None, None), "nbh_get_state")
+        return x
+
     def set_state(self, pos, vel, t=0.0):
         pos = _c64(pos, (self.n, 3))
         vel = _c64(vel, (self.n, 3))
