@@ -1,4 +1,5 @@
-// nbody_cli.cpp -- command-line driver over nbody::HipSimulation.
+// nbody_cli.cpp -- command-line driver over nbody::HipSimulation (or the
+// multi-core nbody::CpuSimulation, --backend cpu).
 //
 // Plays the role of the reference's src/main.cpp: builds the initial
 // particles, steps the simulation to a final time and appends one CSV row
@@ -7,24 +8,30 @@
 // std::rand().  Usage:
 //   nbody_cli [--ic uniform|plummer] [--n N] [--seed S] [--dt DT] [--t-end T]
 //             [--csv FILE] [--theta TH] [--order P] [--leaf-cap C] [--eps E]
+//             [--backend hip|cpu]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <vector>
 
+#include "nbody/cpu_simulation.h"
 #include "nbody/hip_simulation.h"
 
-using Simulation = nbody::HipSimulation;
+// the backend is picked at run time; both implement the same interface
+using Simulation = nbody::Simulation<float, nbody::hip_vector_t>;
+using HipSim = nbody::HipSimulation;
+using CpuSim = nbody::CpuSimulation<float, nbody::hip_vector_t>;
 
 int main(int argc, char** argv) {
-	std::string ic = "uniform", csv;
+	std::string ic = "uniform", csv, backend = "hip";
 	std::size_t n = 1000;
 	unsigned long long seed = 0;
 	double dt = 1e-3, t_end = 0.01;
-	nbh_config cfg = Simulation::defaultConfig();
+	nbh_config cfg = HipSim::defaultConfig();
 	for (int i = 1; i + 1 < argc; i += 2) {
 		std::string a = argv[i];
 		const char* v = argv[i + 1];
@@ -38,6 +45,7 @@ int main(int argc, char** argv) {
 		else if (a == "--order") cfg.order = (unsigned)std::atoi(v);
 		else if (a == "--leaf-cap") cfg.leaf_capacity = (unsigned)std::atoi(v);
 		else if (a == "--eps") cfg.eps = std::atof(v);
+		else if (a == "--backend") backend = v;
 		else {
 			std::cerr << "unknown option " << a << "\n";
 			return 2;
@@ -56,15 +64,26 @@ int main(int argc, char** argv) {
 			particles.emplace_back(x, v, (float)mass[i], (float)mass[i]);
 		}
 		Simulation::Vector bounds{1.f, 1.f, 1.f, 0.f};
-		Simulation sim(bounds, particles, (float)dt, std::cout, cfg);
+		std::unique_ptr<Simulation> sim;
+		if (backend == "cpu") {
+			nbody::CpuOptions opt;
+			opt.G = cfg.G;
+			opt.eps = cfg.eps;
+			sim.reset(new CpuSim(bounds, particles, (float)dt, std::cout, opt));
+		} else if (backend == "hip") {
+			sim.reset(new HipSim(bounds, particles, (float)dt, std::cout, cfg));
+		} else {
+			std::cerr << "unknown backend " << backend << "\n";
+			return 2;
+		}
 		std::ofstream out;
 		if (!csv.empty()) out.open(csv);
 		Simulation::Scalar t = 0;
 		const long steps = (long)(t_end / dt + 0.5);
 		for (long s = 0; s < steps; ++s) {
-			t = sim.step();
+			t = sim->step();
 			if (out) {
-				std::vector<Simulation::Particle> ps = sim.particles();
+				std::vector<Simulation::Particle> ps = sim->particles();
 				out << t;
 				for (auto const& p : ps) out << "," << p.position[0] << "," << p.position[1] << "," << p.position[2];
 				out << "\n";
