@@ -297,6 +297,7 @@ def main():
     ap.add_argument("--m2l", default="grouped", choices=["plain", "grouped", "bf16x3"])
     ap.add_argument("--p2p", default="lane", choices=["leaf", "lane"])
     ap.add_argument("--order", type=int, default=4)
+    ap.add_argument("--m2l-order", type=int, default=0, help="M2L truncation |k|+|n| <= q (0: order + 1)")
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -332,7 +333,8 @@ def main():
     n = args.n
     p = args.order
     x, v, m = nb.ics.plummer(n, seed=1, a=1.0, M=1.0, G=1.0, rmax=20.0)
-    cfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, dt=1e-3, leaf_capacity=32, order=p, m2l=m2l, p2p=p2p)
+    cfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, dt=1e-3, leaf_capacity=32, order=p, m2l=m2l, p2p=p2p,
+                            m2l_order=args.m2l_order)
     sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
 
     # one force evaluation, checked against the FP64 direct sum
@@ -412,7 +414,7 @@ def main():
         # the plain all-FP32 port: one thread per target cell for M2L (tensor
         # built per pair), one wavefront per leaf for P2P, same operator
         pcfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, dt=1e-3, leaf_capacity=32, order=p,
-                                 m2l=nb.M2L_PLAIN, p2p=nb.P2P_LEAF)
+                                 m2l=nb.M2L_PLAIN, p2p=nb.P2P_LEAF, m2l_order=args.m2l_order)
         psim = nb.HipSimulation.from_arrays(x, v, m, m, config=pcfg)
         _, pwall, _, _, _, _, _ = time_steps(psim, 1, 3)
         psim.close()

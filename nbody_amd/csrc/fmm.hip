@@ -37,14 +37,16 @@ void launch_gather_state(const State& src, const State& dst, const uint32_t* per
 }
 
 // ---------------------------------------------------------------------------
-// P2M (moment.cl:6-67 at order P): one wavefront per leaf cell.  Also writes
-// the particles' FP32 offsets from the leaf centre (used by P2P and L2P).
+// P2M (moment.cl:6-67 at order P): one lane per leaf cell, streaming its
+// particles (leaves hold ~10 particles on average, so a wavefront per leaf
+// would leave most lanes idle).  Also writes the particles' FP32 offsets from
+// the leaf centre (used by P2P and L2P) and their leaf.
 template <int P>
 __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t num_leaves) {
     constexpr int NC = ncoef(P);
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    if (wid >= num_leaves) return;
-    const int A = (int)F.leaf_ids[wid];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_leaves) return;
+    const int A = (int)F.leaf_ids[t];
     const int4 g = F.nodes.geo[A];
     const int4 lk = F.nodes.link[A];
     const int s = F.nodes.sys[A];
@@ -53,8 +55,7 @@ __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t
     float M[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) M[k] = 0.f;
-    for (int j = lane_id(); j < lk.y; j += WAVE) {
-        const int i = lk.x + j;
+    for (int i = lk.x; i < lk.x + lk.y; ++i) {
         const float bx = (float)(S.x[i] - c[0]), by = (float)(S.y[i] - c[1]), bz = (float)(S.z[i] - c[2]);
         const float q = (float)S.charge[i];
         F.pf[i] = make_float4(bx, by, bz, q);
@@ -62,15 +63,32 @@ __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t
         float mono[NC];
         monomials<P>(bx, by, bz, mono);
 #pragma unroll
-        for (int k = 0; k < NC; ++k) M[k] += q * mono[k];
+        for (int k = 0; k < NC; ++k) M[k] = fmaf(q, mono[k], M[k]);
     }
+    float* out = F.M + (size_t)A * F.ncp;
 #pragma unroll
-    for (int k = 0; k < NC; ++k) M[k] = wave_sum(M[k]);
-    if (lane_id() == 0) {
-        float* out = F.M + (size_t)A * F.ncp;
+    for (int k = 0; k < NC; ++k) out[k] = M[k];
+}
+
+// L2P of the far field, one lane per particle: phi and grad phi of its
+// leaf's local expansion, written to (ax, ay, az, pot) for the P2P kernel to
+// complete.
+template <int P>
+__global__ __launch_bounds__(256) void l2p_kernel(FmmParams F, State S, uint32_t n) {
+    constexpr int NC = ncoef(P);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 me = F.pf[i];
+    const float* Ls = F.L + (size_t)F.pleaf[i] * F.ncp;
+    float L[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) out[k] = M[k];
-    }
+    for (int k = 0; k < NC; ++k) L[k] = Ls[k];
+    float phi, gx, gy, gz;
+    l2p_eval<P>(me.x, me.y, me.z, L, phi, gx, gy, gz);
+    S.ax[i] = gx;
+    S.ay[i] = gy;
+    S.az[i] = gz;
+    S.pot[i] = phi;
 }
 
 // M2M (replaces moment.cl:72-160, which sums untranslated moments): one
@@ -237,12 +255,8 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
             }
         }
         if (sp == 0) {
-            float lphi, gx, gy, gz;
-            const float* Lsrc = F.L + (size_t)A * F.ncp;
-            float Lr[NC];
-#pragma unroll
-            for (int k = 0; k < NC; ++k) Lr[k] = Lsrc[k];
-            l2p_eval<P>(me.x, me.y, me.z, Lr, lphi, gx, gy, gz);
+            // far field from l2p_kernel
+            const float lphi = S.pot[i], gx = S.ax[i], gy = S.ay[i], gz = S.az[i];
             const double qm = S.charge[i] / S.mass[i];
             const float fac = (float)qm * F.G;
             const float axo = fac * (ax + gx), ayo = fac * (ay + gy), azo = fac * (az + gz);
@@ -389,12 +403,8 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
                 }
             }
             if (sp == 0) {
-                float lphi, gx, gy, gz;
-                const float* Lsrc = F.L + (size_t)A * F.ncp;
-                float Lr[NC];
-#pragma unroll
-                for (int k = 0; k < NC; ++k) Lr[k] = Lsrc[k];
-                l2p_eval<P>(me.x, me.y, me.z, Lr, lphi, gx, gy, gz);
+                // far field from l2p_kernel
+                const float lphi = S.pot[i], gx = S.ax[i], gy = S.ay[i], gz = S.az[i];
                 const float fac = (float)(S.charge[i] / S.mass[i]) * F.G;
                 const float axo = fac * (ax + gx), ayo = fac * (ay + gy), azo = fac * (az + gz);
                 S.ax[i] = axo;
@@ -748,7 +758,11 @@ static void launch_dtable_t(const FmmParams& F, float* out, int level0, int nlev
 
 template <int P>
 static void launch_p2m_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
-    hipLaunchKernelGGL((p2m_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves);
+    hipLaunchKernelGGL((p2m_kernel<P>), dim3(ceil_div(num_leaves, 256)), dim3(256), 0, st, F, S, num_leaves);
+}
+template <int P>
+static void launch_l2p_t(const FmmParams& F, const State& S, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL((l2p_kernel<P>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, S, n);
 }
 template <int P>
 static void launch_m2m_t(const FmmParams& F, uint32_t count, uint32_t off, hipStream_t st) {
@@ -783,6 +797,10 @@ static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num
 void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     if (!num_leaves) return;
     NBH_DISPATCH_P(p, launch_p2m_t<PP>(F, S, num_leaves, st));
+}
+void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, hipStream_t st) {
+    if (!n) return;
+    NBH_DISPATCH_P(p, launch_l2p_t<PP>(F, S, n, st));
 }
 void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st) {
     (void)level;

@@ -130,6 +130,8 @@ void launch_gather_state(const State& src, const State& dst, const uint32_t* per
 void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
 void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
 void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
+// far field of every particle into (ax, ay, az, pot); the P2P kernels add the near field
+void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, hipStream_t st);
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st);
 void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt,
                        hipStream_t st);

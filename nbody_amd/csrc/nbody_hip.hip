@@ -489,6 +489,8 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     record(h, 4);
     for (uint32_t l = 1; l <= h->depth; ++l)
         launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
+    // far field per particle (timed with L2L); the P2P kernel completes it
+    if (!for_vjp) launch_l2p(p, F, S, (uint32_t)h->n, h->stream);
     record(h, 5);
     h->lastF = F;
     if (for_vjp) {
