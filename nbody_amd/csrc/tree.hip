@@ -96,11 +96,12 @@ __global__ __launch_bounds__(BB_THREADS) void bbox_partial_kernel(const double* 
     }
 }
 
-// one thread per system: fold partials, write box = {lo, size} and unit.
-__global__ void bbox_final_kernel(const double* part, int chunks, int nsys, const uint32_t* sys_off, int bits,
-                                  int fixed, double flo0, double flo1, double flo2, double fsize, double4* box,
-                                  double* inv, float* unit) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per system: fold its partials, write box = {lo, size} and unit.
+__global__ __launch_bounds__(WAVE) void bbox_final_kernel(const double* part, int chunks, int nsys,
+                                                          const uint32_t* sys_off, int bits, int fixed, double flo0,
+                                                          double flo1, double flo2, double fsize, double4* box,
+                                                          double* inv, float* unit) {
+    const int s = blockIdx.x;
     if (s >= nsys) return;
     double lo[3], size;
     if (fixed) {
@@ -109,18 +110,28 @@ __global__ void bbox_final_kernel(const double* part, int chunks, int nsys, cons
     } else {
         const int nch = (int)ceil_div(sys_off[s + 1] - sys_off[s], BB_TILE);
         double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int c = 0; c < nch; ++c)
+        for (int c = lane_id(); c < nch; c += WAVE)
             for (int a = 0; a < 3; ++a) {
                 l[a] = fmin(l[a], part[((size_t)s * chunks + c) * 6 + a]);
                 h[a] = fmax(h[a], part[((size_t)s * chunks + c) * 6 + 3 + a]);
             }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                l[a] = fmin(l[a], __shfl_xor(l[a], o, WAVE));
+                h[a] = fmax(h[a], __shfl_xor(h[a], o, WAVE));
+            }
+        }
         size = fmax(h[0] - l[0], fmax(h[1] - l[1], h[2] - l[2]));
         if (!(size > 0.0)) size = 1.0;
         for (int a = 0; a < 3; ++a) lo[a] = l[a];
     }
-    box[s] = make_double4(lo[0], lo[1], lo[2], size);
-    inv[s] = ldexp(1.0, bits) / size;
-    unit[s] = (float)ldexp(size, -(bits + 1));  // half of the finest cell
+    if (lane_id() == 0) {
+        box[s] = make_double4(lo[0], lo[1], lo[2], size);
+        inv[s] = ldexp(1.0, bits) / size;
+        unit[s] = (float)ldexp(size, -(bits + 1));  // half of the finest cell
+    }
 }
 
 // key = sys << 3B | interleave(qx, qy, qz), q = clamp(floor((x - lo) * 2^B / size))
@@ -339,7 +350,7 @@ void launch_bbox(const double* x, const double* y, const double* z, const uint32
     if (!fixed)
         hipLaunchKernelGGL(bbox_partial_kernel, dim3(chunks, nsys), dim3(BB_THREADS), 0, st, x, y, z, sys_off,
                            chunks, part, nonfinite);
-    hipLaunchKernelGGL(bbox_final_kernel, dim3(ceil_div(nsys, 64)), dim3(64), 0, st, part, chunks, nsys, sys_off,
+    hipLaunchKernelGGL(bbox_final_kernel, dim3(nsys), dim3(WAVE), 0, st, part, chunks, nsys, sys_off,
                        bits, fixed, flo[0], flo[1], flo[2], fsize, box, inv, unit);
 }
 
