@@ -232,9 +232,9 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
                     const float4 s = src[j];
                     const float rx = dx + s.x, ry = dy + s.y, rz = dz + s.z;
                     const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
-                    const float inv = rsqrtf(r2);
-                    const float q = j == self ? 0.f : s.w;
-                    const float qi = q * inv;
+                    // the self pair contributes nothing (r2 = 0 when unsoftened)
+                    const float inv = j == self ? 0.f : rsqrtf(r2);
+                    const float qi = s.w * inv;
                     const float qi3 = qi * inv * inv;
                     ax = fmaf(rx, qi3, ax);
                     ay = fmaf(ry, qi3, ay);
@@ -364,7 +364,9 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
                                 const float4 s = stage[j];
                                 const float rx = s.x - me.x, ry = s.y - me.y, rz = s.z - me.z;
                                 const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
-                                const float inv = rsqrtf(r2);
+                                // r2 >= eps2 is a normal float: the bare
+                                // v_rsq_f32, without rsqrtf's denormal scaling
+                                const float inv = __builtin_amdgcn_rsqf(r2);
                                 const float qi = s.w * inv;
                                 const float qi3 = qi * inv * inv;
                                 ax = fmaf(rx, qi3, ax);
@@ -391,7 +393,7 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
                 }
             }
             // the self term q_i/eps sits in exactly one stride's partial sum
-            if (eps2 > 0.f && sp == 0) ph -= me.w * rsqrtf(eps2);
+            if (eps2 > 0.f && sp == 0) ph -= me.w * __builtin_amdgcn_rsqf(eps2);
             // combine the source strides of each target
             {
                 const float ax0 = ax, ay0 = ay, az0 = az, ph0 = ph;
@@ -908,8 +910,9 @@ __global__ __launch_bounds__(DS_THREADS) void direct_f32_kernel(State S, uint32_
                 const float4 q = tile[k];
                 const float rx = q.x - px, ry = q.y - py, rz = q.z - pz;
                 const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
-                const float inv = rsqrtf(r2);
-                const float w = (j0 + k == i) ? 0.f : q.w * inv;
+                // the self pair contributes nothing (r2 = 0 when unsoftened)
+                const float inv = (j0 + k == i) ? 0.f : rsqrtf(r2);
+                const float w = q.w * inv;
                 const float w3 = w * inv * inv;
                 ax = fmaf(rx, w3, ax); ay = fmaf(ry, w3, ay); az = fmaf(rz, w3, az);
                 ph += w;

@@ -30,6 +30,7 @@ CASES = [
     ("plummer20k_cap32", "plummer", 20000, 32, 4, (1, 2), 1e-3),
     ("plummer8k_cap8_p6_theta04", "plummer", 8192, 8, 6, (2, 5), 5e-4),
     ("uniform3000_cap1_p2", "uniform", 3000, 1, 2, (1, 2), 1e-2),
+    ("uniform2048_cap16_unsoftened", "uniform", 2048, 16, 4, (1, 2), 0.0),
 ]
 
 
@@ -166,6 +167,21 @@ def test_direct_solver_kdk_matches_oracle():
     xo, vo = po.integrate_direct(x, v, m, m, 1.0, 1e-2, 1e-3, 5, 0)
     assert np.max(np.abs(xg - xo)) < 1e-9
     assert rms_rel(vg - v, vo - v) < 1e-4
+    sim.close()
+
+
+@pytest.mark.gpu
+def test_direct_solver_unsoftened_forces():
+    """eps = 0: the self pair must drop out exactly (no 0 * inf)."""
+    _case("direct_solver_unsoftened_1024")
+    n = 1024
+    x, v, m = nb.ics.uniform(n, 5)
+    sim = make_sim(x, v, m, solver=nb.SOLVER_DIRECT, eps=0.0, dt=1e-3)
+    acc, pot = sim.forces()
+    assert np.all(np.isfinite(acc)) and np.all(np.isfinite(pot))
+    ao, po_ = po.direct(x, m, eps=0.0)
+    assert rms_rel(acc, ao) < 1e-5
+    assert rms_rel(pot, po_) < 1e-6
     sim.close()
 
 
