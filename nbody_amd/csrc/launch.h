@@ -72,6 +72,8 @@ struct WalkParams {
 };
 void launch_walk_root(const WalkParams& W, hipStream_t st);
 void launch_walk_group(const WalkParams& W, int s, uint32_t grid, hipStream_t st);
+// workgroups of the group kernel resident at once on a device with num_cu CUs
+uint32_t walk_group_resident_blocks(int num_cu);
 
 // Read-only view of the lists for the force kernels.
 struct ListView {

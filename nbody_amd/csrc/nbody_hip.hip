@@ -88,6 +88,7 @@ struct nbh_handle {
     bool acc_valid = false;
     bool timing = false;
     int max_lds = 65536;  // LDS bytes per workgroup on this device
+    uint32_t walk_grid = 1024;  // group-kernel workgroups resident at once
     std::vector<uint32_t> sys_off_h;
 
     StateBufs st[2];
@@ -322,14 +323,14 @@ static int run_walk(nbh_handle* h) {
     Counters* dC = h->counters.as<Counters>();
     const uint32_t nn = h->num_nodes;
     if (h->row_cap < 2 * nn + 1024) CHK(grow_rows(h, 2 * nn + 1024));
-    // group-kernel grid per depth: about what stays resident, so that the
+    // group-kernel grid per depth: at most what stays resident, so that the
     // per-wave pool chunks waste little; every wave may leave one unused
     // chunk tail per depth
     uint32_t grid[MAX_LEVELS + 1] = {};
     unsigned long long waves = 0;
     for (uint32_t s = 1; s <= h->depth; ++s) {
         const uint32_t groups = h->hc->level_count[s - 1];
-        grid[s] = std::max<uint32_t>(std::min<uint32_t>(ceil_div(groups, WALK_WAVES) + 16, 1024), 64);
+        grid[s] = std::max<uint32_t>(std::min<uint32_t>(ceil_div(groups, WALK_WAVES) + 16, h->walk_grid), 64);
         waves += std::min<unsigned long long>((unsigned long long)grid[s] * WALK_WAVES, nn);
     }
     const unsigned long long waste_m = waves * POOL_CHUNK_M, waste_p = waves * POOL_CHUNK_P;
@@ -664,6 +665,11 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
     } while (0)
     CCHK(hipSetDevice(h->device));
     CCHK(hipDeviceGetAttribute(&h->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+    {
+        int ncu = 0;
+        CCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+        h->walk_grid = walk_group_resident_blocks(ncu);
+    }
     CCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     for (auto& e : h->ev) CCHK(hipEventCreate(&e));
     CCHK(h->st[0].alloc(n));

@@ -257,17 +257,11 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
         const int lt_level = nch > 0 ? la + 1 : la;
         const int sa = bits - lt_level;
         const int side_a = 1 << (sa + 1);
-        int cax[MAXT], cay[MAXT], caz[MAXT];
-        bool leaf_t[MAXT];
-#pragma unroll
-        for (int t = 0; t < MAXT; ++t) {
-            const int gx = __builtin_amdgcn_readlane(gt_l.x, t), gy = __builtin_amdgcn_readlane(gt_l.y, t);
-            const int gz = __builtin_amdgcn_readlane(gt_l.z, t), gw = __builtin_amdgcn_readlane(gt_l.w, t);
-            cax[t] = (2 * gx + 1) << sa;
-            cay[t] = (2 * gy + 1) << sa;
-            caz[t] = (2 * gz + 1) << sa;
-            leaf_t[t] = (gw >> 8) == 0;
-        }
+        // per-target state lives in lane t (centres, leaf flag, counters,
+        // cursors) and is read with readlane where the loop over targets
+        // needs it, which keeps it out of the scalar registers
+        const int cax_l = (2 * gt_l.x + 1) << sa, cay_l = (2 * gt_l.y + 1) << sa, caz_l = (2 * gt_l.z + 1) << sa;
+        const int leaf_l = (gt_l.w >> 8) == 0;
         const bool filled = sl_n <= (uint32_t)SL_CACHE;
         if (filled && sl_n) sl_cache_fill(W, cache, sl_st, 0, sl_n);
         // per-candidate part of the MAC, then the per-target categories
@@ -282,31 +276,35 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
 #pragma unroll
             for (int t = 0; t < MAXT; ++t) {
                 if (t < nt) {
-                    const int64_t dx = cbx - cax[t], dy = cby - cay[t], dz = cbz - caz[t];
+                    const int64_t dx = cbx - __builtin_amdgcn_readlane(cax_l, t);
+                    const int64_t dy = cby - __builtin_amdgcn_readlane(cay_l, t);
+                    const int64_t dz = cbz - __builtin_amdgcn_readlane(caz_l, t);
                     const uint64_t dsq = (uint64_t)(dx * dx + dy * dy + dz * dz);
                     const bool approx = B != A0 + t && lhs < num4 * dsq;
-                    const int cat = !valid ? 3 : approx ? (lb == lt_level ? 0 : 4) : (leaf_t[t] && b_leaf) ? 1 : 2;
+                    const bool both_leaf = __builtin_amdgcn_readlane(leaf_l, t) && b_leaf;
+                    const int cat = !valid ? 3 : approx ? (lb == lt_level ? 0 : 4) : both_leaf ? 1 : 2;
                     per_target(t, cat);
                 }
             }
         };
-        // pass 1: counts per target (uniform)
-        uint32_t nm[MAXT], nx[MAXT], np[MAXT], ns[MAXT];
-#pragma unroll
-        for (int t = 0; t < MAXT; ++t) nm[t] = nx[t] = np[t] = ns[t] = 0;
+        // pass 1: counts per target (lane t)
+        uint32_t nm_l = 0, nx_l = 0, np_l = 0, ns_l = 0;
         if (sl_n) {
             enumerate_sl(W, cache, sl_st, sl_n, filled, [&](bool valid, int B, int4 gb) {
                 classify_all(valid, B, gb, [&](int t, int cat) {
-                    nm[t] += __popcll(__ballot(cat == 0));
-                    nx[t] += __popcll(__ballot(cat == 4));
-                    np[t] += __popcll(__ballot(cat == 1));
-                    ns[t] += __popcll(__ballot(cat == 2));
+                    const uint32_t cm = __popcll(__ballot(cat == 0)), cx = __popcll(__ballot(cat == 4));
+                    const uint32_t cp = __popcll(__ballot(cat == 1)), cs = __popcll(__ballot(cat == 2));
+                    if (lane == t) {
+                        nm_l += cm;
+                        nx_l += cx;
+                        np_l += cp;
+                        ns_l += cs;
+                    }
                 });
             });
         }
-        uint32_t tm = 0, tp = 0, ts = 0;
-#pragma unroll
-        for (int t = 0; t < MAXT; ++t) { tm += nm[t] + nx[t]; tp += np[t]; ts += ns[t]; }
+        const uint32_t tm = wave_sum(nm_l + nx_l), tp = wave_sum(np_l), ts = wave_sum(ns_l);
+        const uint32_t tx = wave_sum(nx_l);
         // reserve one block per pool for all targets of the group
         uint32_t ms0 = 0, ps0 = 0, ss0 = 0, fail = 0;
         if (lane == 0) {
@@ -323,9 +321,7 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
                 if (ss0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_SL); fail = 1; }
             }
             if (!fail) {
-                acc_m += tm; acc_p += tp; acc_s += ts;
-#pragma unroll
-                for (int t = 0; t < MAXT; ++t) acc_x += nx[t];
+                acc_m += tm; acc_p += tp; acc_s += ts; acc_x += tx;
             }
         }
         fail = __shfl(fail, 0, WAVE);
@@ -333,54 +329,49 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
         ps0 = __shfl(ps0, 0, WAVE);
         ss0 = __shfl(ss0, 0, WAVE);
         if (fail) {  // the host retries with larger pools; leave empty rows
-#pragma unroll
-            for (int t = 0; t < MAXT; ++t) nm[t] = nx[t] = np[t] = ns[t] = 0;
-            tm = tp = ts = 0;
+            nm_l = nx_l = np_l = ns_l = 0;
         }
-        // per-target segment starts (uniform) and the rows (lane t); a row's
-        // M2L segment holds its same-level sources, then the others
-        uint32_t cm[MAXT], cx[MAXT], cp[MAXT], cs[MAXT];
-        {
-            uint32_t om = ms0, op = ps0, os = ss0;
-            uint32_t my_ms = 0, my_nm = 0, my_nx = 0, my_ps = 0, my_np = 0, my_ss = 0, my_ns = 0;
+        // segment starts of lane t's target: exclusive prefix over the
+        // targets (lanes 0..7); a row's M2L segment holds its same-level
+        // sources, then the others
+        uint32_t em = nm_l + nx_l, ep = np_l, es = ns_l;
 #pragma unroll
-            for (int t = 0; t < MAXT; ++t) {
-                cm[t] = om; cx[t] = om + nm[t]; cp[t] = op; cs[t] = os;
-                if (lane == t) {
-                    my_ms = om; my_nm = nm[t] + nx[t]; my_nx = nx[t];
-                    my_ps = op; my_np = np[t]; my_ss = os; my_ns = ns[t];
-                }
-                om += nm[t] + nx[t]; op += np[t]; os += ns[t];
-            }
-            if (lane < nt) {
-                const uint32_t r = row0 + lane;
-                W.row_node[r] = A0 + lane;
-                W.row_next[r] = NO_ROW;
-                W.row_m_start[r] = my_ms;
-                W.row_m_cnt[r] = my_nm;
-                W.row_mx_cnt[r] = my_nx;
-                W.row_p_start[r] = my_ps;
-                W.row_p_cnt[r] = my_np;
-                W.row_sl_start[r] = my_ss;
-                W.row_sl_cnt[r] = my_ns;
-                if (nch > 0) W.row_first[A0 + lane] = r;
-            }
-            if (nch == 0 && lane == 0) W.row_next[rg] = row0;
+        for (int o = 1; o < MAXT; o <<= 1) {
+            const uint32_t ym = __shfl_up(em, o, WAVE), yp = __shfl_up(ep, o, WAVE), ys = __shfl_up(es, o, WAVE);
+            if (lane >= o) { em += ym; ep += yp; es += ys; }
         }
+        uint32_t cm_l = ms0 + em - (nm_l + nx_l), cp_l = ps0 + ep - np_l, cs_l = ss0 + es - ns_l;
+        uint32_t cx_l = cm_l + nm_l;
+        if (lane < nt) {
+            const uint32_t r = row0 + lane;
+            W.row_node[r] = A0 + lane;
+            W.row_next[r] = NO_ROW;
+            W.row_m_start[r] = cm_l;
+            W.row_m_cnt[r] = nm_l + nx_l;
+            W.row_mx_cnt[r] = nx_l;
+            W.row_p_start[r] = cp_l;
+            W.row_p_cnt[r] = np_l;
+            W.row_sl_start[r] = cs_l;
+            W.row_sl_cnt[r] = ns_l;
+            if (nch > 0) W.row_first[A0 + lane] = r;
+        }
+        if (nch == 0 && lane == 0) W.row_next[rg] = row0;
         // pass 2: write every target's segments
-        if (tm + tp + ts) {
+        if (!fail && (tm | tp | ts)) {
             enumerate_sl(W, cache, sl_st, sl_n, filled, [&](bool valid, int B, int4 gb) {
                 classify_all(valid, B, gb, [&](int t, int cat) {
                     const uint64_t bm = __ballot(cat == 0), bp = __ballot(cat == 1), bs = __ballot(cat == 2);
                     const uint64_t bx = __ballot(cat == 4);
-                    if (cat == 0) W.m2l_pool[cm[t] + __popcll(bm & lt)] = B;
-                    if (cat == 4) W.m2l_pool[cx[t] + __popcll(bx & lt)] = B;
-                    cx[t] += __popcll(bx);
-                    if (cat == 1) W.p2p_pool[cp[t] + __popcll(bp & lt)] = B;
-                    if (cat == 2) W.sl[cs[t] + __popcll(bs & lt)] = B;
-                    cm[t] += __popcll(bm);
-                    cp[t] += __popcll(bp);
-                    cs[t] += __popcll(bs);
+                    if (cat == 0) W.m2l_pool[__builtin_amdgcn_readlane(cm_l, t) + __popcll(bm & lt)] = B;
+                    if (cat == 4) W.m2l_pool[__builtin_amdgcn_readlane(cx_l, t) + __popcll(bx & lt)] = B;
+                    if (cat == 1) W.p2p_pool[__builtin_amdgcn_readlane(cp_l, t) + __popcll(bp & lt)] = B;
+                    if (cat == 2) W.sl[__builtin_amdgcn_readlane(cs_l, t) + __popcll(bs & lt)] = B;
+                    if (lane == t) {
+                        cm_l += __popcll(bm);
+                        cx_l += __popcll(bx);
+                        cp_l += __popcll(bp);
+                        cs_l += __popcll(bs);
+                    }
                 });
             });
         }
@@ -396,6 +387,14 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
 
 void launch_walk_root(const WalkParams& W, hipStream_t st) {
     hipLaunchKernelGGL(walk_root_kernel, dim3(1), dim3(64), 0, st, W);
+}
+uint32_t walk_group_resident_blocks(int num_cu) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&walk_group_kernel),
+                                                     WAVE * WALK_WAVES, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)per_cu * (uint32_t)(num_cu > 0 ? num_cu : 1);
 }
 void launch_walk_group(const WalkParams& W, int s, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(walk_group_kernel, dim3(grid), dim3(WAVE * WALK_WAVES), 0, st, W, s);
