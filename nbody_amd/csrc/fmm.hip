@@ -632,15 +632,15 @@ __global__ __launch_bounds__(256) void m2l_cross_kernel(FmmParams F, uint32_t nu
         }
         nfar = wave_sum(nfar);
         if (nfar) {
-            uint32_t b = 0, s0 = 0;
+            uint32_t s0 = 0;
             if (lane == 0) {
-                b = atomicAdd(&F.far_cnt[0], 1u);
+                atomicAdd(&F.far_cnt[0], 1u);
                 s0 = atomicAdd(&F.far_cnt[1], nfar);
             }
-            b = __shfl(b, 0, WAVE);
             s0 = __shfl(s0, 0, WAVE);
-            // capacities are sized from the walk's count of other-level entries
-            if (b >= F.far_blk_cap || (unsigned long long)s0 + nfar > F.far_cap) continue;
+            // the capacity is the walk's count of other-level entries, which
+            // bounds the deferred pairs
+            if ((unsigned long long)s0 + nfar > F.far_cap) continue;
             uint32_t pos = s0;
             const uint64_t lt = lanemask_lt();
             for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
@@ -656,47 +656,65 @@ __global__ __launch_bounds__(256) void m2l_cross_kernel(FmmParams F, uint32_t nu
                         far = xtab_index(F, ga, F.nodes.geo[B], neg) < 0;
                     }
                     const uint64_t bal = __ballot(far);
-                    if (far) F.far_src[pos + __popcll(bal & lt)] = B;
+                    if (far) {
+                        F.far_src[pos + __popcll(bal & lt)] = B;
+                        F.far_tgt[pos + __popcll(bal & lt)] = (int)A;
+                    }
                     pos += __popcll(bal);
                 }
             }
-            if (lane == 0) F.far_blk[b] = make_uint4(A, s0, nfar, 0);
         }
     }
 }
 
-// Deferred pairs (levels two or more apart): tensor built per pair; one
-// wavefront per target block; adds to L.
+// Deferred pairs (levels two or more apart, ~0.1% of the pairs, a handful
+// per target): tensor built per pair, one lane per pair (a wavefront per
+// target would leave most lanes idle).  A target's pairs are contiguous, so
+// the lanes of a wavefront form runs of equal targets: a segmented suffix sum
+// over the lanes leaves each run's total in its first lane, which adds it to
+// the target's L with float atomics (a run may continue in the next
+// wavefront).
 template <int P, int Q>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_far_kernel(FmmParams F) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_far_kernel(FmmParams F,
+                                                                                                uint32_t num_nodes) {
     constexpr int NC = ncoef(P);
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const uint32_t nw = gridDim.x * blockDim.x / WAVE;
+    const uint32_t npairs = min(F.far_cnt[1], F.far_cap);
     const int lane = lane_id();
-    const uint32_t nblk = min(F.far_cnt[0], F.far_blk_cap);
-    for (uint32_t b = wid; b < nblk; b += nw) {
-        const uint4 blk = F.far_blk[b];
-        if ((unsigned long long)blk.y + blk.z > F.far_cap) continue;
-        const int A = (int)blk.x;
-        const int4 ga = F.nodes.geo[A];
-        const float unit = F.unit[F.nodes.sys[A]];
+    const uint32_t w0 = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE * WAVE;
+    for (uint32_t base = w0; base < npairs; base += gridDim.x * blockDim.x) {  // uniform
+        const uint32_t k = base + lane;
+        int A = -1, B = -1;
+        if (k < npairs) {
+            A = F.far_tgt[k];
+            B = F.far_src[k];
+            if ((uint32_t)A >= num_nodes || (uint32_t)B >= num_nodes) A = B = -1;
+        }
         float L[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) L[k] = 0.f;
-        for (uint32_t e = lane; e < blk.z; e += WAVE) {
-            const int B = F.far_src[blk.y + e];
-            const int4 gb = F.nodes.geo[B];
-            const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
+        for (int c = 0; c < NC; ++c) L[c] = 0.f;
+        if (A >= 0) {
+            const int4 ga = F.nodes.geo[A];
+            const float unit = F.unit[F.nodes.sys[A]];
+            const float3 d = centre_diff(F.nodes.geo[B], ga, F.bits, unit);  // R = c_A - c_B
             float D[ncoef(Q)];
             derivs<Q>(d.x, d.y, d.z, F.eps2, D);
             m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
         }
 #pragma unroll
-        for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
-        if (lane == 0) {
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const int An = __shfl_down(A, off, WAVE);
+            const bool same = lane + off < WAVE && An == A;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const float o = __shfl_down(L[c], off, WAVE);
+                if (same) L[c] += o;
+            }
+        }
+        const int Ap = __shfl_up(A, 1, WAVE);
+        if (A >= 0 && (lane == 0 || Ap != A)) {
             float* out = F.L + (size_t)A * F.ncp;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) out[k] += L[k];
+            for (int c = 0; c < NC; ++c) atomicAdd(out + c, L[c]);
         }
     }
 }
@@ -747,7 +765,9 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, hipStream_t s
                                l - F.dlevel0);
     }
     hipLaunchKernelGGL((m2l_cross_kernel<P, Q>), dim3(grid), dim3(256), 0, st, F, nn);
-    hipLaunchKernelGGL((m2l_far_kernel<P, Q>), dim3(1024), dim3(256), 0, st, F);
+    // the deferred pairs are a subset of the other-level entries (far_cap)
+    const unsigned gfar = std::max<unsigned>(1u, std::min<unsigned>(ceil_div(F.far_cap, 256u), 1024u));
+    hipLaunchKernelGGL((m2l_far_kernel<P, Q>), dim3(gfar), dim3(256), 0, st, F, nn);
 }
 
 template <int Q>

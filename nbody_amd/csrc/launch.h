@@ -120,11 +120,12 @@ struct FmmParams {
     // d (units of half the finer cell, odd) with |d_i| = 1, 3, .., 2 xE - 1
     float* xtab;
     int xE, xlevel0, xnlev;
-    // pairs further apart in level: deferred per target, (target, first, count)
-    uint4* far_blk;
+    // pairs further apart in level: deferred as (target, source) pairs, the
+    // pairs of one target contiguous
+    int* far_tgt;
     int* far_src;
-    uint32_t far_cap, far_blk_cap;
-    uint32_t* far_cnt;     // [blocks, pairs]
+    uint32_t far_cap;
+    uint32_t* far_cnt;     // [targets, pairs]
     double eps2_64;
 };
 

@@ -95,7 +95,7 @@ struct nbh_handle {
     int cur = 0;
     DevBuf sys_off, keys, keys_alt, perm, perm_alt, sort_tmp, tree_tmp, scan_tmp;
     DevBuf box, inv, unit, bbox_part;
-    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab, xtab, far_src, far_blk;
+    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab, xtab, far_src, far_tgt;
     DevBuf row_node, row_first, row_next, row_sl_start, row_sl_cnt, sl, row_m_start, row_m_cnt, row_mx_cnt,
         row_p_start, row_p_cnt, m2l_pool, p2p_pool;
     unsigned long long m2l_pool_cap = 0, p2p_pool_cap = 0;
@@ -478,11 +478,10 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
             // deferred pairs: at most every other-level entry, one block per target
             const unsigned long long nx = std::max<unsigned long long>(h->hc->m2lx_total, 1);
             HIPCHK(h, h->far_src.ensure((size_t)nx * 4));
-            HIPCHK(h, h->far_blk.ensure((size_t)h->num_nodes * 16));
+            HIPCHK(h, h->far_tgt.ensure((size_t)nx * 4));
             F.far_src = h->far_src.as<int>();
-            F.far_blk = h->far_blk.as<uint4>();
+            F.far_tgt = h->far_tgt.as<int>();
             F.far_cap = (uint32_t)std::min<unsigned long long>(nx, 0xffffffffull);
-            F.far_blk_cap = h->num_nodes;
             F.far_cnt = &h->counters.as<Counters>()->far_blocks;
         }
     }
@@ -752,7 +751,7 @@ void nbh_destroy(nbh_handle* h) {
     h->st[1].release();
     DevBuf* all[] = {&h->sys_off, &h->keys, &h->keys_alt, &h->perm, &h->perm_alt, &h->sort_tmp, &h->tree_tmp,
                      &h->scan_tmp, &h->box, &h->inv, &h->unit, &h->bbox_part, &h->geo, &h->link, &h->nsys_of,
-                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->xtab, &h->far_src, &h->far_blk, &h->row_node, &h->row_first,
+                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->xtab, &h->far_src, &h->far_tgt, &h->row_node, &h->row_first,
                      &h->row_next, &h->row_sl_start, &h->row_sl_cnt, &h->row_m_start, &h->row_m_cnt,
                      &h->row_mx_cnt, &h->row_p_start, &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->counters, &h->diag,
                      &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf};
