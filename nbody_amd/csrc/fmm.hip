@@ -756,7 +756,9 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, hipStream_t s
         const uint32_t cnt = plan.level_count[l];
         if (!cnt) continue;
         constexpr int T = m2l_table_threads(P, Q);
-        const unsigned g = std::min<unsigned>(ceil_div(cnt, T / WAVE), 2048u);
+        // one target per wavefront: the workgroup scheduler balances the
+        // uneven list lengths better than a grid-stride loop (measured)
+        const unsigned g = std::min<unsigned>(ceil_div(cnt, T / WAVE), 65535u);
         if (in_lds)
             hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true>), dim3(g), dim3(T), lds, st, F, plan.level_off[l], cnt,
                                l - F.dlevel0);
