@@ -323,18 +323,18 @@ static int run_walk(nbh_handle* h) {
     Counters* dC = h->counters.as<Counters>();
     const uint32_t nn = h->num_nodes;
     if (h->row_cap < 2 * nn + 1024) CHK(grow_rows(h, 2 * nn + 1024));
-    // group-kernel grid per depth: at most what stays resident, so that the
-    // per-wave pool chunks waste little; every wave may leave one unused
-    // chunk tail per depth
+    // group-kernel grid per depth (one workgroup per group at a time): at
+    // most what stays resident, so that the per-workgroup pool chunks waste
+    // little; every workgroup may leave one unused chunk tail per depth
     uint32_t grid[MAX_LEVELS + 1] = {};
-    unsigned long long waves = 0;
+    unsigned long long blocks = 0;
     for (uint32_t s = 1; s <= h->depth; ++s) {
         const uint32_t groups = h->hc->level_count[s - 1];
-        grid[s] = std::max<uint32_t>(std::min<uint32_t>(ceil_div(groups, WALK_WAVES) + 16, h->walk_grid), 64);
-        waves += std::min<unsigned long long>((unsigned long long)grid[s] * WALK_WAVES, nn);
+        grid[s] = std::max<uint32_t>(std::min<uint32_t>(groups + 16, h->walk_grid), 64);
+        blocks += std::min<unsigned long long>(grid[s], nn);
     }
-    const unsigned long long waste_m = waves * POOL_CHUNK_M, waste_p = waves * POOL_CHUNK_P;
-    const unsigned long long waste_s = waves * POOL_CHUNK_S;
+    const unsigned long long waste_m = blocks * POOL_CHUNK_M, waste_p = blocks * POOL_CHUNK_P;
+    const unsigned long long waste_s = blocks * POOL_CHUNK_S;
     // pools: sized from the last evaluation (or a per-node guess) and grown
     // on overflow
     if (h->sl_cap < 64ull * nn + waste_s) {

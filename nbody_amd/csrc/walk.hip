@@ -71,20 +71,18 @@ __device__ __forceinline__ void sl_cache_fill(const WalkParams& W, const SLCache
     wave_lds_sync();
 }
 
-// Enumerate expand(SL) for the split list at sl[st, st+cnt); f(valid, B, geo)
-// is called once per chunk of 64 lanes (uniformly), in ascending candidate
-// order: first the entries without children (kept as themselves), then the
-// children of the others, entry by entry.  `filled`: the cache already holds
-// the whole list (cnt <= SL_CACHE).
+// Enumerate expand(SL) for the split list at sl[st, st+cnt), one wavefront
+// with the cache to itself; f(phase, valid, B, geo) is called once per chunk
+// of 64 lanes (uniformly), in ascending candidate order: phase 0 the entries
+// without children (kept as themselves), then phase 1 the children of the
+// others, entry by entry.  The cache is refilled SL_CACHE entries at a time.
 template <typename F>
 __device__ __forceinline__ void enumerate_sl(const WalkParams& W, const SLCache& cache, uint32_t st, uint32_t cnt,
-                                             bool filled, F&& f) {
+                                             F&& f) {
     const int lane = lane_id();
-    const bool one_block = cnt <= (uint32_t)SL_CACHE;
-    // phase a: sources kept as themselves (childless)
     for (uint32_t p0 = 0; p0 < cnt; p0 += SL_CACHE) {
         const uint32_t pn = min((uint32_t)SL_CACHE, cnt - p0);
-        if (!(filled && one_block)) sl_cache_fill(W, cache, st, p0, pn);
+        sl_cache_fill(W, cache, st, p0, pn);
         for (uint32_t e0 = 0; e0 < pn; e0 += WAVE) {
             const uint32_t e = e0 + lane;
             int B = -1;
@@ -95,15 +93,13 @@ __device__ __forceinline__ void enumerate_sl(const WalkParams& W, const SLCache&
             }
             const bool valid = B >= 0 && (gb.w >> 8) == 0;
             if (__ballot(valid) == 0) continue;
-            f(valid, B, gb);
+            f(0, valid, B, gb);
         }
-        if (!one_block) wave_lds_sync();
+        wave_lds_sync();
     }
-    // phase b: children of the sources that have children, 8 entries x 8
-    // slots per chunk, two chunks in flight
     for (uint32_t p0 = 0; p0 < cnt; p0 += SL_CACHE) {
         const uint32_t pn = min((uint32_t)SL_CACHE, cnt - p0);
-        if (!one_block) sl_cache_fill(W, cache, st, p0, pn);
+        sl_cache_fill(W, cache, st, p0, pn);
         const int c = lane & 7;
         for (uint32_t e0 = 0; e0 < pn; e0 += 16) {
             bool v0 = false, v1 = false;
@@ -114,10 +110,44 @@ __device__ __forceinline__ void enumerate_sl(const WalkParams& W, const SLCache&
             if (eb < pn && c < (cache.geo[eb].w >> 8)) { C1 = cache.fc[eb] + c; v1 = true; }
             if (v0) g0 = W.nodes.geo[C0];
             if (v1) g1 = W.nodes.geo[C1];
-            if (__ballot(v0)) f(v0, C0, g0);
-            if (__ballot(v1)) f(v1, C1, g1);
+            if (__ballot(v0)) f(1, v0, C0, g0);
+            if (__ballot(v1)) f(1, v1, C1, g1);
         }
-        if (!one_block) wave_lds_sync();
+        wave_lds_sync();
+    }
+}
+
+// The same for the cached entries [lo, hi) of a list that the cache holds
+// whole (one wavefront's slice of a workgroup-shared cache).
+template <typename F>
+__device__ __forceinline__ void enumerate_slice(const WalkParams& W, const SLCache& cache, uint32_t lo, uint32_t hi,
+                                                F&& f) {
+    const int lane = lane_id();
+    for (uint32_t e0 = lo; e0 < hi; e0 += WAVE) {
+        const uint32_t e = e0 + lane;
+        int B = -1;
+        int4 gb = make_int4(0, 0, 0, 0);
+        if (e < hi) {
+            B = cache.node[e];
+            gb = cache.geo[e];
+        }
+        const bool valid = B >= 0 && (gb.w >> 8) == 0;
+        if (__ballot(valid) == 0) continue;
+        f(0, valid, B, gb);
+    }
+    // children: 8 entries x 8 slots per chunk, two chunks in flight
+    const int c = lane & 7;
+    for (uint32_t e0 = lo; e0 < hi; e0 += 16) {
+        bool v0 = false, v1 = false;
+        int C0 = -1, C1 = -1;
+        int4 g0 = make_int4(0, 0, 0, 0), g1 = make_int4(0, 0, 0, 0);
+        const uint32_t ea = e0 + (lane >> 3), eb = ea + 8;
+        if (ea < hi && c < (cache.geo[ea].w >> 8)) { C0 = cache.fc[ea] + c; v0 = true; }
+        if (eb < hi && c < (cache.geo[eb].w >> 8)) { C1 = cache.fc[eb] + c; v1 = true; }
+        if (v0) g0 = W.nodes.geo[C0];
+        if (v1) g1 = W.nodes.geo[C1];
+        if (__ballot(v0)) f(1, v0, C0, g0);
+        if (__ballot(v1)) f(1, v1, C1, g1);
     }
 }
 
@@ -190,11 +220,23 @@ __global__ __launch_bounds__(64) void walk_root_kernel(WalkParams W) {
 
 constexpr int MAXT = 8;  // targets per group: the children of one cell
 
-// Depth s >= 1: one wavefront per group (row of depth s-1).
+// Depth s >= 1: one workgroup of WALK_WAVES wavefronts per group (row of
+// depth s-1).  The group's split list is cached once for the workgroup and
+// each wavefront classifies the candidates of a contiguous slice of it
+// against all targets.  Pass 1 counts per (wavefront, phase, target,
+// category); thread 0 reserves one block per pool for the group; every
+// wavefront then writes its entries at offsets ordered phase-major,
+// wavefront-minor, which is the order of one wavefront walking the whole
+// list.  Lists longer than the cache are walked by wavefront 0 alone.  Rows
+// are taken with a fixed stride over the workgroups; their headers are read
+// 64 at a time so rows without work (leaves with nothing left to split) cost
+// one batched read.
 __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParams W, int s) {
-    __shared__ int4 cgeo[WALK_WAVES][SL_CACHE];
-    __shared__ int cnode[WALK_WAVES][SL_CACHE];
-    __shared__ int cfc[WALK_WAVES][SL_CACHE];
+    __shared__ int4 cgeo[SL_CACHE];
+    __shared__ int cnode[SL_CACHE];
+    __shared__ int cfc[SL_CACHE];
+    __shared__ uint32_t cnt_s[WALK_WAVES][2][4][MAXT];  // [wave][phase][same-level, other-level, P2P, split][target]
+    __shared__ uint32_t bcast[8];                       // carried row, pool starts, failure
     Counters* C = W.C;
     const uint32_t n_prev_level = C->level_count[s - 1];
     const uint32_t n_groups = n_prev_level + C->carried[s - 1];
@@ -203,181 +245,230 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
     const uint32_t n_level = C->level_count[s];
     const uint32_t loff = C->level_off[s], loff_prev = C->level_off[s - 1];
     const int lane = lane_id();
-    const int wslot = threadIdx.x / WAVE;
-    const SLCache cache{cgeo[wslot], cnode[wslot], cfc[wslot]};
-    const uint32_t wid = blockIdx.x * WALK_WAVES + wslot;
-    const uint32_t nw = gridDim.x * WALK_WAVES;
+    const int wv = threadIdx.x / WAVE;
+    const SLCache cache{cgeo, cnode, cfc};
     const uint64_t lt = lanemask_lt();
     const int bits = W.bits;
     const uint64_t num4 = W.num4, den3 = W.den3;
-    uint2 mchunk = make_uint2(0, 0), pchunk = make_uint2(0, 0), schunk = make_uint2(0, 0);  // lane 0 only
-    unsigned long long acc_m = 0, acc_p = 0, acc_s = 0, acc_x = 0;                           // lane 0 only
-    for (uint32_t g = wid; g < n_groups; g += nw) {
-        const uint32_t rg = base_prev + g;
-        if (rg >= W.row_cap) continue;  // flagged when the row was made
-        const int P = g < n_prev_level ? (int)(loff_prev + g) : W.row_node[rg];
-        const int4 gp = W.nodes.geo[P];
-        const int nch = gp.w >> 8;
-        const uint32_t sl_st = W.row_sl_start[rg];
-        uint32_t sl_n = W.row_sl_cnt[rg];
-        if ((unsigned long long)sl_st + sl_n > W.sl_cap) sl_n = 0;  // only after an overflow
-        int nt, A0;
-        uint32_t row0;
-        if (nch > 0) {  // the cell's children: level-s rows in id order
-            nt = nch;
-            A0 = W.nodes.link[P].z;
-            row0 = base + (uint32_t)(A0 - (int)loff);
-        } else {  // a leaf with sources left to split: carried to depth s
-            if (sl_n == 0) continue;
-            nt = 1;
-            A0 = P;
-            uint32_t k = 0;
-            if (lane == 0) k = atomicAdd(&C->carried[s], 1u);
-            k = __shfl(k, 0, WAVE);
-            row0 = base + n_level + k;
+    const uint32_t nblk = gridDim.x;
+    uint2 mchunk = make_uint2(0, 0), pchunk = make_uint2(0, 0), schunk = make_uint2(0, 0);  // thread 0 only
+    unsigned long long acc_m = 0, acc_p = 0, acc_s = 0, acc_x = 0;                           // thread 0 only
+    for (uint64_t j0 = 0; (uint64_t)blockIdx.x + j0 * nblk < n_groups; j0 += WAVE) {
+        // headers of the next 64 rows of this workgroup (identical in every wavefront)
+        const uint64_t gl = (uint64_t)blockIdx.x + (j0 + lane) * nblk;
+        int P_l = 0;
+        int4 gp_l = make_int4(0, 0, 0, 0);
+        uint32_t st_l = 0, sn_l = 0;
+        bool live = false;
+        if (gl < n_groups && base_prev + gl < W.row_cap) {  // rows past the cap were flagged when made
+            const uint32_t g = (uint32_t)gl, rg = base_prev + g;
+            P_l = g < n_prev_level ? (int)(loff_prev + g) : W.row_node[rg];
+            gp_l = W.nodes.geo[P_l];
+            st_l = W.row_sl_start[rg];
+            sn_l = W.row_sl_cnt[rg];
+            if ((unsigned long long)st_l + sn_l > W.sl_cap) sn_l = 0;  // only after an overflow
+            live = (gp_l.w >> 8) > 0 || sn_l > 0;
         }
-        if ((unsigned long long)row0 + nt > W.row_cap) {
-            // the host retries with more rows; rows that fit are left empty so
-            // that the next depth reads only rows of this evaluation
-            if (lane == 0) atomicOr(&C->overflow, (uint32_t)OVF_ROWS);
-            const uint32_t r = row0 + lane;
-            if (lane < nt && r < W.row_cap) {
-                W.row_node[r] = A0 + lane;
-                W.row_next[r] = NO_ROW;
-                W.row_m_start[r] = W.row_m_cnt[r] = W.row_mx_cnt[r] = 0;
-                W.row_p_start[r] = W.row_p_cnt[r] = 0;
-                W.row_sl_start[r] = W.row_sl_cnt[r] = 0;
+        for (uint64_t todo = __ballot(live); todo; todo &= todo - 1) {  // uniform over the workgroup
+            const int jl = __ffsll((unsigned long long)todo) - 1;
+            const uint32_t g = (uint32_t)(blockIdx.x + (j0 + jl) * nblk);
+            const uint32_t rg = base_prev + g;
+            const int P = __builtin_amdgcn_readlane(P_l, jl);
+            const int gpw = __builtin_amdgcn_readlane(gp_l.w, jl);
+            const uint32_t sl_st = __builtin_amdgcn_readlane(st_l, jl);
+            const uint32_t sl_n = __builtin_amdgcn_readlane(sn_l, jl);
+            const int nch = gpw >> 8;
+            __syncthreads();  // the previous group is done with the cache, counts and broadcasts
+            if (nch == 0 && threadIdx.x == 0) bcast[0] = atomicAdd(&C->carried[s], 1u);
+            __syncthreads();
+            int nt, A0;
+            uint32_t row0;
+            if (nch > 0) {  // the cell's children: level-s rows in id order
+                nt = nch;
+                A0 = W.nodes.link[P].z;
+                row0 = base + (uint32_t)(A0 - (int)loff);
+            } else {  // a leaf with sources left to split: carried to depth s
+                nt = 1;
+                A0 = P;
+                row0 = base + n_level + bcast[0];
             }
-            continue;
-        }
-        // targets (all of one level): lane t < nt holds target t
-        int4 gt_l = make_int4(0, 0, 0, 0);
-        if (lane < nt) gt_l = W.nodes.geo[A0 + lane];
-        const int la = gp.w & 0xff;
-        const int lt_level = nch > 0 ? la + 1 : la;
-        const int sa = bits - lt_level;
-        const int side_a = 1 << (sa + 1);
-        // per-target state lives in lane t (centres, leaf flag, counters,
-        // cursors) and is read with readlane where the loop over targets
-        // needs it, which keeps it out of the scalar registers
-        const int cax_l = (2 * gt_l.x + 1) << sa, cay_l = (2 * gt_l.y + 1) << sa, caz_l = (2 * gt_l.z + 1) << sa;
-        const int leaf_l = (gt_l.w >> 8) == 0;
-        const bool filled = sl_n <= (uint32_t)SL_CACHE;
-        if (filled && sl_n) sl_cache_fill(W, cache, sl_st, 0, sl_n);
-        // per-candidate part of the MAC, then the per-target categories
-        // (0 = M2L same level, 4 = M2L other level, 1 = P2P, 2 = split, 3 = none)
-        auto classify_all = [&](bool valid, int B, int4 gb, auto&& per_target) {
-            const int lb = gb.w & 0xff;
-            const int sb = bits - lb;
-            const int cbx = (2 * gb.x + 1) << sb, cby = (2 * gb.y + 1) << sb, cbz = (2 * gb.z + 1) << sb;
-            const uint64_t side = (uint64_t)side_a + (1ull << (sb + 1));
-            const uint64_t lhs = den3 * side * side;
-            const bool b_leaf = (gb.w >> 8) == 0;
-#pragma unroll
-            for (int t = 0; t < MAXT; ++t) {
-                if (t < nt) {
-                    const int64_t dx = cbx - __builtin_amdgcn_readlane(cax_l, t);
-                    const int64_t dy = cby - __builtin_amdgcn_readlane(cay_l, t);
-                    const int64_t dz = cbz - __builtin_amdgcn_readlane(caz_l, t);
-                    const uint64_t dsq = (uint64_t)(dx * dx + dy * dy + dz * dz);
-                    const bool approx = B != A0 + t && lhs < num4 * dsq;
-                    const bool both_leaf = __builtin_amdgcn_readlane(leaf_l, t) && b_leaf;
-                    const int cat = !valid ? 3 : approx ? (lb == lt_level ? 0 : 4) : both_leaf ? 1 : 2;
-                    per_target(t, cat);
+            if ((unsigned long long)row0 + nt > W.row_cap) {
+                // the host retries with more rows; rows that fit are left empty so
+                // that the next depth reads only rows of this evaluation
+                if (threadIdx.x == 0) atomicOr(&C->overflow, (uint32_t)OVF_ROWS);
+                const uint32_t r = row0 + lane;
+                if (wv == 0 && lane < nt && r < W.row_cap) {
+                    W.row_node[r] = A0 + lane;
+                    W.row_next[r] = NO_ROW;
+                    W.row_m_start[r] = W.row_m_cnt[r] = W.row_mx_cnt[r] = 0;
+                    W.row_p_start[r] = W.row_p_cnt[r] = 0;
+                    W.row_sl_start[r] = W.row_sl_cnt[r] = 0;
+                }
+                continue;
+            }
+            // targets (all of one level): lane t < nt holds target t; per-target
+            // state lives in lane t and is read with readlane where the loop
+            // over targets needs it
+            int4 gt_l = make_int4(0, 0, 0, 0);
+            if (lane < nt) gt_l = W.nodes.geo[A0 + lane];
+            const int la = gpw & 0xff;
+            const int lt_level = nch > 0 ? la + 1 : la;
+            const int sa = bits - lt_level;
+            const int side_a = 1 << (sa + 1);
+            const int cax_l = (2 * gt_l.x + 1) << sa, cay_l = (2 * gt_l.y + 1) << sa, caz_l = (2 * gt_l.z + 1) << sa;
+            const int leaf_l = (gt_l.w >> 8) == 0;
+            // the split list: cached for the workgroup and sliced over the
+            // wavefronts, or (longer than the cache) walked by wavefront 0
+            const bool solo = sl_n > (uint32_t)SL_CACHE;
+            if (!solo) {
+                for (uint32_t k = threadIdx.x; k < sl_n; k += blockDim.x) {
+                    const int B = W.sl[sl_st + k];
+                    cache.node[k] = B;
+                    cache.geo[k] = W.nodes.geo[B];
+                    cache.fc[k] = W.nodes.link[B].z;
                 }
             }
-        };
-        // pass 1: counts per target (lane t)
-        uint32_t nm_l = 0, nx_l = 0, np_l = 0, ns_l = 0;
-        if (sl_n) {
-            enumerate_sl(W, cache, sl_st, sl_n, filled, [&](bool valid, int B, int4 gb) {
+            __syncthreads();
+            const uint32_t lo = solo ? 0u : (uint32_t)(((uint64_t)sl_n * wv) / WALK_WAVES);
+            const uint32_t hi = solo ? (wv == 0 ? sl_n : 0u) : (uint32_t)(((uint64_t)sl_n * (wv + 1)) / WALK_WAVES);
+            auto each = [&](auto&& f) {
+                if (solo) {
+                    if (wv == 0 && sl_n) enumerate_sl(W, cache, sl_st, sl_n, f);
+                } else if (lo < hi) {
+                    enumerate_slice(W, cache, lo, hi, f);
+                }
+            };
+            // per-candidate part of the MAC, then the per-target categories
+            // (0 = M2L same level, 4 = M2L other level, 1 = P2P, 2 = split, 3 = none)
+            auto classify_all = [&](bool valid, int B, int4 gb, auto&& per_target) {
+                const int lb = gb.w & 0xff;
+                const int sb = bits - lb;
+                const int cbx = (2 * gb.x + 1) << sb, cby = (2 * gb.y + 1) << sb, cbz = (2 * gb.z + 1) << sb;
+                const uint64_t side = (uint64_t)side_a + (1ull << (sb + 1));
+                const uint64_t lhs = den3 * side * side;
+                const bool b_leaf = (gb.w >> 8) == 0;
+#pragma unroll
+                for (int t = 0; t < MAXT; ++t) {
+                    if (t < nt) {
+                        const int64_t dx = cbx - __builtin_amdgcn_readlane(cax_l, t);
+                        const int64_t dy = cby - __builtin_amdgcn_readlane(cay_l, t);
+                        const int64_t dz = cbz - __builtin_amdgcn_readlane(caz_l, t);
+                        const uint64_t dsq = (uint64_t)(dx * dx + dy * dy + dz * dz);
+                        const bool approx = B != A0 + t && lhs < num4 * dsq;
+                        const bool both_leaf = __builtin_amdgcn_readlane(leaf_l, t) && b_leaf;
+                        const int cat = !valid ? 3 : approx ? (lb == lt_level ? 0 : 4) : both_leaf ? 1 : 2;
+                        per_target(t, cat);
+                    }
+                }
+            };
+            // pass 1: counts of this wavefront's slice per phase, category, target (lane t)
+            uint32_t k0m = 0, k0x = 0, k0p = 0, k0s = 0, k1m = 0, k1x = 0, k1p = 0, k1s = 0;
+            each([&](int ph, bool valid, int B, int4 gb) {
                 classify_all(valid, B, gb, [&](int t, int cat) {
                     const uint32_t cm = __popcll(__ballot(cat == 0)), cx = __popcll(__ballot(cat == 4));
                     const uint32_t cp = __popcll(__ballot(cat == 1)), cs = __popcll(__ballot(cat == 2));
                     if (lane == t) {
-                        nm_l += cm;
-                        nx_l += cx;
-                        np_l += cp;
-                        ns_l += cs;
+                        if (ph == 0) { k0m += cm; k0x += cx; k0p += cp; k0s += cs; }
+                        else { k1m += cm; k1x += cx; k1p += cp; k1s += cs; }
                     }
                 });
             });
-        }
-        const uint32_t tm = wave_sum(nm_l + nx_l), tp = wave_sum(np_l), ts = wave_sum(ns_l);
-        const uint32_t tx = wave_sum(nx_l);
-        // reserve one block per pool for all targets of the group
-        uint32_t ms0 = 0, ps0 = 0, ss0 = 0, fail = 0;
-        if (lane == 0) {
-            if (tm) {
-                ms0 = pool_take(mchunk, tm, POOL_CHUNK_M, &C->m2l_pool_used, W.m2l_pool_cap);
-                if (ms0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_M2L); fail = 1; }
+            if (lane < MAXT) {
+                cnt_s[wv][0][0][lane] = k0m; cnt_s[wv][0][1][lane] = k0x;
+                cnt_s[wv][0][2][lane] = k0p; cnt_s[wv][0][3][lane] = k0s;
+                cnt_s[wv][1][0][lane] = k1m; cnt_s[wv][1][1][lane] = k1x;
+                cnt_s[wv][1][2][lane] = k1p; cnt_s[wv][1][3][lane] = k1s;
             }
-            if (tp) {
-                ps0 = pool_take(pchunk, tp, POOL_CHUNK_P, &C->p2p_pool_used, W.p2p_pool_cap);
-                if (ps0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_P2P); fail = 1; }
-            }
-            if (ts) {
-                ss0 = pool_take(schunk, ts, POOL_CHUNK_S, &C->sl_used, W.sl_cap);
-                if (ss0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_SL); fail = 1; }
-            }
-            if (!fail) {
-                acc_m += tm; acc_p += tp; acc_s += ts; acc_x += tx;
-            }
-        }
-        fail = __shfl(fail, 0, WAVE);
-        ms0 = __shfl(ms0, 0, WAVE);
-        ps0 = __shfl(ps0, 0, WAVE);
-        ss0 = __shfl(ss0, 0, WAVE);
-        if (fail) {  // the host retries with larger pools; leave empty rows
-            nm_l = nx_l = np_l = ns_l = 0;
-        }
-        // segment starts of lane t's target: exclusive prefix over the
-        // targets (lanes 0..7); a row's M2L segment holds its same-level
-        // sources, then the others
-        uint32_t em = nm_l + nx_l, ep = np_l, es = ns_l;
+            __syncthreads();
+            // per target (lane t): totals, and this wavefront's start inside the
+            // target's segments per phase (counts of earlier (phase, wavefront))
+            uint32_t Tm = 0, Tx = 0, Tp = 0, Ts = 0;
+            uint32_t o0m = 0, o0x = 0, o0p = 0, o0s = 0, o1m = 0, o1x = 0, o1p = 0, o1s = 0;
+            if (lane < nt) {
 #pragma unroll
-        for (int o = 1; o < MAXT; o <<= 1) {
-            const uint32_t ym = __shfl_up(em, o, WAVE), yp = __shfl_up(ep, o, WAVE), ys = __shfl_up(es, o, WAVE);
-            if (lane >= o) { em += ym; ep += yp; es += ys; }
-        }
-        uint32_t cm_l = ms0 + em - (nm_l + nx_l), cp_l = ps0 + ep - np_l, cs_l = ss0 + es - ns_l;
-        uint32_t cx_l = cm_l + nm_l;
-        if (lane < nt) {
-            const uint32_t r = row0 + lane;
-            W.row_node[r] = A0 + lane;
-            W.row_next[r] = NO_ROW;
-            W.row_m_start[r] = cm_l;
-            W.row_m_cnt[r] = nm_l + nx_l;
-            W.row_mx_cnt[r] = nx_l;
-            W.row_p_start[r] = cp_l;
-            W.row_p_cnt[r] = np_l;
-            W.row_sl_start[r] = cs_l;
-            W.row_sl_cnt[r] = ns_l;
-            if (nch > 0) W.row_first[A0 + lane] = r;
-        }
-        if (nch == 0 && lane == 0) W.row_next[rg] = row0;
-        // pass 2: write every target's segments
-        if (!fail && (tm | tp | ts)) {
-            enumerate_sl(W, cache, sl_st, sl_n, filled, [&](bool valid, int B, int4 gb) {
+                for (int ph = 0; ph < 2; ++ph)
+#pragma unroll
+                    for (int w = 0; w < (int)WALK_WAVES; ++w) {
+                        const uint32_t m = cnt_s[w][ph][0][lane], x = cnt_s[w][ph][1][lane];
+                        const uint32_t pp = cnt_s[w][ph][2][lane], ss = cnt_s[w][ph][3][lane];
+                        if (ph == 0 && w == wv) { o0m = Tm; o0x = Tx; o0p = Tp; o0s = Ts; }
+                        if (ph == 1 && w == wv) { o1m = Tm; o1x = Tx; o1p = Tp; o1s = Ts; }
+                        Tm += m; Tx += x; Tp += pp; Ts += ss;
+                    }
+            }
+            const uint32_t tm = wave_sum(Tm + Tx), tp = wave_sum(Tp), ts = wave_sum(Ts), tx = wave_sum(Tx);
+            // reserve one block per pool for all targets of the group
+            if (threadIdx.x == 0) {
+                uint32_t ms0 = 0, ps0 = 0, ss0 = 0, fail = 0;
+                if (tm) {
+                    ms0 = pool_take(mchunk, tm, POOL_CHUNK_M, &C->m2l_pool_used, W.m2l_pool_cap);
+                    if (ms0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_M2L); fail = 1; }
+                }
+                if (tp) {
+                    ps0 = pool_take(pchunk, tp, POOL_CHUNK_P, &C->p2p_pool_used, W.p2p_pool_cap);
+                    if (ps0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_P2P); fail = 1; }
+                }
+                if (ts) {
+                    ss0 = pool_take(schunk, ts, POOL_CHUNK_S, &C->sl_used, W.sl_cap);
+                    if (ss0 == 0xffffffffu) { atomicOr(&C->overflow, (uint32_t)OVF_SL); fail = 1; }
+                }
+                if (!fail) { acc_m += tm; acc_p += tp; acc_s += ts; acc_x += tx; }
+                bcast[1] = ms0; bcast[2] = ps0; bcast[3] = ss0; bcast[4] = fail;
+            }
+            __syncthreads();
+            const uint32_t fail = bcast[4];
+            // segment starts of lane t's target: exclusive prefix over lanes
+            // 0..7; a row's M2L segment holds its same-level sources, then the others
+            uint32_t em = fail ? 0u : Tm + Tx, ep = fail ? 0u : Tp, es = fail ? 0u : Ts;
+            const uint32_t nm_l = em, np_l = ep, ns_l = es;
+#pragma unroll
+            for (int o = 1; o < MAXT; o <<= 1) {
+                const uint32_t ym = __shfl_up(em, o, WAVE), yp = __shfl_up(ep, o, WAVE), ys = __shfl_up(es, o, WAVE);
+                if (lane >= o) { em += ym; ep += yp; es += ys; }
+            }
+            const uint32_t m0 = bcast[1] + em - nm_l, p0 = bcast[2] + ep - np_l, s0 = bcast[3] + es - ns_l;
+            if (wv == 0 && lane < nt) {
+                const uint32_t r = row0 + lane;
+                W.row_node[r] = A0 + lane;
+                W.row_next[r] = NO_ROW;
+                W.row_m_start[r] = m0;
+                W.row_m_cnt[r] = nm_l;
+                W.row_mx_cnt[r] = fail ? 0u : Tx;
+                W.row_p_start[r] = p0;
+                W.row_p_cnt[r] = np_l;
+                W.row_sl_start[r] = s0;
+                W.row_sl_cnt[r] = ns_l;
+                if (nch > 0) W.row_first[A0 + lane] = r;
+            }
+            if (nch == 0 && threadIdx.x == 0) W.row_next[rg] = row0;
+            if (fail || !(tm | tp | ts)) continue;  // uniform; the host retries after a failure
+            // pass 2: write this wavefront's entries
+            uint32_t c0m = m0 + o0m, c0x = m0 + Tm + o0x, c0p = p0 + o0p, c0s = s0 + o0s;
+            uint32_t c1m = m0 + o1m, c1x = m0 + Tm + o1x, c1p = p0 + o1p, c1s = s0 + o1s;
+            each([&](int ph, bool valid, int B, int4 gb) {
                 classify_all(valid, B, gb, [&](int t, int cat) {
                     const uint64_t bm = __ballot(cat == 0), bp = __ballot(cat == 1), bs = __ballot(cat == 2);
                     const uint64_t bx = __ballot(cat == 4);
-                    if (cat == 0) W.m2l_pool[__builtin_amdgcn_readlane(cm_l, t) + __popcll(bm & lt)] = B;
-                    if (cat == 4) W.m2l_pool[__builtin_amdgcn_readlane(cx_l, t) + __popcll(bx & lt)] = B;
-                    if (cat == 1) W.p2p_pool[__builtin_amdgcn_readlane(cp_l, t) + __popcll(bp & lt)] = B;
-                    if (cat == 2) W.sl[__builtin_amdgcn_readlane(cs_l, t) + __popcll(bs & lt)] = B;
+                    const uint32_t cm = __builtin_amdgcn_readlane(ph == 0 ? c0m : c1m, t);
+                    const uint32_t cx = __builtin_amdgcn_readlane(ph == 0 ? c0x : c1x, t);
+                    const uint32_t cp = __builtin_amdgcn_readlane(ph == 0 ? c0p : c1p, t);
+                    const uint32_t cs = __builtin_amdgcn_readlane(ph == 0 ? c0s : c1s, t);
+                    if (cat == 0) W.m2l_pool[cm + __popcll(bm & lt)] = B;
+                    if (cat == 4) W.m2l_pool[cx + __popcll(bx & lt)] = B;
+                    if (cat == 1) W.p2p_pool[cp + __popcll(bp & lt)] = B;
+                    if (cat == 2) W.sl[cs + __popcll(bs & lt)] = B;
                     if (lane == t) {
-                        cm_l += __popcll(bm);
-                        cx_l += __popcll(bx);
-                        cp_l += __popcll(bp);
-                        cs_l += __popcll(bs);
+                        if (ph == 0) {
+                            c0m += __popcll(bm); c0x += __popcll(bx); c0p += __popcll(bp); c0s += __popcll(bs);
+                        } else {
+                            c1m += __popcll(bm); c1x += __popcll(bx); c1p += __popcll(bp); c1s += __popcll(bs);
+                        }
                     }
                 });
             });
         }
-        wave_lds_sync();  // the cache is refilled by the next group
     }
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         if (acc_m) atomicAdd(&C->m2l_total, acc_m);
         if (acc_p) atomicAdd(&C->p2p_total, acc_p);
         if (acc_s) atomicAdd(&C->split_total, acc_s);
