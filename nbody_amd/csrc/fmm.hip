@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t
 // leaf's local expansion, written to (ax, ay, az, pot) for the P2P kernel to
 // complete.
 template <int P>
-__global__ __launch_bounds__(256) void l2p_kernel(FmmParams F, State S, uint32_t n) {
+__global__ __launch_bounds__(256) void l2p_kernel(FmmParams F, State S, uint32_t n, double kick_dt) {
     constexpr int NC = ncoef(P);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -85,10 +85,19 @@ __global__ __launch_bounds__(256) void l2p_kernel(FmmParams F, State S, uint32_t
     for (int k = 0; k < NC; ++k) L[k] = Ls[k];
     float phi, gx, gy, gz;
     l2p_eval<P>(me.x, me.y, me.z, L, phi, gx, gy, gz);
-    S.ax[i] = gx;
-    S.ay[i] = gy;
-    S.az[i] = gz;
-    S.pot[i] = phi;
+    // the P2P kernel left the raw near-field sums in (ax, ay, az, pot)
+    const float nx = S.ax[i], ny = S.ay[i], nz = S.az[i], nph = S.pot[i];
+    const float fac = (float)(S.charge[i] / S.mass[i]) * F.G;
+    const float axo = fac * (nx + gx), ayo = fac * (ny + gy), azo = fac * (nz + gz);
+    S.ax[i] = axo;
+    S.ay[i] = ayo;
+    S.az[i] = azo;
+    S.pot[i] = -F.G * (nph + phi);
+    if (kick_dt != 0.0) {
+        S.vx[i] += (double)axo * kick_dt;
+        S.vy[i] += (double)ayo * kick_dt;
+        S.vz[i] += (double)azo * kick_dt;
+    }
 }
 
 // M2M (replaces moment.cl:72-160, which sums untranslated moments): one
@@ -183,7 +192,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 // handles target l % nt and every (64/nt)-th particle of each source leaf, so
 // all 64 lanes work for any leaf occupancy; partial sums meet in LDS.
 template <int P>
-__global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t num_leaves, double kick_dt) {
+__global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t num_leaves) {
     constexpr int NC = ncoef(P);
     const int lane = lane_id();
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
@@ -255,20 +264,11 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
             }
         }
         if (sp == 0) {
-            // far field from l2p_kernel
-            const float lphi = S.pot[i], gx = S.ax[i], gy = S.ay[i], gz = S.az[i];
-            const double qm = S.charge[i] / S.mass[i];
-            const float fac = (float)qm * F.G;
-            const float axo = fac * (ax + gx), ayo = fac * (ay + gy), azo = fac * (az + gz);
-            S.ax[i] = axo;
-            S.ay[i] = ayo;
-            S.az[i] = azo;
-            S.pot[i] = -F.G * (ph + lphi);
-            if (kick_dt != 0.0) {
-                S.vx[i] += (double)axo * kick_dt;
-                S.vy[i] += (double)ayo * kick_dt;
-                S.vz[i] += (double)azo * kick_dt;
-            }
+            // raw near-field sums; l2p_kernel adds the far field
+            S.ax[i] = ax;
+            S.ay[i] = ay;
+            S.az[i] = az;
+            S.pot[i] = ph;
         }
     }
     if (lane == 0 && inter) atomicAdd(F.p2p_inter, inter);
@@ -287,8 +287,7 @@ constexpr int P2P_WAVES = 4;
 
 
 template <int P>
-__global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves,
-                                                                     double kick_dt) {
+__global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves) {
     constexpr int NC = ncoef(P);
     __shared__ float4 stage_s[P2P_WAVES][WAVE];
     __shared__ int stage_i[P2P_WAVES][WAVE];
@@ -405,19 +404,11 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
                 }
             }
             if (sp == 0) {
-                // far field from l2p_kernel
-                const float lphi = S.pot[i], gx = S.ax[i], gy = S.ay[i], gz = S.az[i];
-                const float fac = (float)(S.charge[i] / S.mass[i]) * F.G;
-                const float axo = fac * (ax + gx), ayo = fac * (ay + gy), azo = fac * (az + gz);
-                S.ax[i] = axo;
-                S.ay[i] = ayo;
-                S.az[i] = azo;
-                S.pot[i] = -F.G * (ph + lphi);
-                if (kick_dt != 0.0) {
-                    S.vx[i] += (double)axo * kick_dt;
-                    S.vy[i] += (double)ayo * kick_dt;
-                    S.vz[i] += (double)azo * kick_dt;
-                }
+                // raw near-field sums; l2p_kernel adds the far field
+                S.ax[i] = ax;
+                S.ay[i] = ay;
+                S.az[i] = az;
+                S.pot[i] = ph;
             }
         }
     }
@@ -600,17 +591,16 @@ __global__ __launch_bounds__(256) void m2l_cross_kernel(FmmParams F, uint32_t nu
     const int lane = lane_id();
     const float4* X4 = reinterpret_cast<const float4*>(F.xtab);
     for (uint32_t A = wid; A < num_nodes; A += nw) {
-        uint32_t any = 0;
-        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) any |= F.lists.mx_cnt[r];
-        if (!any) continue;
         const int4 ga = F.nodes.geo[A];
         float L[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = 0.f;
-        uint32_t nfar = 0;
+        uint32_t nfar = 0, any = 0;
         for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
+            const uint32_t nx = F.lists.mx_cnt[r];
+            any |= nx;
             const uint32_t e1 = F.lists.m_start[r] + F.lists.m_cnt[r];
-            for (uint32_t e = e1 - F.lists.mx_cnt[r] + lane; e < e1; e += WAVE) {
+            for (uint32_t e = e1 - nx + lane; e < e1; e += WAVE) {
                 const int B = F.lists.m_pool[e];
                 uint32_t neg = 0;
                 const int t = xtab_index(F, ga, F.nodes.geo[B], neg);
@@ -623,6 +613,7 @@ __global__ __launch_bounds__(256) void m2l_cross_kernel(FmmParams F, uint32_t nu
                 m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
             }
         }
+        if (!any) continue;  // uniform
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
         if (lane == 0) {
@@ -785,8 +776,8 @@ static void launch_p2m_t(const FmmParams& F, const State& S, uint32_t num_leaves
     hipLaunchKernelGGL((p2m_kernel<P>), dim3(ceil_div(num_leaves, 256)), dim3(256), 0, st, F, S, num_leaves);
 }
 template <int P>
-static void launch_l2p_t(const FmmParams& F, const State& S, uint32_t n, hipStream_t st) {
-    hipLaunchKernelGGL((l2p_kernel<P>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, S, n);
+static void launch_l2p_t(const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st) {
+    hipLaunchKernelGGL((l2p_kernel<P>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, S, n, kick_dt);
 }
 template <int P>
 static void launch_m2m_t(const FmmParams& F, uint32_t count, uint32_t off, hipStream_t st) {
@@ -797,15 +788,14 @@ static void launch_l2l_t(const FmmParams& F, uint32_t count, uint32_t off, hipSt
     hipLaunchKernelGGL((l2l_kernel<P>), dim3(ceil_div(count, 64)), dim3(64), 0, st, F, count, off);
 }
 template <int P>
-static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st) {
-    hipLaunchKernelGGL((p2p_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves, kick_dt);
+static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
+    hipLaunchKernelGGL((p2p_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves);
 }
 
 template <int P>
-static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt,
-                                hipStream_t st) {
+static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     const unsigned grid = std::min<unsigned>(ceil_div(num_leaves, P2P_WAVES), 65536u);
-    hipLaunchKernelGGL((p2p_staged_kernel<P>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S, num_leaves, kick_dt);
+    hipLaunchKernelGGL((p2p_staged_kernel<P>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S, num_leaves);
 }
 #define NBH_DISPATCH_P(p, call)          \
     switch (p) {                         \
@@ -822,9 +812,9 @@ void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, 
     if (!num_leaves) return;
     NBH_DISPATCH_P(p, launch_p2m_t<PP>(F, S, num_leaves, st));
 }
-void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, hipStream_t st) {
+void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st) {
     if (!n) return;
-    NBH_DISPATCH_P(p, launch_l2p_t<PP>(F, S, n, st));
+    NBH_DISPATCH_P(p, launch_l2p_t<PP>(F, S, n, kick_dt, st));
 }
 void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st) {
     (void)level;
@@ -836,10 +826,9 @@ void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint
     if (!count) return;
     NBH_DISPATCH_P(p, launch_l2l_t<PP>(F, count, off, st));
 }
-void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt,
-                       hipStream_t st) {
+void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     if (!num_leaves) return;
-    NBH_DISPATCH_P(p, launch_p2p_staged_t<PP>(F, S, num_leaves, kick_dt, st));
+    NBH_DISPATCH_P(p, launch_p2p_staged_t<PP>(F, S, num_leaves, st));
 }
 void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
                    hipStream_t st) {
@@ -869,9 +858,9 @@ void launch_m2l(int p, int q, const FmmParams& F, const M2LPlan& plan, hipStream
         default: break;
     }
 }
-void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st) {
+void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     if (!num_leaves) return;
-    NBH_DISPATCH_P(p, launch_p2p_t<PP>(F, S, num_leaves, kick_dt, st));
+    NBH_DISPATCH_P(p, launch_p2p_t<PP>(F, S, num_leaves, st));
 }
 
 // ---------------------------------------------------------------------------

@@ -133,11 +133,12 @@ void launch_gather_state(const State& src, const State& dst, const uint32_t* per
 void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
 void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
 void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
-// far field of every particle into (ax, ay, az, pot); the P2P kernels add the near field
-void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, hipStream_t st);
-void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt, hipStream_t st);
-void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, double kick_dt,
-                       hipStream_t st);
+// the P2P kernels leave the raw near-field sums of every particle in (ax, ay,
+// az, pot); launch_l2p adds the far field of the leaf's local expansion,
+// applies G q/m and the sign of the potential, and the kick v += a kick_dt
+void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
+void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
+void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st);
 // M2L of order (p, q): q = p keeps |k| + |n| <= p, q = p + 1 also the cross
 // terms of degree p + 1 (the expansions themselves stay of order p).
 struct M2LPlan {

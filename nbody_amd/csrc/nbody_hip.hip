@@ -107,7 +107,12 @@ struct nbh_handle {
     uint32_t node_cap = 0, row_cap = 0, sl_cap = 0;
     uint32_t num_nodes = 0, num_leaves = 0, depth = 0;
     nbh_stats stats{};
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[8] = {};           // phase timing on the main stream
+    hipEvent_t evb[4] = {};          // phase timing on the side stream
+    // The side stream runs the moments and tables while the walk runs, then
+    // the near field while M2L and L2L run; these order the two streams.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_side = nullptr, ev_walk = nullptr, ev_near = nullptr;
 
     int fail(int code, const char* fmt, ...) {
         char buf[512];
@@ -232,6 +237,9 @@ static NodeView node_view(nbh_handle* h) {
 
 static void record(nbh_handle* h, int k) {
     if (h->timing) (void)hipEventRecord(h->ev[k], h->stream);
+}
+static void record_side(nbh_handle* h, int k) {
+    if (h->timing) (void)hipEventRecord(h->evb[k], h->stream2);
 }
 
 // bbox -> keys -> sort -> gather: the state ends in Morton order
@@ -439,15 +447,18 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     record(h, 1);
     FmmParams F = fmm_params(h);
     State S = h->st[h->cur].view();
-    launch_p2m(p, F, S, h->num_leaves, h->stream);
+    // side stream, overlapping the walk: moments (P2M, M2M) and the tables
+    HIPCHK(h, hipEventRecord(h->ev_fork, h->stream));
+    HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+    record_side(h, 0);
+    launch_p2m(p, F, S, h->num_leaves, h->stream2);
     for (int l = (int)h->depth - 1; l >= 0; --l)
-        launch_m2m_level(p, F, l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
-    record(h, 2);
-    CHK(run_walk(h));
-    record(h, 3);
-    F = fmm_params(h);
+        launch_m2m_level(p, F, l, h->hc->level_count[l], h->hc->level_off[l], h->stream2);
+    record_side(h, 1);
     const int q = m2l_q(h);
-    if (h->cfg.m2l != NBH_M2L_PLAIN && h->nsys == 1 && h->depth >= 2) {
+    const bool tables = h->cfg.m2l != NBH_M2L_PLAIN && h->nsys == 1 && h->depth >= 2;
+    FmmParams T{};  // the table part of the parameters
+    if (tables) {
         // derivative tensors of the same-level offsets [0, K]^3 of levels
         // 2..depth (cells of levels 0 and 1 are never well separated); one
         // system only, since the tensors depend on the root size
@@ -455,50 +466,77 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         const int K = 2 * Kp + 1;
         const int dsz = (K + 1) * (K + 1) * (K + 1), dcp = ncp_of(q);
         const int nlev = (int)h->depth - 1;
-        {
-            HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4));
-            F.dtab = h->dtab.as<float>();
-            F.dK = K;
-            F.dlevel0 = 2;
-            F.dnlev = nlev;
-            F.dcp = dcp;
-            F.dsz = dsz;
-            launch_dtable(q, F, F.dtab, 2, nlev, K + 1, 0, 1, 0, h->stream);
-            // pairs one level apart, by finer level 1..depth: offsets |d_i| <=
-            // 4 Kp + 1 half finer cells (the coarser side's parent pair was
-            // split), tabled at d_i = 1, 3, .., 4 Kp + 1
-            const int E = 2 * Kp + 1;
-            const int xn = (int)h->depth;
-            HIPCHK(h, h->xtab.ensure((size_t)E * E * E * xn * dcp * 4));
-            F.xtab = h->xtab.as<float>();
-            F.xE = E;
-            F.xlevel0 = 1;
-            F.xnlev = xn;
-            launch_dtable(q, F, F.xtab, 1, xn, E, 1, 2, 1, h->stream);
-            // deferred pairs: at most every other-level entry, one block per target
-            const unsigned long long nx = std::max<unsigned long long>(h->hc->m2lx_total, 1);
-            HIPCHK(h, h->far_src.ensure((size_t)nx * 4));
-            HIPCHK(h, h->far_tgt.ensure((size_t)nx * 4));
-            F.far_src = h->far_src.as<int>();
-            F.far_tgt = h->far_tgt.as<int>();
-            F.far_cap = (uint32_t)std::min<unsigned long long>(nx, 0xffffffffull);
-            F.far_cnt = &h->counters.as<Counters>()->far_blocks;
-        }
+        HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4));
+        T = F;
+        T.dtab = h->dtab.as<float>();
+        T.dK = K;
+        T.dlevel0 = 2;
+        T.dnlev = nlev;
+        T.dcp = dcp;
+        T.dsz = dsz;
+        launch_dtable(q, T, T.dtab, 2, nlev, K + 1, 0, 1, 0, h->stream2);
+        // pairs one level apart, by finer level 1..depth: offsets |d_i| <=
+        // 4 Kp + 1 half finer cells (the coarser side's parent pair was
+        // split), tabled at d_i = 1, 3, .., 4 Kp + 1
+        const int E = 2 * Kp + 1;
+        const int xn = (int)h->depth;
+        HIPCHK(h, h->xtab.ensure((size_t)E * E * E * xn * dcp * 4));
+        T.xtab = h->xtab.as<float>();
+        T.xE = E;
+        T.xlevel0 = 1;
+        T.xnlev = xn;
+        launch_dtable(q, T, T.xtab, 1, xn, E, 1, 2, 1, h->stream2);
     }
+    HIPCHK(h, hipEventRecord(h->ev_side, h->stream2));
+    CHK(run_walk(h));
+    record(h, 2);
+    F = fmm_params(h);
+    if (tables) {
+        F.dtab = T.dtab;
+        F.dK = T.dK;
+        F.dlevel0 = T.dlevel0;
+        F.dnlev = T.dnlev;
+        F.dcp = T.dcp;
+        F.dsz = T.dsz;
+        F.xtab = T.xtab;
+        F.xE = T.xE;
+        F.xlevel0 = T.xlevel0;
+        F.xnlev = T.xnlev;
+        // deferred pairs: at most every other-level entry
+        const unsigned long long nx = std::max<unsigned long long>(h->hc->m2lx_total, 1);
+        HIPCHK(h, h->far_src.ensure((size_t)nx * 4));
+        HIPCHK(h, h->far_tgt.ensure((size_t)nx * 4));
+        F.far_src = h->far_src.as<int>();
+        F.far_tgt = h->far_tgt.as<int>();
+        F.far_cap = (uint32_t)std::min<unsigned long long>(nx, 0xffffffffull);
+        F.far_cnt = &h->counters.as<Counters>()->far_blocks;
+    }
+    // side stream, overlapping M2L and L2L: the near field (raw sums)
+    if (!for_vjp) {
+        HIPCHK(h, hipEventRecord(h->ev_walk, h->stream));
+        HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_walk, 0));
+        record_side(h, 2);
+        if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_staged(p, F, S, h->num_leaves, h->stream2);
+        else launch_p2p(p, F, S, h->num_leaves, h->stream2);
+        record_side(h, 3);
+        HIPCHK(h, hipEventRecord(h->ev_near, h->stream2));
+    }
+    HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_side, 0));
     launch_m2l(p, q, F, m2l_plan(h), h->stream);
-    record(h, 4);
+    record(h, 3);
     for (uint32_t l = 1; l <= h->depth; ++l)
         launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
-    // far field per particle (timed with L2L); the P2P kernel completes it
-    if (!for_vjp) launch_l2p(p, F, S, (uint32_t)h->n, h->stream);
-    record(h, 5);
+    record(h, 4);
     h->lastF = F;
     if (for_vjp) {
         HIPCHK(h, hipGetLastError());
         return NBH_OK;
     }
-    if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_staged(p, F, S, h->num_leaves, kick_dt, h->stream);
-    else launch_p2p(p, F, S, h->num_leaves, kick_dt, h->stream);
+    // join: far field of every particle added to the near field, forces,
+    // closing half kick
+    HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_near, 0));
+    record(h, 5);
+    launch_l2p(p, F, S, (uint32_t)h->n, kick_dt, h->stream);
     record(h, 6);
     HIPCHK(h, hipGetLastError());
     // stats (read at the end of the evaluation; one more sync)
@@ -512,14 +550,17 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     h->stats.p2p_interactions = h->hc->p2p_inter;
     h->stats.split_pairs = h->hc->split_total;
     if (h->timing) {
-        float ms[6];
+        // phases on the side stream overlap the others: moments run beside
+        // the walk, P2P beside M2L and L2L; the total is the main stream's
+        float ms[6], mb[3];
         for (int k = 0; k < 6; ++k) (void)hipEventElapsedTime(&ms[k], h->ev[k], h->ev[k + 1]);
+        for (int k = 0; k < 3; ++k) (void)hipEventElapsedTime(&mb[k], h->evb[k], h->evb[k + 1]);
         h->stats.ms_tree = ms[0];
-        h->stats.ms_moments = ms[1];
-        h->stats.ms_walk = ms[2];
-        h->stats.ms_m2l = ms[3];
-        h->stats.ms_l2l = ms[4];
-        h->stats.ms_p2p = ms[5];
+        h->stats.ms_moments = mb[0];
+        h->stats.ms_walk = ms[1];
+        h->stats.ms_m2l = ms[2];
+        h->stats.ms_l2l = ms[3] + ms[5];  // L2L and the far-field join
+        h->stats.ms_p2p = mb[2];
         float tot = 0;
         (void)hipEventElapsedTime(&tot, h->ev[0], h->ev[6]);
         h->stats.ms_total = tot;
@@ -670,7 +711,11 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
         h->walk_grid = walk_group_resident_blocks(ncu);
     }
     CCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    CCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
     for (auto& e : h->ev) CCHK(hipEventCreate(&e));
+    for (auto& e : h->evb) CCHK(hipEventCreate(&e));
+    for (hipEvent_t* e : {&h->ev_fork, &h->ev_side, &h->ev_walk, &h->ev_near})
+        CCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     CCHK(h->st[0].alloc(n));
     CCHK(h->st[1].alloc(n));
     CCHK(h->sys_off.ensure((nsys + 1) * 4));
@@ -759,6 +804,11 @@ void nbh_destroy(nbh_handle* h) {
     if (h->hc) (void)hipHostFree(h->hc);
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : h->evb)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {h->ev_fork, h->ev_side, h->ev_walk, h->ev_near})
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
