@@ -471,13 +471,20 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
     const float4* tab = IN_LDS ? dtab_lds : src;
     const int lane = lane_id();
     const uint32_t wpb = blockDim.x / WAVE;
+    // launched beside the walk: after an overflow at this depth or above the
+    // level's rows are incomplete, and the host repeats the walk and M2L
+    if (F.C->overflow & (OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P)) return;
     for (uint32_t t = blockIdx.x * wpb + threadIdx.x / WAVE; t < count; t += gridDim.x * wpb) {
         const uint32_t A = off + t;
         const int4 ga = F.nodes.geo[A];
         float L[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = 0.f;
-        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
+        // every same-level entry of a level-l target is emitted at depth l,
+        // in its first row; the rows behind it (a leaf carried deeper) hold
+        // other levels only and may still be being written by the walk
+        const uint32_t r = F.lists.first[A];
+        {
             const uint32_t s0 = F.lists.m_start[r];
             const uint32_t e1 = s0 + F.lists.m_cnt[r] - F.lists.mx_cnt[r];
             for (uint32_t e = s0 + lane; e < e1; e += WAVE) {
@@ -720,30 +727,37 @@ __global__ void zero_f32_kernel(float* p, size_t n) {
 }
 
 template <int P, int Q>
-static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, hipStream_t st) {
+// parts: M2L_LEVELS runs the same-level kernel of level `level` (all tabled
+// levels when level < 0); M2L_REST runs everything else (initialisation, the
+// other-level and deferred pairs), after the same-level kernels.
+static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, int level, hipStream_t st) {
     const uint32_t nn = plan.num_nodes;
     const unsigned grid = std::min<unsigned>(ceil_div(nn, 4), 65536u);
     if (plan.mode == 0) {
-        hipLaunchKernelGGL((m2l_plain_kernel<P, Q>), dim3(ceil_div(nn, 256)), dim3(256), 0, st, F, nn);
+        if (parts & M2L_REST)
+            hipLaunchKernelGGL((m2l_plain_kernel<P, Q>), dim3(ceil_div(nn, 256)), dim3(256), 0, st, F, nn);
         return;
     }
     if (!F.dtab) {
-        hipLaunchKernelGGL((m2l_pairs_kernel<P, Q>), dim3(grid), dim3(256), 0, st, F, nn);
+        if (parts & M2L_REST) hipLaunchKernelGGL((m2l_pairs_kernel<P, Q>), dim3(grid), dim3(256), 0, st, F, nn);
         return;
     }
-    hipLaunchKernelGGL(zero_u32x2_kernel, dim3(1), dim3(64), 0, st, F.far_cnt);
-    // cells above the tabled levels have no same-level pairs: L starts at 0
-    const uint32_t top = plan.level_off[F.dlevel0];
-    if (top) {
-        const size_t nz = (size_t)top * F.ncp;
-        hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)ceil_div(nz, (size_t)256)), dim3(256), 0, st, F.L, nz);
+    if (parts & M2L_REST) {
+        hipLaunchKernelGGL(zero_u32x2_kernel, dim3(1), dim3(64), 0, st, F.far_cnt);
+        // cells above the tabled levels have no same-level pairs: L starts at 0
+        const uint32_t top = plan.level_off[F.dlevel0];
+        if (top) {
+            const size_t nz = (size_t)top * F.ncp;
+            hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)ceil_div(nz, (size_t)256)), dim3(256), 0, st, F.L, nz);
+        }
     }
     const size_t lds = (size_t)F.dsz * F.dcp * 4;
     const bool in_lds = lds <= plan.max_lds && m2l_table_threads(P, Q) == 1024;
-    if (in_lds)
+    if (in_lds && (parts & M2L_LEVELS))
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     for (int l = F.dlevel0; l < F.dlevel0 + F.dnlev; ++l) {
+        if (!(parts & M2L_LEVELS) || (level >= 0 && l != level)) continue;
         const uint32_t cnt = plan.level_count[l];
         if (!cnt) continue;
         constexpr int T = m2l_table_threads(P, Q);
@@ -757,6 +771,7 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, hipStream_t s
             hipLaunchKernelGGL((m2l_lds_kernel<P, Q, false>), dim3(g), dim3(T), 0, st, F, plan.level_off[l], cnt,
                                l - F.dlevel0);
     }
+    if (!(parts & M2L_REST)) return;
     hipLaunchKernelGGL((m2l_cross_kernel<P, Q>), dim3(grid), dim3(256), 0, st, F, nn);
     // the deferred pairs are a subset of the other-level entries (far_cap)
     const unsigned gfar = std::max<unsigned>(1u, std::min<unsigned>(ceil_div(F.far_cap, 256u), 1024u));
@@ -845,18 +860,23 @@ void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, 
     }
 #undef NBH_DT
 }
-void launch_m2l(int p, int q, const FmmParams& F, const M2LPlan& plan, hipStream_t st) {
+void launch_m2l_parts(int p, int q, const FmmParams& F, const M2LPlan& plan, int parts, int level, hipStream_t st) {
     if (!plan.num_nodes) return;
     const bool ext = q == p + 1;
+#define NBH_M2L(PP, QQ) launch_m2l_pq<PP, QQ>(F, plan, parts, level, st)
     switch (p) {
-        case 1: ext ? launch_m2l_pq<1, 2>(F, plan, st) : launch_m2l_pq<1, 1>(F, plan, st); break;
-        case 2: ext ? launch_m2l_pq<2, 3>(F, plan, st) : launch_m2l_pq<2, 2>(F, plan, st); break;
-        case 3: ext ? launch_m2l_pq<3, 4>(F, plan, st) : launch_m2l_pq<3, 3>(F, plan, st); break;
-        case 4: ext ? launch_m2l_pq<4, 5>(F, plan, st) : launch_m2l_pq<4, 4>(F, plan, st); break;
-        case 5: ext ? launch_m2l_pq<5, 6>(F, plan, st) : launch_m2l_pq<5, 5>(F, plan, st); break;
-        case 6: ext ? launch_m2l_pq<6, 7>(F, plan, st) : launch_m2l_pq<6, 6>(F, plan, st); break;
+        case 1: ext ? NBH_M2L(1, 2) : NBH_M2L(1, 1); break;
+        case 2: ext ? NBH_M2L(2, 3) : NBH_M2L(2, 2); break;
+        case 3: ext ? NBH_M2L(3, 4) : NBH_M2L(3, 3); break;
+        case 4: ext ? NBH_M2L(4, 5) : NBH_M2L(4, 4); break;
+        case 5: ext ? NBH_M2L(5, 6) : NBH_M2L(5, 5); break;
+        case 6: ext ? NBH_M2L(6, 7) : NBH_M2L(6, 6); break;
         default: break;
     }
+#undef NBH_M2L
+}
+void launch_m2l(int p, int q, const FmmParams& F, const M2LPlan& plan, hipStream_t st) {
+    launch_m2l_parts(p, q, F, plan, M2L_LEVELS | M2L_REST, -1, st);
 }
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     if (!num_leaves) return;

@@ -153,6 +153,11 @@ struct M2LPlan {
 void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
                    hipStream_t st);
 void launch_m2l(int p, int q, const FmmParams& F, const M2LPlan& plan, hipStream_t st);
+// The grouped M2L in parts (when F carries the tables): the same-level kernel
+// of one level (level < 0: every level), and the rest (initialisation, the
+// other-level and deferred pairs), which must follow all same-level kernels.
+enum : int { M2L_LEVELS = 1, M2L_REST = 2 };
+void launch_m2l_parts(int p, int q, const FmmParams& F, const M2LPlan& plan, int parts, int level, hipStream_t st);
 
 // integrator updates: mode 0 = KDK first half (kick dt_k, drift dt_d),
 // 1 = Euler (x += v dt; v += a dt), 2 = kick-drift (v += a dt; x += v dt)

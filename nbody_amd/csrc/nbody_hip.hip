@@ -113,6 +113,12 @@ struct nbh_handle {
     // the near field while M2L and L2L run; these order the two streams.
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_side = nullptr, ev_walk = nullptr, ev_near = nullptr;
+    // A third stream runs each level's same-level M2L as soon as the walk
+    // depth that emits it is done (ev_depth), while the walk goes deeper.
+    hipStream_t stream3 = nullptr;
+    hipEvent_t ev_depth[MAX_LEVELS + 1] = {};
+    hipEvent_t ev_same = nullptr;  // same-level M2L done
+    hipEvent_t evc0 = nullptr;     // timing: first same-level launch
 
     int fail(int code, const char* fmt, ...) {
         char buf[512];
@@ -327,7 +333,32 @@ static int grow_rows(nbh_handle* h, uint32_t cap) {
     return NBH_OK;
 }
 
-static int run_walk(nbh_handle* h) {
+static M2LPlan m2l_plan(const nbh_handle* h);
+static FmmParams fmm_params(nbh_handle* h);
+
+// Same-level M2L launched level by level during the walk: every same-level
+// entry of a level-l target is emitted at depth l (carried rows of a leaf
+// only meet finer candidates), so level l's kernel can start after depth l.
+struct EarlyM2L {
+    bool on = false;
+    int p = 0, q = 0;
+    FmmParams T{};  // the table fields
+};
+
+static void copy_tables(FmmParams& F, const FmmParams& T) {
+    F.dtab = T.dtab;
+    F.dK = T.dK;
+    F.dlevel0 = T.dlevel0;
+    F.dnlev = T.dnlev;
+    F.dcp = T.dcp;
+    F.dsz = T.dsz;
+    F.xtab = T.xtab;
+    F.xE = T.xE;
+    F.xlevel0 = T.xlevel0;
+    F.xnlev = T.xnlev;
+}
+
+static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
     Counters* dC = h->counters.as<Counters>();
     const uint32_t nn = h->num_nodes;
     if (h->row_cap < 2 * nn + 1024) CHK(grow_rows(h, 2 * nn + 1024));
@@ -382,10 +413,31 @@ static int run_walk(nbh_handle* h) {
         W.p2p_pool = h->p2p_pool.as<int>();
         W.m2l_pool_cap = h->m2l_pool_cap;
         W.p2p_pool_cap = h->p2p_pool_cap;
+        const bool early = em && em->on;
+        FmmParams Fe{};
+        M2LPlan plan{};
+        bool started = false;
+        if (early) {
+            // same-level kernels of an earlier attempt read the rows this one rewrites
+            if (attempt > 0) HIPCHK(h, hipStreamSynchronize(h->stream3));
+            Fe = fmm_params(h);  // this attempt's rows and pools
+            copy_tables(Fe, em->T);
+            plan = m2l_plan(h);
+            HIPCHK(h, hipStreamWaitEvent(h->stream3, h->ev_side, 0));  // moments and tables
+        }
         // reset the walk part of the counters
         zero_u32(&dC->carried[0], (offsetof(Counters, nonfinite) - offsetof(Counters, carried)) / 4, h->stream);
         launch_walk_root(W, h->stream);
-        for (uint32_t s = 1; s <= h->depth; ++s) launch_walk_group(W, (int)s, grid[s], h->stream);
+        for (uint32_t s = 1; s <= h->depth; ++s) {
+            launch_walk_group(W, (int)s, grid[s], h->stream);
+            if (early && (int)s >= Fe.dlevel0 && (int)s < Fe.dlevel0 + Fe.dnlev && h->hc->level_count[s]) {
+                HIPCHK(h, hipEventRecord(h->ev_depth[s], h->stream));
+                HIPCHK(h, hipStreamWaitEvent(h->stream3, h->ev_depth[s], 0));
+                if (!started && h->timing) (void)hipEventRecord(h->evc0, h->stream3);
+                started = true;
+                launch_m2l_parts(em->p, em->q, Fe, plan, M2L_LEVELS, (int)s, h->stream3);
+            }
+        }
         CHK(read_counters(h));
         const uint32_t ovf = h->hc->overflow;
         const uint32_t walk_bits = OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P;
@@ -488,20 +540,16 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         launch_dtable(q, T, T.xtab, 1, xn, E, 1, 2, 1, h->stream2);
     }
     HIPCHK(h, hipEventRecord(h->ev_side, h->stream2));
-    CHK(run_walk(h));
+    EarlyM2L em;
+    em.on = tables;
+    em.p = p;
+    em.q = q;
+    em.T = T;
+    CHK(run_walk(h, &em));
     record(h, 2);
     F = fmm_params(h);
     if (tables) {
-        F.dtab = T.dtab;
-        F.dK = T.dK;
-        F.dlevel0 = T.dlevel0;
-        F.dnlev = T.dnlev;
-        F.dcp = T.dcp;
-        F.dsz = T.dsz;
-        F.xtab = T.xtab;
-        F.xE = T.xE;
-        F.xlevel0 = T.xlevel0;
-        F.xnlev = T.xnlev;
+        copy_tables(F, T);
         // deferred pairs: at most every other-level entry
         const unsigned long long nx = std::max<unsigned long long>(h->hc->m2lx_total, 1);
         HIPCHK(h, h->far_src.ensure((size_t)nx * 4));
@@ -522,7 +570,14 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         HIPCHK(h, hipEventRecord(h->ev_near, h->stream2));
     }
     HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_side, 0));
-    launch_m2l(p, q, F, m2l_plan(h), h->stream);
+    if (em.on) {
+        // the same-level kernels ran beside the walk; the rest follows them
+        HIPCHK(h, hipEventRecord(h->ev_same, h->stream3));
+        HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_same, 0));
+        launch_m2l_parts(p, q, F, m2l_plan(h), M2L_REST, -1, h->stream);
+    } else {
+        launch_m2l(p, q, F, m2l_plan(h), h->stream);
+    }
     record(h, 3);
     for (uint32_t l = 1; l <= h->depth; ++l)
         launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
@@ -559,6 +614,10 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         h->stats.ms_moments = mb[0];
         h->stats.ms_walk = ms[1];
         h->stats.ms_m2l = ms[2];
+        if (tables) {  // from the first same-level launch, beside the walk
+            float m = 0;
+            if (hipEventElapsedTime(&m, h->evc0, h->ev[3]) == hipSuccess) h->stats.ms_m2l = m;
+        }
         h->stats.ms_l2l = ms[3] + ms[5];  // L2L and the far-field join
         h->stats.ms_p2p = mb[2];
         float tot = 0;
@@ -712,6 +771,10 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
     }
     CCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     CCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    CCHK(hipStreamCreateWithFlags(&h->stream3, hipStreamNonBlocking));
+    CCHK(hipEventCreate(&h->evc0));
+    CCHK(hipEventCreateWithFlags(&h->ev_same, hipEventDisableTiming));
+    for (auto& e : h->ev_depth) CCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : h->ev) CCHK(hipEventCreate(&e));
     for (auto& e : h->evb) CCHK(hipEventCreate(&e));
     for (hipEvent_t* e : {&h->ev_fork, &h->ev_side, &h->ev_walk, &h->ev_near})
@@ -806,8 +869,11 @@ void nbh_destroy(nbh_handle* h) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : h->evb)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {h->ev_fork, h->ev_side, h->ev_walk, h->ev_near})
+    for (hipEvent_t e : {h->ev_fork, h->ev_side, h->ev_walk, h->ev_near, h->ev_same, h->evc0})
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : h->ev_depth)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream3) (void)hipStreamDestroy(h->stream3);
     if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
