@@ -179,6 +179,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     float* sc = F.L + (size_t)A * F.ncp;
 #pragma unroll
     for (int k = 0; k < NC; ++k) { Lp[k] = sp[k]; Lc[k] = sc[k]; }
+    if (F.Lx) {  // the other-level and deferred pairs' part
+        const float* sx = F.Lx + (size_t)A * F.ncp;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) Lc[k] += sx[k];
+    }
     l2l_apply<P>(d.x, d.y, d.z, Lp, Lc);
 #pragma unroll
     for (int k = 0; k < NC; ++k) sc[k] = Lc[k];
@@ -620,13 +625,23 @@ __global__ __launch_bounds__(256) void m2l_cross_kernel(FmmParams F, uint32_t nu
                 m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
             }
         }
-        if (!any) continue;  // uniform
+        if (!any) {  // uniform
+            if (F.Lx)
+                for (int k = lane; k < NC; k += WAVE) F.Lx[(size_t)A * F.ncp + k] = 0.f;
+            continue;
+        }
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
         if (lane == 0) {
-            float* out = F.L + (size_t)A * F.ncp;
+            if (F.Lx) {
+                float* out = F.Lx + (size_t)A * F.ncp;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) out[k] += L[k];
+                for (int k = 0; k < NC; ++k) out[k] = L[k];
+            } else {
+                float* out = F.L + (size_t)A * F.ncp;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) out[k] += L[k];
+            }
         }
         nfar = wave_sum(nfar);
         if (nfar) {
@@ -710,7 +725,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
         }
         const int Ap = __shfl_up(A, 1, WAVE);
         if (A >= 0 && (lane == 0 || Ap != A)) {
-            float* out = F.L + (size_t)A * F.ncp;
+            float* out = (F.Lx ? F.Lx : F.L) + (size_t)A * F.ncp;
 #pragma unroll
             for (int c = 0; c < NC; ++c) atomicAdd(out + c, L[c]);
         }

@@ -106,6 +106,10 @@ struct FmmParams {
     float G;
     float* M;              // [nodes][ncp]
     float* L;              // [nodes][ncp]
+    // with the tables: the other-level and deferred pairs' part of L, kept
+    // apart so that it can be computed beside the same-level kernels; L2L
+    // adds it in (nullptr: those kernels add to L)
+    float* Lx;
     int ncp;               // padded coefficient stride
     const uint32_t* leaf_ids;
     float4* pf;            // per particle: offset from its leaf centre, charge

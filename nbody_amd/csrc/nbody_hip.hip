@@ -95,7 +95,7 @@ struct nbh_handle {
     int cur = 0;
     DevBuf sys_off, keys, keys_alt, perm, perm_alt, sort_tmp, tree_tmp, scan_tmp;
     DevBuf box, inv, unit, bbox_part;
-    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, pf, pleaf, dtab, xtab, far_src, far_tgt;
+    DevBuf geo, link, nsys_of, node_begin, leaf_ids, M, L, Lx, pf, pleaf, dtab, xtab, far_src, far_tgt;
     DevBuf row_node, row_first, row_next, row_sl_start, row_sl_cnt, sl, row_m_start, row_m_cnt, row_mx_cnt,
         row_p_start, row_p_cnt, m2l_pool, p2p_pool;
     unsigned long long m2l_pool_cap = 0, p2p_pool_cap = 0;
@@ -225,6 +225,7 @@ static int alloc_nodes(nbh_handle* h, uint32_t need) {
     HIPCHK(h, h->node_begin.ensure((size_t)cap * 4));
     HIPCHK(h, h->M.ensure((size_t)cap * ncp * 4));
     HIPCHK(h, h->L.ensure((size_t)cap * ncp * 4));
+    HIPCHK(h, h->Lx.ensure((size_t)cap * ncp * 4));
     HIPCHK(h, h->row_first.ensure((size_t)cap * 4));
     HIPCHK(h, h->scan_tmp.ensure(scan_tmp_bytes(cap + 1) + 256));
     h->node_cap = cap;
@@ -321,6 +322,7 @@ static FmmParams fmm_params(nbh_handle* h) {
     F.p2p_inter = &h->counters.as<Counters>()->p2p_inter;
     F.pleaf = h->pleaf.as<uint32_t>();
     F.dtab = nullptr;
+    F.Lx = nullptr;
     F.eps2_64 = h->cfg.eps * h->cfg.eps;
     return F;
 }
@@ -550,6 +552,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     F = fmm_params(h);
     if (tables) {
         copy_tables(F, T);
+        F.Lx = h->Lx.as<float>();
         // deferred pairs: at most every other-level entry
         const unsigned long long nx = std::max<unsigned long long>(h->hc->m2lx_total, 1);
         HIPCHK(h, h->far_src.ensure((size_t)nx * 4));
@@ -571,10 +574,11 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     }
     HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_side, 0));
     if (em.on) {
-        // the same-level kernels ran beside the walk; the rest follows them
+        // the same-level kernels run beside the walk; the other-level pairs
+        // (into Lx) beside them; L2L waits for both
+        launch_m2l_parts(p, q, F, m2l_plan(h), M2L_REST, -1, h->stream);
         HIPCHK(h, hipEventRecord(h->ev_same, h->stream3));
         HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_same, 0));
-        launch_m2l_parts(p, q, F, m2l_plan(h), M2L_REST, -1, h->stream);
     } else {
         launch_m2l(p, q, F, m2l_plan(h), h->stream);
     }
@@ -859,7 +863,7 @@ void nbh_destroy(nbh_handle* h) {
     h->st[1].release();
     DevBuf* all[] = {&h->sys_off, &h->keys, &h->keys_alt, &h->perm, &h->perm_alt, &h->sort_tmp, &h->tree_tmp,
                      &h->scan_tmp, &h->box, &h->inv, &h->unit, &h->bbox_part, &h->geo, &h->link, &h->nsys_of,
-                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->pf, &h->pleaf, &h->dtab, &h->xtab, &h->far_src, &h->far_tgt, &h->row_node, &h->row_first,
+                     &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->Lx, &h->pf, &h->pleaf, &h->dtab, &h->xtab, &h->far_src, &h->far_tgt, &h->row_node, &h->row_first,
                      &h->row_next, &h->row_sl_start, &h->row_sl_cnt, &h->row_m_start, &h->row_m_cnt,
                      &h->row_mx_cnt, &h->row_p_start, &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->counters, &h->diag,
                      &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf};
