@@ -12,10 +12,10 @@ for 3 steps.
 Prints one JSON line.  The roofline entry is for the P2P kernel, timed with
 HIP events on the stream it runs on inside the timed region; its `traffic`
 comes from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.  The library
-runs two streams: the moments run beside the walk and P2P beside M2L and L2L,
-so `phases_ms` overlap (`overlapped` lists the side-stream phases) and only
-`total` is wall time; the kernels of an overlapped pair share the GPU, so
-their times are longer than alone.
+runs three streams: the moments and the same-level M2L run beside the walk, P2P
+beside M2L and L2L, so `phases_ms` overlap (`overlapped` names what each ran
+beside) and only `total` is wall time; kernels that share the GPU take longer
+than alone.
 """
 import argparse
 import json
@@ -410,7 +410,7 @@ def main():
                      "traffic_source": tr["source"] if tr else None,
                      "kernel": kernel, "ms_per_launch": p2p_ms, "flop_per_launch": p2p_flop},
         "phases_ms": {k[3:]: agg[k] / K for k in agg},
-        "overlapped": {"moments": "walk", "p2p": "m2l+l2l"},
+        "overlapped": {"moments": "walk", "m2l": "walk+p2p", "p2p": "m2l+l2l"},
         "phase_roofline": phase_roofline,
         "m2l": {"pairs_per_step": m2l_pairs / K, "tflops": m2l_tf},
         "tree": {"nodes": nodes, "leaves": leaves, "depth": st["depth"]},
