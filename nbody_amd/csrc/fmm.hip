@@ -595,44 +595,61 @@ __device__ __forceinline__ int xtab_index(const FmmParams& F, int4 ga, int4 gb, 
 // The other-level tails of the rows (after the same-level kernel): pairs one
 // level apart read their tensor from F.xtab; the rest are deferred to
 // m2l_far_kernel as one contiguous block per target.  Adds to L.
+// Sum over the 32 lanes of each half wavefront
+template <typename T>
+__device__ __forceinline__ T half_sum(T v) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
 template <int P, int Q>
-__global__ __launch_bounds__(256) void m2l_cross_kernel(FmmParams F, uint32_t num_nodes) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 : 1))) void m2l_cross_kernel(
+    FmmParams F, uint32_t num_nodes) {
     constexpr int NC = ncoef(P), NCQ = ncoef(Q), NQD = (NCQ + 3) / 4;
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const uint32_t nw = gridDim.x * blockDim.x / WAVE;
     const int lane = lane_id();
+    // two targets per wavefront, one per half: a target has ~60 such pairs,
+    // and the 35-coefficient reduction is shared by the two
+    const int half = lane >> 5, sub = lane & 31;
+    const uint64_t hmask = half ? 0xffffffff00000000ull : 0xffffffffull;
+    const uint64_t lt = lanemask_lt();
     const float4* X4 = reinterpret_cast<const float4*>(F.xtab);
-    for (uint32_t A = wid; A < num_nodes; A += nw) {
-        const int4 ga = F.nodes.geo[A];
+    for (uint32_t A0 = 2 * wid; A0 < num_nodes; A0 += 2 * nw) {  // uniform
+        const uint32_t A = A0 + half;
+        const bool act = A < num_nodes;
+        const int4 ga = act ? F.nodes.geo[A] : make_int4(0, 0, 0, 0);
         float L[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = 0.f;
         uint32_t nfar = 0, any = 0;
-        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
-            const uint32_t nx = F.lists.mx_cnt[r];
-            any |= nx;
-            const uint32_t e1 = F.lists.m_start[r] + F.lists.m_cnt[r];
-            for (uint32_t e = e1 - nx + lane; e < e1; e += WAVE) {
-                const int B = F.lists.m_pool[e];
-                uint32_t neg = 0;
-                const int t = xtab_index(F, ga, F.nodes.geo[B], neg);
-                if (t < 0) {
-                    ++nfar;
-                    continue;
+        if (act) {
+            for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
+                const uint32_t nx = F.lists.mx_cnt[r];
+                any |= nx;
+                const uint32_t e1 = F.lists.m_start[r] + F.lists.m_cnt[r];
+                for (uint32_t e = e1 - nx + sub; e < e1; e += 32) {
+                    const int B = F.lists.m_pool[e];
+                    uint32_t neg = 0;
+                    const int t = xtab_index(F, ga, F.nodes.geo[B], neg);
+                    if (t < 0) {
+                        ++nfar;
+                        continue;
+                    }
+                    float D[NCQ];
+                    load_tensor<Q>(X4 + (size_t)t * NQD, neg, D);
+                    m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
                 }
-                float D[NCQ];
-                load_tensor<Q>(X4 + (size_t)t * NQD, neg, D);
-                m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
             }
         }
-        if (!any) {  // uniform
-            if (F.Lx)
-                for (int k = lane; k < NC; k += WAVE) F.Lx[(size_t)A * F.ncp + k] = 0.f;
-            continue;
-        }
 #pragma unroll
-        for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
-        if (lane == 0) {
+        for (int k = 0; k < NC; ++k) L[k] = half_sum(L[k]);
+        nfar = half_sum(nfar);
+        if (act && !any) {  // uniform over the half
+            if (F.Lx)
+                for (int k = sub; k < NC; k += 32) F.Lx[(size_t)A * F.ncp + k] = 0.f;
+        } else if (act && sub == 0) {
             if (F.Lx) {
                 float* out = F.Lx + (size_t)A * F.ncp;
 #pragma unroll
@@ -643,38 +660,37 @@ __global__ __launch_bounds__(256) void m2l_cross_kernel(FmmParams F, uint32_t nu
                 for (int k = 0; k < NC; ++k) out[k] += L[k];
             }
         }
-        nfar = wave_sum(nfar);
-        if (nfar) {
-            uint32_t s0 = 0;
-            if (lane == 0) {
-                atomicAdd(&F.far_cnt[0], 1u);
-                s0 = atomicAdd(&F.far_cnt[1], nfar);
-            }
-            s0 = __shfl(s0, 0, WAVE);
-            // the capacity is the walk's count of other-level entries, which
-            // bounds the deferred pairs
-            if ((unsigned long long)s0 + nfar > F.far_cap) continue;
-            uint32_t pos = s0;
-            const uint64_t lt = lanemask_lt();
-            for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
-                const uint32_t e1 = F.lists.m_start[r] + F.lists.m_cnt[r];
-                const uint32_t e0 = e1 - F.lists.mx_cnt[r];
-                for (uint32_t c0 = e0; c0 < e1; c0 += WAVE) {
-                    const uint32_t e = c0 + lane;
-                    int B = -1;
-                    bool far = false;
-                    if (e < e1) {
-                        B = F.lists.m_pool[e];
-                        uint32_t neg = 0;
-                        far = xtab_index(F, ga, F.nodes.geo[B], neg) < 0;
-                    }
-                    const uint64_t bal = __ballot(far);
-                    if (far) {
-                        F.far_src[pos + __popcll(bal & lt)] = B;
-                        F.far_tgt[pos + __popcll(bal & lt)] = (int)A;
-                    }
-                    pos += __popcll(bal);
+        // deferred pairs: one contiguous run per target
+        const bool defer = act && nfar > 0;
+        if (!__ballot(defer)) continue;  // uniform
+        uint32_t s0 = 0;
+        if (defer && sub == 0) {
+            atomicAdd(&F.far_cnt[0], 1u);
+            s0 = atomicAdd(&F.far_cnt[1], nfar);
+        }
+        s0 = __shfl(s0, lane & 32, WAVE);
+        // the capacity is the walk's count of other-level entries, which
+        // bounds the deferred pairs
+        if (!defer || (unsigned long long)s0 + nfar > F.far_cap) continue;
+        uint32_t pos = s0;
+        for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
+            const uint32_t e1 = F.lists.m_start[r] + F.lists.m_cnt[r];
+            const uint32_t e0 = e1 - F.lists.mx_cnt[r];
+            for (uint32_t c0 = e0; c0 < e1; c0 += 32) {
+                const uint32_t e = c0 + sub;
+                int B = -1;
+                bool far = false;
+                if (e < e1) {
+                    B = F.lists.m_pool[e];
+                    uint32_t neg = 0;
+                    far = xtab_index(F, ga, F.nodes.geo[B], neg) < 0;
                 }
+                const uint64_t bal = __ballot(far) & hmask;
+                if (far) {
+                    F.far_src[pos + __popcll(bal & lt)] = B;
+                    F.far_tgt[pos + __popcll(bal & lt)] = (int)A;
+                }
+                pos += __popcll(bal);
             }
         }
     }
@@ -787,7 +803,8 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
                                l - F.dlevel0);
     }
     if (!(parts & M2L_REST)) return;
-    hipLaunchKernelGGL((m2l_cross_kernel<P, Q>), dim3(grid), dim3(256), 0, st, F, nn);
+    hipLaunchKernelGGL((m2l_cross_kernel<P, Q>), dim3(std::min<unsigned>(ceil_div(nn, 8), 65536u)), dim3(256), 0, st,
+                       F, nn);
     // the deferred pairs are a subset of the other-level entries (far_cap)
     const unsigned gfar = std::max<unsigned>(1u, std::min<unsigned>(ceil_div(F.far_cap, 256u), 1024u));
     hipLaunchKernelGGL((m2l_far_kernel<P, Q>), dim3(gfar), dim3(256), 0, st, F, nn);
