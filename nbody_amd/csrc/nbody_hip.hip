@@ -115,9 +115,11 @@ struct nbh_handle {
     hipEvent_t ev_fork = nullptr, ev_side = nullptr, ev_walk = nullptr, ev_near = nullptr;
     // A third stream runs each level's same-level M2L as soon as the walk
     // depth that emits it is done (ev_depth), while the walk goes deeper.
-    hipStream_t stream3 = nullptr;
+    // (two such streams, taking alternate levels, so that a big level
+    // does not wait behind the one before it)
+    hipStream_t stream3 = nullptr, stream4 = nullptr;
     hipEvent_t ev_depth[MAX_LEVELS + 1] = {};
-    hipEvent_t ev_same = nullptr;  // same-level M2L done
+    hipEvent_t ev_same = nullptr, ev_same4 = nullptr;  // same-level M2L done
     hipEvent_t evc0 = nullptr;     // timing: first same-level launch
 
     int fail(int code, const char* fmt, ...) {
@@ -421,11 +423,15 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
         bool started = false;
         if (early) {
             // same-level kernels of an earlier attempt read the rows this one rewrites
-            if (attempt > 0) HIPCHK(h, hipStreamSynchronize(h->stream3));
+            if (attempt > 0) {
+                HIPCHK(h, hipStreamSynchronize(h->stream3));
+                HIPCHK(h, hipStreamSynchronize(h->stream4));
+            }
             Fe = fmm_params(h);  // this attempt's rows and pools
             copy_tables(Fe, em->T);
             plan = m2l_plan(h);
             HIPCHK(h, hipStreamWaitEvent(h->stream3, h->ev_side, 0));  // moments and tables
+            HIPCHK(h, hipStreamWaitEvent(h->stream4, h->ev_side, 0));
         }
         // reset the walk part of the counters
         zero_u32(&dC->carried[0], (offsetof(Counters, nonfinite) - offsetof(Counters, carried)) / 4, h->stream);
@@ -433,11 +439,12 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
         for (uint32_t s = 1; s <= h->depth; ++s) {
             launch_walk_group(W, (int)s, grid[s], h->stream);
             if (early && (int)s >= Fe.dlevel0 && (int)s < Fe.dlevel0 + Fe.dnlev && h->hc->level_count[s]) {
+                hipStream_t sl = (s & 1) ? h->stream4 : h->stream3;
                 HIPCHK(h, hipEventRecord(h->ev_depth[s], h->stream));
-                HIPCHK(h, hipStreamWaitEvent(h->stream3, h->ev_depth[s], 0));
-                if (!started && h->timing) (void)hipEventRecord(h->evc0, h->stream3);
+                HIPCHK(h, hipStreamWaitEvent(sl, h->ev_depth[s], 0));
+                if (!started && h->timing) (void)hipEventRecord(h->evc0, sl);
                 started = true;
-                launch_m2l_parts(em->p, em->q, Fe, plan, M2L_LEVELS, (int)s, h->stream3);
+                launch_m2l_parts(em->p, em->q, Fe, plan, M2L_LEVELS, (int)s, sl);
             }
         }
         CHK(read_counters(h));
@@ -578,7 +585,9 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         // (into Lx) beside them; L2L waits for both
         launch_m2l_parts(p, q, F, m2l_plan(h), M2L_REST, -1, h->stream);
         HIPCHK(h, hipEventRecord(h->ev_same, h->stream3));
+        HIPCHK(h, hipEventRecord(h->ev_same4, h->stream4));
         HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_same, 0));
+        HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_same4, 0));
     } else {
         launch_m2l(p, q, F, m2l_plan(h), h->stream);
     }
@@ -776,6 +785,8 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
     CCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     CCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
     CCHK(hipStreamCreateWithFlags(&h->stream3, hipStreamNonBlocking));
+    CCHK(hipStreamCreateWithFlags(&h->stream4, hipStreamNonBlocking));
+    CCHK(hipEventCreateWithFlags(&h->ev_same4, hipEventDisableTiming));
     CCHK(hipEventCreate(&h->evc0));
     CCHK(hipEventCreateWithFlags(&h->ev_same, hipEventDisableTiming));
     for (auto& e : h->ev_depth) CCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -873,10 +884,11 @@ void nbh_destroy(nbh_handle* h) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : h->evb)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {h->ev_fork, h->ev_side, h->ev_walk, h->ev_near, h->ev_same, h->evc0})
+    for (hipEvent_t e : {h->ev_fork, h->ev_side, h->ev_walk, h->ev_near, h->ev_same, h->ev_same4, h->evc0})
         if (e) (void)hipEventDestroy(e);
     for (auto& e : h->ev_depth)
         if (e) (void)hipEventDestroy(e);
+    if (h->stream4) (void)hipStreamDestroy(h->stream4);
     if (h->stream3) (void)hipStreamDestroy(h->stream3);
     if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
