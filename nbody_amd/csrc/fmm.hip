@@ -296,10 +296,15 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
     constexpr int NC = ncoef(P);
     __shared__ float4 stage_s[P2P_WAVES][WAVE];
     __shared__ int stage_i[P2P_WAVES][WAVE];
+    __shared__ int2 ent_s[P2P_WAVES][WAVE];      // list entry: first particle, flattened start
+    __shared__ float4 entd_s[P2P_WAVES][WAVE];   // list entry: source centre - target centre
+    __shared__ unsigned long long start_s[P2P_WAVES];  // entries starting in the chunk
     const int wslot = threadIdx.x / WAVE;
     const int lane = lane_id();
     float4* stage = stage_s[wslot];
     int* sidx = stage_i[wslot];
+    int2* ent = ent_s[wslot];
+    float4* entd = entd_s[wslot];
     const uint32_t nw = gridDim.x * P2P_WAVES;
     unsigned long long inter = 0;  // particle pairs, summed over the lanes at the end
     for (uint32_t w = blockIdx.x * P2P_WAVES + wslot; w < num_leaves; w += nw) {
@@ -339,22 +344,30 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
                 }
                 const int total = __shfl(incl, WAVE - 1, WAVE);
                 const int excl = incl - cnt;
+                if (lane < ne) {
+                    ent[lane] = make_int2(beg, excl);
+                    entd[lane] = make_float4(dx, dy, dz, 0.f);
+                }
                 for (int f0 = 0; f0 < total; f0 += WAVE) {
                     const int fidx = f0 + lane;
-                    // entry k: the last lane with excl <= fidx (binary search over lanes)
-                    int lo = 0, hi = ne - 1;
-#pragma unroll
-                    for (int step = 0; step < 6; ++step) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        const int em = __shfl(excl, mid, WAVE);
-                        if (em <= fidx) lo = mid; else hi = mid - 1;
-                    }
-                    const int kb = __shfl(beg, lo, WAVE), ke = __shfl(excl, lo, WAVE);
-                    const float kx = __shfl(dx, lo, WAVE), ky = __shfl(dy, lo, WAVE), kz = __shfl(dz, lo, WAVE);
+                    // entry of flattened source fidx: the last entry with excl <=
+                    // fidx.  Every entry has at least one particle, so the starts
+                    // are distinct: the entries before the chunk, plus one bit
+                    // per start inside it (set with one LDS atomic)
+                    const int kb0 = __popcll(__ballot(lane < ne && excl < f0));
+                    if (lane == 0) start_s[wslot] = 0ull;
+                    wave_lds_sync();
+                    if (lane < ne && excl >= f0 && excl < f0 + WAVE)
+                        atomicOr(&start_s[wslot], 1ull << (excl - f0));
+                    wave_lds_sync();
+                    const unsigned long long starts = start_s[wslot];
                     if (fidx < total) {
-                        const int src = kb + (fidx - ke);
+                        const int k = kb0 + __popcll(starts & ((2ull << lane) - 1ull)) - 1;
+                        const int2 e = ent[k];
+                        const float4 d = entd[k];
+                        const int src = e.x + (fidx - e.y);
                         const float4 s = F.pf[src];
-                        stage[lane] = make_float4(kx + s.x, ky + s.y, kz + s.z, s.w);
+                        stage[lane] = make_float4(d.x + s.x, d.y + s.y, d.z + s.z, s.w);
                         sidx[lane] = src;
                     }
                     wave_lds_sync();
