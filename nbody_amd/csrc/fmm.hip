@@ -198,7 +198,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 // all 64 lanes work for any leaf occupancy; partial sums meet in LDS.
 template <int P>
 __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t num_leaves) {
-    constexpr int NC = ncoef(P);
     const int lane = lane_id();
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     if (wid >= num_leaves) return;  // whole wave exits together
@@ -293,7 +292,6 @@ constexpr int P2P_WAVES = 4;
 
 template <int P>
 __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves) {
-    constexpr int NC = ncoef(P);
     __shared__ float4 stage_s[P2P_WAVES][WAVE];
     __shared__ int stage_i[P2P_WAVES][WAVE];
     __shared__ int2 ent_s[P2P_WAVES][WAVE];      // list entry: first particle, flattened start

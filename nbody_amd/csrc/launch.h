@@ -176,11 +176,18 @@ void launch_direct_fp64(const State& S, const uint32_t* sys_off, int nsys, const
 void launch_diagnostics(const State& S, uint32_t n, double* partial, uint32_t nblk, hipStream_t st);
 uint32_t diagnostics_blocks(uint32_t n);
 
-// adjoint (vjp.hip): g = [gx | gy | gz | gm] in sorted order, FP64
-void launch_force_vjp(int p, int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const double* w_in,
-                      const uint32_t* id,
-                      float4* pw, uint32_t n, uint32_t num_leaves, const uint32_t* level_count,
-                      const uint32_t* level_off, int depth, double* g, hipStream_t st);
+// adjoint (vjp.hip): g = [gx | gy | gz | gm] in sorted order, FP64.  In
+// order: gather the cotangents (pw); the near field (raw sums into g) and the
+// transposed far-field chain (dipole P2M, M2M, M2L, L2L into Fw) are
+// independent; the far-field join adds the rest to g.
+void launch_vjp_gather(const double* w_in, const uint32_t* id, float4* pw, uint32_t n, hipStream_t st);
+void launch_vjp_near(int p, const FmmParams& F, const float4* pw, uint32_t n, uint32_t num_leaves, double* g,
+                     hipStream_t st);
+void launch_vjp_chain(int p, int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const float4* pw,
+                      uint32_t num_leaves, const uint32_t* level_count, const uint32_t* level_off, int depth,
+                      hipStream_t st);
+void launch_vjp_far(int p, const FmmParams& F, const FmmParams& Fw, const float4* pw, uint32_t n, double* g,
+                    hipStream_t st);
 
 void launch_inverse_perm(const uint32_t* id, uint32_t* slot_of, uint32_t n, hipStream_t st);
 

@@ -1073,8 +1073,18 @@ int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
     Fw.M = h->Mw.as<float>();
     Fw.L = h->Lw.as<float>();
     State S = h->st[h->cur].view();
-    launch_force_vjp(p, m2l_q(h), m2l_plan(h), F, Fw, h->wbuf.as<double>(), S.id, h->pw.as<float4>(), (uint32_t)n, h->num_leaves,
-                     h->hc->level_count, h->hc->level_off, (int)h->depth, h->gbuf.as<double>(), h->stream);
+    float4* pw = h->pw.as<float4>();
+    double* gdev = h->gbuf.as<double>();
+    launch_vjp_gather(h->wbuf.as<double>(), S.id, pw, (uint32_t)n, h->stream);
+    // the near field on the side stream, beside the transposed far-field chain
+    HIPCHK(h, hipEventRecord(h->ev_fork, h->stream));
+    HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+    launch_vjp_near(p, F, pw, (uint32_t)n, h->num_leaves, gdev, h->stream2);
+    HIPCHK(h, hipEventRecord(h->ev_near, h->stream2));
+    launch_vjp_chain(p, m2l_q(h), m2l_plan(h), F, Fw, pw, h->num_leaves, h->hc->level_count, h->hc->level_off,
+                     (int)h->depth, h->stream);
+    HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_near, 0));
+    launch_vjp_far(p, F, Fw, pw, (uint32_t)n, gdev, h->stream);
     HIPCHK(h, hipGetLastError());
     std::vector<double> g(4 * n);
     std::vector<uint32_t> ids(n);

@@ -66,18 +66,21 @@ __global__ __launch_bounds__(256) void dipole_p2m_kernel(FmmParams F, const floa
 constexpr int VJP_WAVES = 4;
 
 template <int P>
-__global__ __launch_bounds__(WAVE * VJP_WAVES) void vjp_staged_kernel(FmmParams F, const float4* pw, const float* Lw,
-                                                                      uint32_t num_leaves, double* gx, double* gy,
-                                                                      double* gz, double* gm) {
-    constexpr int NC = ncoef(P);
+__global__ __launch_bounds__(WAVE * VJP_WAVES) void vjp_staged_kernel(FmmParams F, const float4* pw, uint32_t num_leaves,
+                                                                      double* gx, double* gy, double* gz, double* gm) {
     __shared__ float4 stage_p[VJP_WAVES][WAVE];
     __shared__ float4 stage_w[VJP_WAVES][WAVE];
     __shared__ int stage_i[VJP_WAVES][WAVE];
+    __shared__ int2 ent_s[VJP_WAVES][WAVE];      // list entry: first particle, flattened start
+    __shared__ float4 entd_s[VJP_WAVES][WAVE];   // list entry: source centre - target centre
+    __shared__ unsigned long long start_s[VJP_WAVES];  // entries starting in the chunk
     const int wslot = threadIdx.x / WAVE;
     const int lane = lane_id();
     float4* sp4 = stage_p[wslot];
     float4* sw4 = stage_w[wslot];
     int* sidx = stage_i[wslot];
+    int2* ent = ent_s[wslot];
+    float4* entd = entd_s[wslot];
     const uint32_t nw = gridDim.x * VJP_WAVES;
     const float eps2 = F.eps2;
     for (uint32_t w = blockIdx.x * VJP_WAVES + wslot; w < num_leaves; w += nw) {
@@ -116,21 +119,28 @@ __global__ __launch_bounds__(WAVE * VJP_WAVES) void vjp_staged_kernel(FmmParams 
                 }
                 const int total = __shfl(incl, WAVE - 1, WAVE);
                 const int excl = incl - cnt;
+                if (lane < ne) {
+                    ent[lane] = make_int2(beg, excl);
+                    entd[lane] = make_float4(dx, dy, dz, 0.f);
+                }
                 for (int f0 = 0; f0 < total; f0 += WAVE) {
                     const int fidx = f0 + lane;
-                    int lo = 0, hi = ne - 1;
-#pragma unroll
-                    for (int step = 0; step < 6; ++step) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        const int em = __shfl(excl, mid, WAVE);
-                        if (em <= fidx) lo = mid; else hi = mid - 1;
-                    }
-                    const int kb = __shfl(beg, lo, WAVE), ke = __shfl(excl, lo, WAVE);
-                    const float kx = __shfl(dx, lo, WAVE), ky = __shfl(dy, lo, WAVE), kz = __shfl(dz, lo, WAVE);
+                    // entry of flattened source fidx (as in p2p_staged_kernel):
+                    // the entries before the chunk plus the starts inside it
+                    const int kb0 = __popcll(__ballot(lane < ne && excl < f0));
+                    if (lane == 0) start_s[wslot] = 0ull;
+                    wave_lds_sync();
+                    if (lane < ne && excl >= f0 && excl < f0 + WAVE)
+                        atomicOr(&start_s[wslot], 1ull << (excl - f0));
+                    wave_lds_sync();
+                    const unsigned long long starts = start_s[wslot];
                     if (fidx < total) {
-                        const int src = kb + (fidx - ke);
+                        const int k = kb0 + __popcll(starts & ((2ull << lane) - 1ull)) - 1;
+                        const int2 e = ent[k];
+                        const float4 dk = entd[k];
+                        const int src = e.x + (fidx - e.y);
                         const float4 s = F.pf[src];
-                        sp4[lane] = make_float4(kx + s.x, ky + s.y, kz + s.z, s.w);
+                        sp4[lane] = make_float4(dk.x + s.x, dk.y + s.y, dk.z + s.z, s.w);
                         sw4[lane] = pw[src];
                         sidx[lane] = src;
                     }
@@ -170,66 +180,103 @@ __global__ __launch_bounds__(WAVE * VJP_WAVES) void vjp_staged_kernel(FmmParams 
                 }
             }
             if (sp == 0) {
-                // far field
-                float L[NC];
-                float H[6];
-                {
-                    const float* Ls = F.L + (size_t)A * F.ncp;
-#pragma unroll
-                    for (int k = 0; k < NC; ++k) L[k] = Ls[k];
-                    l2p_hessian<P>(me.x, me.y, me.z, L, H);
-                }
-                float psi, px, py, pz;
-                {
-                    const float* Ls = Lw + (size_t)A * F.ncp;
-#pragma unroll
-                    for (int k = 0; k < NC; ++k) L[k] = Ls[k];
-                    l2p_eval<P>(me.x, me.y, me.z, L, psi, px, py, pz);
-                }
-                g0 += mi * px + H[0] * wi.x + H[3] * wi.y + H[4] * wi.z;
-                g1 += mi * py + H[3] * wi.x + H[1] * wi.y + H[5] * wi.z;
-                g2 += mi * pz + H[4] * wi.x + H[5] * wi.y + H[2] * wi.z;
-                gmm += psi;
-                const double G = (double)F.G;
-                gx[i] = G * g0;
-                gy[i] = G * g1;
-                gz[i] = G * g2;
-                gm[i] = G * gmm;
+                // raw near-field sums (FP32 values); vjp_far_kernel adds the far field
+                gx[i] = g0;
+                gy[i] = g1;
+                gz[i] = g2;
+                gm[i] = gmm;
             }
         }
     }
 }
 
+// Far-field part of the VJP, one lane per particle, after the near-field
+// kernel (raw sums in g) and the transposed far-field chain (Lw): the tidal
+// term from the forward local expansion's Hessian, the field of the
+// cotangent dipoles from Lw, then G.
 template <int P>
-static void launch_vjp_t(int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const double* w_in,
-                         const uint32_t* id, float4* pw, uint32_t n, uint32_t num_leaves,
-                         const uint32_t* level_count, const uint32_t* level_off, int depth, double* g,
-                         hipStream_t st) {
-    hipLaunchKernelGGL(vjp_gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, w_in, id, pw, n);
+__global__ __launch_bounds__(256) void vjp_far_kernel(FmmParams F, const float4* pw, const float* Lw, uint32_t n,
+                                                      double* gx, double* gy, double* gz, double* gm) {
+    constexpr int NC = ncoef(P);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 me = F.pf[i];
+    const float4 wi = pw[i];
+    const float mi = me.w;
+    const uint32_t A = F.pleaf[i];
+    float L[NC];
+    float H[6];
+    {
+        const float* Ls = F.L + (size_t)A * F.ncp;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = Ls[k];
+        l2p_hessian<P>(me.x, me.y, me.z, L, H);
+    }
+    float psi, px, py, pz;
+    {
+        const float* Ls = Lw + (size_t)A * F.ncp;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) L[k] = Ls[k];
+        l2p_eval<P>(me.x, me.y, me.z, L, psi, px, py, pz);
+    }
+    float g0 = (float)gx[i], g1 = (float)gy[i], g2 = (float)gz[i], gmm = (float)gm[i];
+    g0 += mi * px + H[0] * wi.x + H[3] * wi.y + H[4] * wi.z;
+    g1 += mi * py + H[3] * wi.x + H[1] * wi.y + H[5] * wi.z;
+    g2 += mi * pz + H[4] * wi.x + H[5] * wi.y + H[2] * wi.z;
+    gmm += psi;
+    const double G = (double)F.G;
+    gx[i] = G * g0;
+    gy[i] = G * g1;
+    gz[i] = G * g2;
+    gm[i] = G * gmm;
+}
+
+template <int P>
+static void launch_vjp_near_t(const FmmParams& F, const float4* pw, uint32_t n, uint32_t num_leaves, double* g,
+                              hipStream_t st) {
+    const unsigned grid = std::min<unsigned>(ceil_div(num_leaves, VJP_WAVES), 65536u);
+    hipLaunchKernelGGL((vjp_staged_kernel<P>), dim3(grid), dim3(WAVE * VJP_WAVES), 0, st, F, pw, num_leaves, g, g + n,
+                       g + 2 * (size_t)n, g + 3 * (size_t)n);
+}
+
+template <int P>
+static void launch_vjp_chain_t(int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const float4* pw,
+                               uint32_t num_leaves, const uint32_t* level_count, const uint32_t* level_off,
+                               int depth, hipStream_t st) {
     hipLaunchKernelGGL((dipole_p2m_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, pw, Fw.M,
                        num_leaves);
     for (int l = depth - 1; l >= 0; --l) launch_m2m_level(P, Fw, l, level_count[l], level_off[l], st);
     launch_m2l(P, q, Fw, plan, st);
     for (int l = 1; l <= depth; ++l) launch_l2l_level(P, Fw, l, level_count[l], level_off[l], st);
-    const unsigned grid = std::min<unsigned>(ceil_div(num_leaves, VJP_WAVES), 65536u);
-    hipLaunchKernelGGL((vjp_staged_kernel<P>), dim3(grid), dim3(WAVE * VJP_WAVES), 0, st, F, pw, Fw.L, num_leaves, g,
-                       g + n, g + 2 * (size_t)n, g + 3 * (size_t)n);
 }
 
-void launch_force_vjp(int p, int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const double* w_in,
-                      const uint32_t* id, float4* pw, uint32_t n, uint32_t num_leaves, const uint32_t* level_count,
-                      const uint32_t* level_off, int depth, double* g, hipStream_t st) {
-#define NBH_VJP(PP) launch_vjp_t<PP>(q, plan, F, Fw, w_in, id, pw, n, num_leaves, level_count, level_off, depth, g, st)
-    switch (p) {
-        case 1: NBH_VJP(1); break;
-        case 2: NBH_VJP(2); break;
-        case 3: NBH_VJP(3); break;
-        case 4: NBH_VJP(4); break;
-        case 5: NBH_VJP(5); break;
-        case 6: NBH_VJP(6); break;
-        default: break;
+#define NBH_VJP_DISPATCH(p, call)                      \
+    switch (p) {                                       \
+        case 1: { constexpr int PP = 1; call; } break; \
+        case 2: { constexpr int PP = 2; call; } break; \
+        case 3: { constexpr int PP = 3; call; } break; \
+        case 4: { constexpr int PP = 4; call; } break; \
+        case 5: { constexpr int PP = 5; call; } break; \
+        case 6: { constexpr int PP = 6; call; } break; \
+        default: break;                                \
     }
-#undef NBH_VJP
+
+void launch_vjp_gather(const double* w_in, const uint32_t* id, float4* pw, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(vjp_gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, w_in, id, pw, n);
+}
+void launch_vjp_near(int p, const FmmParams& F, const float4* pw, uint32_t n, uint32_t num_leaves, double* g,
+                     hipStream_t st) {
+    NBH_VJP_DISPATCH(p, launch_vjp_near_t<PP>(F, pw, n, num_leaves, g, st));
+}
+void launch_vjp_chain(int p, int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const float4* pw,
+                      uint32_t num_leaves, const uint32_t* level_count, const uint32_t* level_off, int depth,
+                      hipStream_t st) {
+    NBH_VJP_DISPATCH(p, launch_vjp_chain_t<PP>(q, plan, F, Fw, pw, num_leaves, level_count, level_off, depth, st));
+}
+void launch_vjp_far(int p, const FmmParams& F, const FmmParams& Fw, const float4* pw, uint32_t n, double* g,
+                    hipStream_t st) {
+    NBH_VJP_DISPATCH(p, hipLaunchKernelGGL((vjp_far_kernel<PP>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, pw,
+                                           Fw.L, n, g, g + n, g + 2 * (size_t)n, g + 3 * (size_t)n));
 }
 
 }  // namespace nbh
