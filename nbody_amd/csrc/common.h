@@ -64,6 +64,52 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Values per lane after s halvings of n (rounding up)
+constexpr int rs_len(int n, int s) {
+    for (; s > 0; --s) n = (n + 1) / 2;
+    return n;
+}
+
+// Butterfly reduce-scatter of v[0, N) over each aligned group of 2^S lanes:
+// at the step over lane bit b the pair (l, l ^ 2^b) splits the values in two
+// halves, each lane keeps one half and adds its partner's copy of it.  After
+// the S steps v[0, rs_len(N, S)) of a lane holds group sums; rs_slot names
+// which.  It takes sum(rs_len(N, 1..S)) shuffles where a sum per value
+// (wave_sum) takes N S.
+template <int N, int S>
+__device__ __forceinline__ void group_reduce_scatter(float* v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int b = S - 1 - t;
+        const int n = rs_len(N, t), h = (n + 1) / 2;
+        const bool up = (lane >> b) & 1;
+#pragma unroll
+        for (int i = 0; i < h; ++i) {
+            const float lo = v[i], hi = h + i < n ? v[h + i] : 0.f;
+            v[i] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, 1 << b, WAVE);
+        }
+    }
+}
+
+// After group_reduce_scatter<N, S>: v[j] of this lane, j < cnt, is the group
+// sum of value start + j (cnt may be 0).
+__device__ __forceinline__ void rs_slot(int N, int S, int lane, int& start, int& cnt) {
+    int n = N;
+    start = 0;
+    cnt = N;
+    for (int b = S - 1; b >= 0; --b) {
+        const int h = (n + 1) / 2;
+        if ((lane >> b) & 1) {
+            start += h;
+            cnt = cnt > h ? cnt - h : 0;
+        } else {
+            cnt = cnt < h ? cnt : h;
+        }
+        n = h;
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll

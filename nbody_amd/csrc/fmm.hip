@@ -476,17 +476,27 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
                                                                   int lvl) {
     constexpr int NC = ncoef(P), NCQ = ncoef(Q);
     constexpr int NQD = (NCQ + 3) / 4;  // float4 per tensor row (the host sets dcp = 4 NQD)
+    // LDS row stride in float4: odd, so that the rows that the lanes of a
+    // wavefront read at once start in different bank groups (an even stride,
+    // 14 at q = 5, maps every row to one of four groups of the 32 banks)
+    constexpr int NQS = NQD | 1;
     extern __shared__ float4 dtab_lds[];
     const int K = F.dK, K1 = K + 1;
     const float4* src = reinterpret_cast<const float4*>(F.dtab + (size_t)lvl * F.dsz * F.dcp);
     if (IN_LDS) {
         const int tot = F.dsz * NQD;
-        for (int i = threadIdx.x; i < tot; i += blockDim.x) dtab_lds[i] = src[i];
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+            const int r = i / NQD;
+            dtab_lds[r * NQS + (i - r * NQD)] = src[i];
+        }
         __syncthreads();
     }
     const float4* tab = IN_LDS ? dtab_lds : src;
+    constexpr int RS = IN_LDS ? NQS : NQD;
     const int lane = lane_id();
     const uint32_t wpb = blockDim.x / WAVE;
+    int rs0, rsn;  // coefficients [rs0, rs0 + rsn) of this lane after the reduction
+    rs_slot(NC, 6, lane, rs0, rsn);
     // launched beside the walk: after an overflow at this depth or above the
     // level's rows are incomplete, and the host repeats the walk and M2L
     if (F.C->overflow & (OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P)) return;
@@ -508,7 +518,7 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
                 const int4 gb = F.nodes.geo[B];
                 const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
                 const uint32_t neg = (uint32_t)(ox < 0) | ((uint32_t)(oy < 0) << 1) | ((uint32_t)(oz < 0) << 2);
-                const float4* row = tab + ((abs(ox) * K1 + abs(oy)) * K1 + abs(oz)) * NQD;
+                const float4* row = tab + ((abs(ox) * K1 + abs(oy)) * K1 + abs(oz)) * RS;
                 float D[NCQ];
 #pragma unroll
                 for (int q = 0; q < NQD; ++q) {
@@ -522,13 +532,11 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
                 m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
             }
         }
+        group_reduce_scatter<NC, 6>(L);
+        float* out = F.L + (size_t)A * F.ncp + rs0;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
-        if (lane == 0) {
-            float* out = F.L + (size_t)A * F.ncp;
-#pragma unroll
-            for (int k = 0; k < NC; ++k) out[k] = L[k];
-        }
+        for (int j = 0; j < rs_len(NC, 6); ++j)
+            if (j < rsn) out[j] = L[j];
     }
 }
 
@@ -557,6 +565,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const uint32_t nw = gridDim.x * blockDim.x / WAVE;
     const int lane = lane_id();
+    int rs0, rsn;
+    rs_slot(NC, 6, lane, rs0, rsn);
     for (uint32_t A = wid; A < num_nodes; A += nw) {
         const int4 ga = F.nodes.geo[A];
         const float unit = F.unit[F.nodes.sys[A]];
@@ -574,13 +584,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
                 m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
             }
         }
+        group_reduce_scatter<NC, 6>(L);
+        float* out = F.L + (size_t)A * F.ncp + rs0;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) L[k] = wave_sum(L[k]);
-        if (lane == 0) {
-            float* out = F.L + (size_t)A * F.ncp;
-#pragma unroll
-            for (int k = 0; k < NC; ++k) out[k] = L[k];
-        }
+        for (int j = 0; j < rs_len(NC, 6); ++j)
+            if (j < rsn) out[j] = L[j];
     }
 }
 
@@ -626,6 +634,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 
     const int half = lane >> 5, sub = lane & 31;
     const uint64_t hmask = half ? 0xffffffff00000000ull : 0xffffffffull;
     const uint64_t lt = lanemask_lt();
+    int rs0, rsn;  // coefficients of this lane after the reduction over the half
+    rs_slot(NC, 5, lane, rs0, rsn);
     const float4* X4 = reinterpret_cast<const float4*>(F.xtab);
     for (uint32_t A0 = 2 * wid; A0 < num_nodes; A0 += 2 * nw) {  // uniform
         const uint32_t A = A0 + half;
@@ -654,22 +664,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 
                 }
             }
         }
-#pragma unroll
-        for (int k = 0; k < NC; ++k) L[k] = half_sum(L[k]);
+        group_reduce_scatter<NC, 5>(L);
         nfar = half_sum(nfar);
-        if (act && !any) {  // uniform over the half
-            if (F.Lx)
-                for (int k = sub; k < NC; k += 32) F.Lx[(size_t)A * F.ncp + k] = 0.f;
-        } else if (act && sub == 0) {
-            if (F.Lx) {
-                float* out = F.Lx + (size_t)A * F.ncp;
+        // a target without such entries has L = 0 here: Lx is still written
+        // (it replaces the previous evaluation's), L is left alone
+        if (act && F.Lx) {
+            float* out = F.Lx + (size_t)A * F.ncp + rs0;
 #pragma unroll
-                for (int k = 0; k < NC; ++k) out[k] = L[k];
-            } else {
-                float* out = F.L + (size_t)A * F.ncp;
+            for (int j = 0; j < rs_len(NC, 5); ++j)
+                if (j < rsn) out[j] = L[j];
+        } else if (act && any) {
+            float* out = F.L + (size_t)A * F.ncp + rs0;
 #pragma unroll
-                for (int k = 0; k < NC; ++k) out[k] += L[k];
-            }
+            for (int j = 0; j < rs_len(NC, 5); ++j)
+                if (j < rsn) out[j] += L[j];
         }
         // deferred pairs: one contiguous run per target
         const bool defer = act && nfar > 0;
@@ -793,7 +801,7 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
             hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)ceil_div(nz, (size_t)256)), dim3(256), 0, st, F.L, nz);
         }
     }
-    const size_t lds = (size_t)F.dsz * F.dcp * 4;
+    const size_t lds = (size_t)F.dsz * ((F.dcp / 4) | 1) * 16;  // rows padded to an odd float4 count
     const bool in_lds = lds <= plan.max_lds && m2l_table_threads(P, Q) == 1024;
     if (in_lds && (parts & M2L_LEVELS))
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true>),

@@ -48,13 +48,13 @@ __global__ __launch_bounds__(256) void dipole_p2m_kernel(FmmParams F, const floa
         const float4 w = pw[i];
         dipole_p2m_accum<P>(b.x, b.y, b.z, w.x, w.y, w.z, M);
     }
+    group_reduce_scatter<NC, 6>(M);
+    int rs0, rsn;
+    rs_slot(NC, 6, lane_id(), rs0, rsn);
+    float* out = Mw + (size_t)A * F.ncp + rs0;
 #pragma unroll
-    for (int k = 0; k < NC; ++k) M[k] = wave_sum(M[k]);
-    if (lane_id() == 0) {
-        float* out = Mw + (size_t)A * F.ncp;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) out[k] = M[k];
-    }
+    for (int j = 0; j < rs_len(NC, 6); ++j)
+        if (j < rsn) out[j] = M[j];
 }
 
 // near field (pair VJP over the P2P lists) + far field (Hessian of the mass
