@@ -282,27 +282,40 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
 // P2P, staged: one wavefront per target leaf, sources flattened across the
 // leaf's P2P list.  Up to 64 list entries at a time are read by the lanes
 // (one entry each: centre offset, first particle, count) and prefix-summed;
-// then 64 source particles at a time are gathered by a shuffle binary search
-// over that prefix, shifted into the target leaf's frame and staged in LDS.
+// then 64 source particles at a time are gathered (each lane finds its
+// source's entry from a bitmask of the entry starts), shifted into the target
+// leaf's frame and staged in LDS.
 // Lanes are (target particle) x (source stride): with nt targets, lane l
-// takes target l % nt and staged sources j == l / nt (mod 64 / nt), so the
-// wave stays full for any leaf size and every lane runs the same trip count.
+// takes target l % nt and the staged sources j = sp + k splits, k < Tc, of
+// stride sp = l / nt (splits = 64 / nt), so the wave stays full for any leaf
+// size.  Tc = ceil(nchunk / splits) is the same for every lane: slots past the
+// chunk's nchunk sources hold zero charges, every lane runs Tc iterations, and
+// a partial chunk costs only the iterations that it fills.
 constexpr int P2P_WAVES = 4;
-
+// stage slots: splits * ceil(nchunk / splits) <= 84 for nt, nchunk in [1, 64]
+constexpr int P2P_STAGE = 96;
 
 template <int P>
 __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves) {
-    __shared__ float4 stage_s[P2P_WAVES][WAVE];
-    __shared__ int stage_i[P2P_WAVES][WAVE];
+    __shared__ float4 stage_s[P2P_WAVES][P2P_STAGE];
+    __shared__ int stage_i[P2P_WAVES][P2P_STAGE];
     __shared__ int2 ent_s[P2P_WAVES][WAVE];      // list entry: first particle, flattened start
     __shared__ float4 entd_s[P2P_WAVES][WAVE];   // list entry: source centre - target centre
     __shared__ unsigned long long start_s[P2P_WAVES];  // entries starting in the chunk
+    // (left as a vector value: made uniform with readfirstlane, the leaf
+    // records come through scalar loads, whose waits share the counter of
+    // the LDS reads, and the kernel is slower)
     const int wslot = threadIdx.x / WAVE;
     const int lane = lane_id();
     float4* stage = stage_s[wslot];
     int* sidx = stage_i[wslot];
     int2* ent = ent_s[wslot];
     float4* entd = entd_s[wslot];
+    // slots past 64 are never staged: zero charges for good
+    if (lane < P2P_STAGE - WAVE) {
+        stage[WAVE + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        sidx[WAVE + lane] = -1;
+    }
     const uint32_t nw = gridDim.x * P2P_WAVES;
     unsigned long long inter = 0;  // particle pairs, summed over the lanes at the end
     for (uint32_t w = blockIdx.x * P2P_WAVES + wslot; w < num_leaves; w += nw) {
@@ -314,6 +327,8 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
         for (int t0 = 0; t0 < la.y; t0 += WAVE) {
             const int nt = min(WAVE, la.y - t0);
             const int splits = WAVE / nt;
+            // ceil(x / splits) = umulhi(x + splits - 1, magic) for x <= 64
+            const uint32_t magic = splits == 1 ? 0u : 0xffffffffu / (uint32_t)splits + 1u;
             const int ti = lane % nt;
             const int sp = lane / nt;
             const int i = la.x + t0 + ti;
@@ -359,24 +374,33 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
                         atomicOr(&start_s[wslot], 1ull << (excl - f0));
                     wave_lds_sync();
                     const unsigned long long starts = start_s[wslot];
-                    if (fidx < total) {
-                        const int k = kb0 + __popcll(starts & ((2ull << lane) - 1ull)) - 1;
-                        const int2 e = ent[k];
-                        const float4 d = entd[k];
-                        const int src = e.x + (fidx - e.y);
-                        const float4 s = F.pf[src];
-                        stage[lane] = make_float4(d.x + s.x, d.y + s.y, d.z + s.z, s.w);
+                    {
+                        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);  // past the sources: no charge
+                        int src = -1;
+                        if (fidx < total) {
+                            const int k = kb0 + __popcll(starts & ((2ull << lane) - 1ull)) - 1;
+                            const int2 e = ent[k];
+                            const float4 d = entd[k];
+                            src = e.x + (fidx - e.y);
+                            const float4 s = F.pf[src];
+                            v = make_float4(d.x + s.x, d.y + s.y, d.z + s.z, s.w);
+                        }
+                        stage[lane] = v;
                         sidx[lane] = src;
                     }
-                    wave_lds_sync();
                     const int nchunk = min(WAVE, total - f0);
+                    const int Tc = splits == 1 ? nchunk : (int)__umulhi((uint32_t)(nchunk + splits - 1), magic);
+                    const float4* blk = stage + sp;
+                    const int* blki = sidx + sp;
+                    wave_lds_sync();
                     if (sp < splits) {
                         if (eps2 > 0.f) {
                             // the self pair has r = 0 exactly: no force, potential
-                            // q_i/eps (removed after the loop)
+                            // q_i/eps (removed after the loop); a zero-charge
+                            // slot at the target's own position gives q/eps = 0
 #pragma unroll 4
-                            for (int j = sp; j < nchunk; j += splits) {
-                                const float4 s = stage[j];
+                            for (int k = 0; k < Tc; ++k) {
+                                const float4 s = blk[k * splits];
                                 const float rx = s.x - me.x, ry = s.y - me.y, rz = s.z - me.z;
                                 const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
                                 // r2 >= eps2 is a normal float: the bare
@@ -390,11 +414,13 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
                                 ph += qi;
                             }
                         } else {
-                            for (int j = sp; j < nchunk; j += splits) {
-                                const float4 s = stage[j];
+                            // zero-charge slots (index -1) are skipped like the self pair
+                            for (int k = 0; k < Tc; ++k) {
+                                const float4 s = blk[k * splits];
                                 const float rx = s.x - me.x, ry = s.y - me.y, rz = s.z - me.z;
                                 const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-                                const float inv = sidx[j] == i ? 0.f : rsqrtf(r2);
+                                const int sj = blki[k * splits];
+                                const float inv = (sj == i || sj < 0) ? 0.f : rsqrtf(r2);
                                 const float qi = s.w * inv;
                                 const float qi3 = qi * inv * inv;
                                 ax = fmaf(rx, qi3, ax);
