@@ -295,8 +295,10 @@ constexpr int P2P_WAVES = 4;
 // stage slots: splits * ceil(nchunk / splits) <= 84 for nt, nchunk in [1, 64]
 constexpr int P2P_STAGE = 96;
 
-template <int P>
-__global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves) {
+// SOFT: eps > 0 (each target meets itself at r = 0 with a finite softened
+// potential); else the self pair and the empty slots are skipped by index.
+template <bool SOFT>
+__device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State& S, uint32_t num_leaves) {
     __shared__ float4 stage_s[P2P_WAVES][P2P_STAGE];
     __shared__ int stage_i[P2P_WAVES][P2P_STAGE];
     __shared__ int2 ent_s[P2P_WAVES][WAVE];      // list entry: first particle, flattened start
@@ -389,12 +391,14 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
                         sidx[lane] = src;
                     }
                     const int nchunk = min(WAVE, total - f0);
-                    const int Tc = splits == 1 ? nchunk : (int)__umulhi((uint32_t)(nchunk + splits - 1), magic);
+                    // uniform: counted on the scalar unit
+                    const int Tc = __builtin_amdgcn_readfirstlane(
+                        splits == 1 ? nchunk : (int)__umulhi((uint32_t)(nchunk + splits - 1), magic));
                     const float4* blk = stage + sp;
                     const int* blki = sidx + sp;
                     wave_lds_sync();
                     if (sp < splits) {
-                        if (eps2 > 0.f) {
+                        if (SOFT) {
                             // the self pair has r = 0 exactly: no force, potential
                             // q_i/eps (removed after the loop); a zero-charge
                             // slot at the target's own position gives q/eps = 0
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
                 }
             }
             // the self term q_i/eps sits in exactly one stride's partial sum
-            if (eps2 > 0.f && sp == 0) ph -= me.w * __builtin_amdgcn_rsqf(eps2);
+            if (SOFT && sp == 0) ph -= me.w * __builtin_amdgcn_rsqf(eps2);
             // combine the source strides of each target
             {
                 const float ax0 = ax, ay0 = ay, az0 = az, ph0 = ph;
@@ -456,6 +460,17 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams 
     }
     inter = wave_sum(inter);
     if (lane == 0 && inter) atomicAdd(F.p2p_inter, inter);
+}
+
+template <int P>
+__global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves) {
+    p2p_staged_body<true>(F, S, num_leaves);
+}
+
+template <int P>
+__global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_unsoftened_kernel(FmmParams F, State S,
+                                                                                 uint32_t num_leaves) {
+    p2p_staged_body<false>(F, S, num_leaves);
 }
 
 // ---------------------------------------------------------------------------
@@ -887,7 +902,11 @@ static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves
 template <int P>
 static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     const unsigned grid = std::min<unsigned>(ceil_div(num_leaves, P2P_WAVES), 65536u);
-    hipLaunchKernelGGL((p2p_staged_kernel<P>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S, num_leaves);
+    if (F.eps2 > 0.f)
+        hipLaunchKernelGGL((p2p_staged_kernel<P>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S, num_leaves);
+    else
+        hipLaunchKernelGGL((p2p_staged_unsoftened_kernel<P>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S,
+                           num_leaves);
 }
 #define NBH_DISPATCH_P(p, call)          \
     switch (p) {                         \
