@@ -10,12 +10,15 @@
 // -fopenmp.  State is FP64; the interface's float Particle is converted on the
 // way in and out.  Integrators as the HIP backend: KDK leapfrog (default),
 // kick-drift (naive_simulation.cpp:28-41) or explicit Euler
-// (open_cl_simulation.cpp:600-607).
+// (open_cl_simulation.cpp:600-607).  With method = 1 the forces come from the
+// FP64 fast multipole method of nbody/cpu_fmm.h (the HIP library's tree,
+// lists and expansions) instead of the direct sum.
 #ifndef NBODY_CPU_SIMULATION_H_
 #define NBODY_CPU_SIMULATION_H_
 
 #include <cmath>
 #include <cstddef>
+#include <memory>
 #include <ostream>
 #include <stdexcept>
 #include <vector>
@@ -24,6 +27,7 @@
 #include <omp.h>
 #endif
 
+#include "nbody/cpu_fmm.h"
 #include "nbody/simulation.h"
 
 namespace nbody {
@@ -33,6 +37,8 @@ struct CpuOptions {
 	double eps = 0.01;    // reference PARTICLE_RADIUS (field.cl:3-5)
 	int integrator = 0;   // 0 KDK, 1 explicit Euler, 2 kick-drift
 	int threads = 0;      // OpenMP threads, 0 = runtime default
+	int method = 0;       // 0 direct sum, 1 fast multipole method
+	FmmOptions fmm;       // with method = 1
 };
 
 // The FP64 engine, independent of the particle type (used by the C ABI too).
@@ -47,10 +53,27 @@ public:
 		for (std::size_t i = 0; i < n; ++i) {
 			if (!(_m[i] > 0)) throw std::runtime_error("CpuSimulation: masses must be positive");
 		}
+		set_method(opt.method, opt.fmm);
 	}
+
+	// switches the force method (0 direct sum, 1 FMM with the given options)
+	void set_method(int method, FmmOptions fmm = FmmOptions()) {
+		if (method == 1) _fmm = std::make_shared<CpuFmm>(fmm);
+		else if (method == 0) _fmm.reset();
+		else throw std::runtime_error("CpuSimulation: method must be 0 (direct) or 1 (FMM)");
+		_opt.method = method;
+		_opt.fmm = fmm;
+		_acc_valid = false;
+	}
+
+	const CpuFmm* fmm() const { return _fmm.get(); }
 
 	// accelerations and potentials at the current positions
 	void forces() {
+		if (_fmm) {
+			forces_fmm();
+			return;
+		}
 		const std::size_t n = _n;
 		const double e2 = _opt.eps * _opt.eps, G = _opt.G;
 		const double* x = _x.data();
@@ -81,6 +104,22 @@ public:
 			a[3 * i + 1] = f * ay;
 			a[3 * i + 2] = f * az;
 			pot[i] = -G * ph;
+		}
+		_acc_valid = true;
+	}
+
+	void forces_fmm() {
+#if defined(_OPENMP)
+		if (_opt.threads > 0) omp_set_num_threads(_opt.threads);
+#endif
+		_fmm->evaluate(_n, _x.data(), _q.data(), _opt.eps, _a.data(), _pot.data());
+		const double G = _opt.G;
+		for (std::size_t i = 0; i < _n; ++i) {
+			const double f = G * _q[i] / _m[i];
+			_a[3 * i] *= f;
+			_a[3 * i + 1] *= f;
+			_a[3 * i + 2] *= f;
+			_pot[i] *= -G;
 		}
 		_acc_valid = true;
 	}
@@ -141,6 +180,7 @@ private:
 	std::vector<double> _x, _v, _m, _q, _a, _pot;
 	double _time = 0;
 	bool _acc_valid = false;
+	std::shared_ptr<CpuFmm> _fmm;
 };
 
 template <typename TScalar, typename TVector>

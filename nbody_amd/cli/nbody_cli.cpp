@@ -1,5 +1,7 @@
 // nbody_cli.cpp -- command-line driver over nbody::HipSimulation (or the
-// multi-core nbody::CpuSimulation, --backend cpu).
+// multi-core nbody::CpuSimulation: --backend cpu for its direct sum, cpu-fmm
+// for its FP64 fast multipole method with the same theta, order and leaf
+// capacity options).
 //
 // Plays the role of the reference's src/main.cpp: builds the initial
 // particles, steps the simulation to a final time and appends one CSV row
@@ -8,7 +10,7 @@
 // std::rand().  Usage:
 //   nbody_cli [--ic uniform|plummer] [--n N] [--seed S] [--dt DT] [--t-end T]
 //             [--csv FILE] [--theta TH] [--order P] [--leaf-cap C] [--eps E]
-//             [--backend hip|cpu]
+//             [--backend hip|cpu|cpu-fmm]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -65,10 +67,16 @@ int main(int argc, char** argv) {
 		}
 		Simulation::Vector bounds{1.f, 1.f, 1.f, 0.f};
 		std::unique_ptr<Simulation> sim;
-		if (backend == "cpu") {
+		if (backend == "cpu" || backend == "cpu-fmm") {
 			nbody::CpuOptions opt;
 			opt.G = cfg.G;
 			opt.eps = cfg.eps;
+			if (backend == "cpu-fmm") {
+				opt.method = 1;
+				opt.fmm.order = (int)cfg.order;
+				opt.fmm.theta = cfg.theta;
+				opt.fmm.leaf_capacity = (int)cfg.leaf_capacity;
+			}
 			sim.reset(new CpuSim(bounds, particles, (float)dt, std::cout, opt));
 		} else if (backend == "hip") {
 			sim.reset(new HipSim(bounds, particles, (float)dt, std::cout, cfg));

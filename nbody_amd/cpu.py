@@ -1,6 +1,9 @@
-"""CpuSimulation: the multi-core FP64 direct-sum backend (libnbody_cpu.so,
+"""CpuSimulation: the multi-core FP64 CPU backend (libnbody_cpu.so,
 include/nbody_cpu.h) with the interface of HipSimulation -- step() returning
-the new time and particles() in input order (simulation.h:6-37)."""
+the new time and particles() in input order (simulation.h:6-37).  Forces come
+from the direct sum (method="direct") or from the FP64 fast multipole method
+of include/nbody/cpu_fmm.h (method="fmm": the HIP library's tree, lists and
+expansions on host cores)."""
 from __future__ import annotations
 
 import ctypes
@@ -30,6 +33,9 @@ def lib():
         L.ncpu_get_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.ncpu_set_state.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double]
         L.ncpu_forces.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ncpu_set_fmm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.ncpu_fmm_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.ncpu_fmm_times.argtypes = [ctypes.c_void_p, _dp]
         L.ncpu_destroy.argtypes = [ctypes.c_void_p]
         L.ncpu_last_error.restype = ctypes.c_char_p
         _LIB = L
@@ -42,10 +48,13 @@ def _p(a):
 
 class CpuSimulation(Simulation):
     """Same constructor prefix as HipSimulation / OpenClSimulation; options:
-    G, eps (softening), integrator (INT_*), threads (0 = OpenMP default)."""
+    G, eps (softening), integrator (INT_*), threads (0 = OpenMP default),
+    method ("direct" or "fmm") and, for the FMM, order, m2l_order (0: order +
+    1), theta and leaf_capacity."""
 
     def __init__(self, bounds=None, particles: Optional[Iterable[Particle]] = None, time_step: float = 1e-3,
-                 log=None, *, G=1.0, eps=0.01, integrator=INT_KDK, threads=0, arrays=None):
+                 log=None, *, G=1.0, eps=0.01, integrator=INT_KDK, threads=0, arrays=None, method="direct",
+                 order=4, m2l_order=0, theta=0.5, leaf_capacity=32):
         if arrays is not None:
             pos, vel, mass, charge = arrays
         else:
@@ -67,8 +76,14 @@ class CpuSimulation(Simulation):
         if rc:
             raise RuntimeError(f"ncpu_create failed: {lib().ncpu_last_error().decode()}")
         self._h = h
+        if method == "fmm":
+            self._check(lib().ncpu_set_fmm(self._h, int(order), int(m2l_order), float(theta), int(leaf_capacity)),
+                        "ncpu_set_fmm")
+        elif method != "direct":
+            self.close()
+            raise ValueError(f"method must be 'direct' or 'fmm', not {method!r}")
         if log is not None:
-            log.write(f"CpuSimulation: {self.n} particles\n")
+            log.write(f"CpuSimulation: {self.n} particles ({method})\n")
 
     @classmethod
     def from_arrays(cls, pos, vel, mass, charge=None, time_step=1e-3, **opts):
@@ -112,3 +127,16 @@ class CpuSimulation(Simulation):
         p = np.zeros(self.n)
         self._check(lib().ncpu_forces(self._h, _p(a), _p(p)), "ncpu_forces")
         return a, p
+
+    def fmm_stats(self):
+        """Statistics of the last FMM evaluation: nodes, leaves, depth, unordered
+        M2L and P2P cell pairs, visited cell pairs."""
+        out = (ctypes.c_longlong * 6)()
+        self._check(lib().ncpu_fmm_stats(self._h, out), "ncpu_fmm_stats")
+        return dict(zip(("nodes", "leaves", "depth", "m2l_unordered", "p2p_unordered", "visited"), list(out)))
+
+    def fmm_times(self):
+        """Seconds per phase of the last FMM evaluation."""
+        out = np.zeros(6)
+        self._check(lib().ncpu_fmm_times(self._h, _p(out)), "ncpu_fmm_times")
+        return dict(zip(("tree", "walk", "upward", "m2l", "downward", "leaves"), out.tolist()))

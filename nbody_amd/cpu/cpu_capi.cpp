@@ -41,7 +41,11 @@ int ncpu_create(size_t n, const double* pos, const double* vel, const double* ma
 int ncpu_step(ncpu_handle* h, int nsteps, double* t) {
 	if (!h || nsteps < 0) return fail("invalid handle or step count");
 	double tt = h->engine.time();
-	for (int s = 0; s < nsteps; ++s) tt = h->engine.step();
+	try {
+		for (int s = 0; s < nsteps; ++s) tt = h->engine.step();
+	} catch (std::exception const& e) {
+		return fail(e.what());
+	}
 	if (t) *t = tt;
 	return 0;
 }
@@ -62,9 +66,56 @@ int ncpu_set_state(ncpu_handle* h, const double* pos, const double* vel, double 
 
 int ncpu_forces(ncpu_handle* h, double* acc, double* pot) {
 	if (!h) return fail("invalid handle");
-	h->engine.forces();
+	try {
+		h->engine.forces();
+	} catch (std::exception const& e) {
+		return fail(e.what());
+	}
 	if (acc) std::copy(h->engine.a().begin(), h->engine.a().end(), acc);
 	if (pot) std::copy(h->engine.pot().begin(), h->engine.pot().end(), pot);
+	return 0;
+}
+
+int ncpu_set_fmm(ncpu_handle* h, int order, int m2l_order, double theta, int leaf_capacity) {
+	if (!h) return fail("invalid handle");
+	try {
+		if (order == 0) {
+			h->engine.set_method(0);
+		} else {
+			nbody::FmmOptions o;
+			o.order = order;
+			o.m2l_order = m2l_order;
+			o.theta = theta;
+			o.leaf_capacity = leaf_capacity;
+			h->engine.set_method(1, o);
+		}
+	} catch (std::exception const& e) {
+		return fail(e.what());
+	}
+	return 0;
+}
+
+int ncpu_fmm_stats(ncpu_handle* h, long long* out6) {
+	if (!h || !out6) return fail("handle and output are required");
+	const nbody::CpuFmm* f = h->engine.fmm();
+	if (!f) return fail("the handle uses the direct sum");
+	const nbody::CpuFmm::Stats& s = f->stats();
+	out6[0] = (long long)s.nodes;
+	out6[1] = (long long)s.leaves;
+	out6[2] = (long long)s.depth;
+	out6[3] = (long long)s.m2l_unordered;
+	out6[4] = (long long)s.p2p_unordered;
+	out6[5] = (long long)s.visited;
+	return 0;
+}
+
+int ncpu_fmm_times(ncpu_handle* h, double* out6) {
+	if (!h || !out6) return fail("handle and output are required");
+	const nbody::CpuFmm* f = h->engine.fmm();
+	if (!f) return fail("the handle uses the direct sum");
+	const nbody::CpuFmm::Stats& s = f->stats();
+	const double t[6] = {s.t_tree, s.t_walk, s.t_up, s.t_m2l, s.t_down, s.t_leaves};
+	std::copy(t, t + 6, out6);
 	return 0;
 }
 

@@ -77,3 +77,21 @@ def test_cli_cpu_backend_and_viewer(tmp_path):
     png = tmp_path / "last.png"
     assert viewer.main([str(out), "--frame", "-1", "--save", str(png)]) == 0
     assert png.stat().st_size > 1000
+
+
+def test_cli_cpu_fmm_backend_follows_direct(tmp_path):
+    """--backend cpu-fmm (CpuSimulation's FMM) against --backend cpu (its
+    direct sum) on the same seeded ICs: the trajectories agree to the FMM's
+    force error over four steps."""
+    exe = os.path.join(os.path.dirname(nb._lib.LIB_PATH), "nbody_cli")
+    runs = {}
+    for backend in ("cpu", "cpu-fmm"):
+        out = tmp_path / f"{backend}.csv"
+        r = subprocess.run([exe, "--backend", backend, "--ic", "plummer", "--n", "300", "--dt", "0.001",
+                            "--t-end", "0.004", "--order", "6", "--csv", str(out)],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        runs[backend] = viewer.read_csv(str(out))
+    (td, Pd), (tf, Pf) = runs["cpu"], runs["cpu-fmm"]
+    assert np.allclose(td, tf) and Pd.shape == Pf.shape == (4, 300, 3)
+    assert np.max(np.abs(Pf - Pd)) < 1e-5
