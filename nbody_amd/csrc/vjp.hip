@@ -63,14 +63,17 @@ __global__ __launch_bounds__(256) void dipole_p2m_kernel(FmmParams F, const floa
 // leaf's P2P sources are flattened over its row segments, 64 at a time
 // staged in LDS (offset in the target leaf's frame, mass, cotangent), lanes
 // = (target particle) x (source stride), partial sums combined by shuffles.
+// As there, every lane runs the chunk's ceil(nchunk / splits) iterations and
+// the slots past the chunk hold zero mass and cotangent (index -1, skipped).
 constexpr int VJP_WAVES = 4;
+constexpr int VJP_STAGE = 96;  // splits * ceil(nchunk / splits) <= 84
 
 template <int P>
 __global__ __launch_bounds__(WAVE * VJP_WAVES) void vjp_staged_kernel(FmmParams F, const float4* pw, uint32_t num_leaves,
                                                                       double* gx, double* gy, double* gz, double* gm) {
-    __shared__ float4 stage_p[VJP_WAVES][WAVE];
-    __shared__ float4 stage_w[VJP_WAVES][WAVE];
-    __shared__ int stage_i[VJP_WAVES][WAVE];
+    __shared__ float4 stage_p[VJP_WAVES][VJP_STAGE];
+    __shared__ float4 stage_w[VJP_WAVES][VJP_STAGE];
+    __shared__ int stage_i[VJP_WAVES][VJP_STAGE];
     __shared__ int2 ent_s[VJP_WAVES][WAVE];      // list entry: first particle, flattened start
     __shared__ float4 entd_s[VJP_WAVES][WAVE];   // list entry: source centre - target centre
     __shared__ unsigned long long start_s[VJP_WAVES];  // entries starting in the chunk
@@ -81,6 +84,11 @@ __global__ __launch_bounds__(WAVE * VJP_WAVES) void vjp_staged_kernel(FmmParams 
     int* sidx = stage_i[wslot];
     int2* ent = ent_s[wslot];
     float4* entd = entd_s[wslot];
+    if (lane < VJP_STAGE - WAVE) {  // never staged: empty for good
+        sp4[WAVE + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        sw4[WAVE + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        sidx[WAVE + lane] = -1;
+    }
     const uint32_t nw = gridDim.x * VJP_WAVES;
     const float eps2 = F.eps2;
     for (uint32_t w = blockIdx.x * VJP_WAVES + wslot; w < num_leaves; w += nw) {
@@ -91,6 +99,8 @@ __global__ __launch_bounds__(WAVE * VJP_WAVES) void vjp_staged_kernel(FmmParams 
         for (int t0 = 0; t0 < la.y; t0 += WAVE) {
             const int nt = min(WAVE, la.y - t0);
             const int splits = WAVE / nt;
+            // ceil(x / splits) = umulhi(x + splits - 1, magic) for x <= 64
+            const uint32_t magic = splits == 1 ? 0u : 0xffffffffu / (uint32_t)splits + 1u;
             const int ti = lane % nt;
             const int sp = lane / nt;
             const int i = la.x + t0 + ti;
@@ -134,25 +144,35 @@ __global__ __launch_bounds__(WAVE * VJP_WAVES) void vjp_staged_kernel(FmmParams 
                         atomicOr(&start_s[wslot], 1ull << (excl - f0));
                     wave_lds_sync();
                     const unsigned long long starts = start_s[wslot];
-                    if (fidx < total) {
-                        const int k = kb0 + __popcll(starts & ((2ull << lane) - 1ull)) - 1;
-                        const int2 e = ent[k];
-                        const float4 dk = entd[k];
-                        const int src = e.x + (fidx - e.y);
-                        const float4 s = F.pf[src];
-                        sp4[lane] = make_float4(dk.x + s.x, dk.y + s.y, dk.z + s.z, s.w);
-                        sw4[lane] = pw[src];
+                    {
+                        float4 sv = make_float4(0.f, 0.f, 0.f, 0.f), wv = sv;  // past the sources: empty
+                        int src = -1;
+                        if (fidx < total) {
+                            const int k = kb0 + __popcll(starts & ((2ull << lane) - 1ull)) - 1;
+                            const int2 e = ent[k];
+                            const float4 dk = entd[k];
+                            src = e.x + (fidx - e.y);
+                            const float4 s = F.pf[src];
+                            sv = make_float4(dk.x + s.x, dk.y + s.y, dk.z + s.z, s.w);
+                            wv = pw[src];
+                        }
+                        sp4[lane] = sv;
+                        sw4[lane] = wv;
                         sidx[lane] = src;
                     }
                     wave_lds_sync();
                     const int nchunk = min(WAVE, total - f0);
+                    const int Tc = __builtin_amdgcn_readfirstlane(
+                        splits == 1 ? nchunk : (int)__umulhi((uint32_t)(nchunk + splits - 1), magic));
                     if (sp < splits) {
-                        for (int j = sp; j < nchunk; j += splits) {
+                        for (int kk = 0; kk < Tc; ++kk) {
+                            const int j = sp + kk * splits;
                             const float4 s = sp4[j];
                             const float4 wj = sw4[j];
                             const float rx = s.x - me.x, ry = s.y - me.y, rz = s.z - me.z;
                             const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, eps2)));
-                            const float inv = sidx[j] == i ? 0.f : rsqrtf(r2);
+                            const int sj = sidx[j];
+                            const float inv = (sj == i || sj < 0) ? 0.f : rsqrtf(r2);
                             const float i2 = inv * inv;
                             const float i3 = i2 * inv;
                             const float i5 = i3 * i2;
