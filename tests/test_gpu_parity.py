@@ -31,6 +31,10 @@ CASES = [
     ("plummer8k_cap8_p6_theta04", "plummer", 8192, 8, 6, (2, 5), 5e-4),
     ("uniform3000_cap1_p2", "uniform", 3000, 1, 2, (1, 2), 1e-2),
     ("uniform2048_cap16_unsoftened", "uniform", 2048, 16, 4, (1, 2), 0.0),
+    # P2P lane layouts at the ends: leaves of 33..48 particles (one source
+    # stride per target) and of 3 (21 strides, the most staged slots)
+    ("plummer6000_cap48", "plummer", 6000, 48, 4, (1, 2), 1e-3),
+    ("uniform2500_cap3", "uniform", 2500, 3, 4, (1, 2), 1e-2),
 ]
 
 
