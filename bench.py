@@ -292,6 +292,43 @@ def workload_adjoint(nb, args):
             "grad_norms": {"x0": float(np.linalg.norm(gx)), "v0": float(np.linalg.norm(gv)), "m": float(np.linalg.norm(gm))}}
 
 
+def workload_cpu_fmm(nb, args):
+    """The multi-core CPU comparator on the headline configuration: the FP64
+    OpenMP FMM of CpuSimulation (method "fmm", include/nbody/cpu_fmm.h) on the
+    same Plummer N=1,048,576 (seed 1), eps=1e-3, theta=0.5, p=4, cap 32, on
+    every OpenMP thread of the host; one force evaluation (the force work of
+    a KDK step) timed after one untimed, error on 4096 targets against the
+    GPU's FP64 direct sum."""
+    from nbody_amd.cpu import CpuSimulation
+    n = args.n
+    x, v, m = nb.ics.plummer(n, seed=1, a=1.0, M=1.0, G=1.0, rmax=20.0)
+    cpu = CpuSimulation.from_arrays(x, v, m, m, time_step=1e-3, G=1.0, eps=1e-3, method="fmm", order=args.order,
+                                    theta=0.5, leaf_capacity=32)
+    cpu.forces()
+    t0 = time.perf_counter()
+    acc, _ = cpu.forces()
+    wall = time.perf_counter() - t0
+    phases = cpu.fmm_times()
+    stats = cpu.fmm_stats()
+    cpu.close()
+    cfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, leaf_capacity=32, order=args.order)
+    sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
+    tg = np.sort(np.random.default_rng(4096).choice(n, size=min(4096, n), replace=False))
+    ad, _ = sim.direct_fp64(tg)
+    sim.close()
+    d = np.linalg.norm(acc[tg] - ad, axis=1)
+    ref = np.linalg.norm(ad, axis=1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    return {"metric": "particle-steps/s, N=1M Plummer, CPU FMM (FP64, OpenMP); force error vs FP64 direct",
+            "value": n / wall, "unit": "particle-steps/s", "n_gpus": 0, "cores": threads, "steps": 1, "warmup": 1,
+            "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp64", "data": "synthetic",
+            "config": {"workload": f"cpu_fmm_plummer_N{n}_seed1_eps1e-3_theta0.5_cap32_p{args.order}"},
+            "force_err": {"rms": float(np.sqrt(np.mean(d ** 2) / np.mean(ref ** 2))), "max": float(np.max(d / ref)),
+                          "targets": int(len(tg))},
+            "phases_s": phases, "tree": stats}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,7 +344,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
     ap.add_argument("--workload", default="plummer1m",
-                    choices=["plummer1m", "uniform4096", "clustered", "ensemble", "adjoint"],
+                    choices=["plummer1m", "uniform4096", "clustered", "ensemble", "adjoint", "cpu_fmm"],
                     help="plummer1m: the headline; the others print the line of another configuration")
     ap.add_argument("--no-plain-compare", action="store_true",
                     help="skip timing the plain FP32 port (per-target M2L threads, per-leaf P2P) beside the tuned path")
@@ -328,7 +365,7 @@ def main():
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib << 20)
     others = {"uniform4096": workload_uniform4096, "clustered": workload_clustered,
-              "ensemble": workload_ensemble, "adjoint": workload_adjoint}
+              "ensemble": workload_ensemble, "adjoint": workload_adjoint, "cpu_fmm": workload_cpu_fmm}
     if args.workload in others:
         print(json.dumps(others[args.workload](nb, args)), flush=True)
         return
