@@ -34,7 +34,10 @@
 
 #if defined(_OPENMP)
 #include <omp.h>
-#include <parallel/algorithm>
+#if defined(__GLIBCXX__)
+#include <parallel/algorithm>  // libstdc++'s parallel sort
+#define NBODY_FMM_PARALLEL_SORT 1
+#endif
 #endif
 
 namespace nbody {
@@ -371,7 +374,7 @@ private:
 			for (int a = 0; a < 3; ++a) c[a] = (uint32_t)std::min(std::floor((x[3 * i + a] - lo[a]) * inv), qmax);
 			kp[i] = std::make_pair((spread3(c[0]) << 2) | (spread3(c[1]) << 1) | spread3(c[2]), (uint32_t)i);
 		}
-#if defined(_OPENMP)
+#if defined(NBODY_FMM_PARALLEL_SORT)
 		__gnu_parallel::sort(kp.begin(), kp.end());
 #else
 		std::sort(kp.begin(), kp.end());
