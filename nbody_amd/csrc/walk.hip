@@ -151,6 +151,35 @@ __device__ __forceinline__ void enumerate_slice(const WalkParams& W, const SLCac
     }
 }
 
+// The call sequence of enumerate_slice, with the candidate ids only (no
+// geometry loads): f(phase, valid, B).  The validity tests read the cache
+// alone, so the calls match enumerate_slice's one for one.
+template <typename F>
+__device__ __forceinline__ void enumerate_slice_ids(const SLCache& cache, uint32_t lo, uint32_t hi, F&& f) {
+    const int lane = lane_id();
+    for (uint32_t e0 = lo; e0 < hi; e0 += WAVE) {
+        const uint32_t e = e0 + lane;
+        int B = -1;
+        bool valid = false;
+        if (e < hi) {
+            B = cache.node[e];
+            valid = B >= 0 && (cache.geo[e].w >> 8) == 0;
+        }
+        if (__ballot(valid) == 0) continue;
+        f(0, valid, B);
+    }
+    const int c = lane & 7;
+    for (uint32_t e0 = lo; e0 < hi; e0 += 16) {
+        bool v0 = false, v1 = false;
+        int C0 = -1, C1 = -1;
+        const uint32_t ea = e0 + (lane >> 3), eb = ea + 8;
+        if (ea < hi && c < (cache.geo[ea].w >> 8)) { C0 = cache.fc[ea] + c; v0 = true; }
+        if (eb < hi && c < (cache.geo[eb].w >> 8)) { C1 = cache.fc[eb] + c; v1 = true; }
+        if (__ballot(v0)) f(1, v0, C0);
+        if (__ballot(v1)) f(1, v1, C1);
+    }
+}
+
 // Take n entries from a wave's chunk {next, end}, reserving a new chunk of
 // max(n, chunk) entries from the pool when the current one is exhausted.
 // Returns the first index, or 0xffffffff when the pool is full.
@@ -219,6 +248,10 @@ __global__ __launch_bounds__(64) void walk_root_kernel(WalkParams W) {
 }
 
 constexpr int MAXT = 8;  // targets per group: the children of one cell
+// Calls of enumerate_slice per wavefront and group: a slice of at most
+// SL_CACHE / WALK_WAVES = 64 entries gives one chunk of kept entries and at
+// most 4 x 2 chunks of children.
+constexpr int CAT_CALLS = 1 + 2 * (SL_CACHE / WALK_WAVES / 16);
 
 // Depth s >= 1: one workgroup of WALK_WAVES wavefronts per group (row of
 // depth s-1).  The group's split list is cached once for the workgroup and
@@ -237,6 +270,8 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
     __shared__ int cfc[SL_CACHE];
     __shared__ uint32_t cnt_s[WALK_WAVES][2][4][MAXT];  // [wave][phase][same-level, other-level, P2P, split][target]
     __shared__ uint32_t bcast[8];                       // carried row, pool starts, failure
+    // pass 1's categories of a cached group, 3 bits per target, per call and lane
+    __shared__ uint32_t cat_s[WALK_WAVES][CAT_CALLS][WAVE];
     Counters* C = W.C;
     const uint32_t n_prev_level = C->level_count[s - 1];
     const uint32_t n_groups = n_prev_level + C->carried[s - 1];
@@ -361,10 +396,14 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
                     }
                 }
             };
-            // pass 1: counts of this wavefront's slice per phase, category, target (lane t)
+            // pass 1: counts of this wavefront's slice per phase, category, target
+            // (lane t); a cached group also keeps every call's categories for pass 2
             uint32_t k0m = 0, k0x = 0, k0p = 0, k0s = 0, k1m = 0, k1x = 0, k1p = 0, k1s = 0;
+            int ncall = 0;  // uniform
             each([&](int ph, bool valid, int B, int4 gb) {
+                uint32_t word = 0;
                 classify_all(valid, B, gb, [&](int t, int cat) {
+                    word |= (uint32_t)cat << (3 * t);
                     const uint32_t cm = __popcll(__ballot(cat == 0)), cx = __popcll(__ballot(cat == 4));
                     const uint32_t cp = __popcll(__ballot(cat == 1)), cs = __popcll(__ballot(cat == 2));
                     if (lane == t) {
@@ -372,6 +411,8 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
                         else { k1m += cm; k1x += cx; k1p += cp; k1s += cs; }
                     }
                 });
+                if (!solo && ncall < CAT_CALLS) cat_s[wv][ncall][lane] = word;
+                ++ncall;
             });
             if (lane < MAXT) {
                 cnt_s[wv][0][0][lane] = k0m; cnt_s[wv][0][1][lane] = k0x;
@@ -445,8 +486,7 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
             // pass 2: write this wavefront's entries
             uint32_t c0m = m0 + o0m, c0x = m0 + Tm + o0x, c0p = p0 + o0p, c0s = s0 + o0s;
             uint32_t c1m = m0 + o1m, c1x = m0 + Tm + o1x, c1p = p0 + o1p, c1s = s0 + o1s;
-            each([&](int ph, bool valid, int B, int4 gb) {
-                classify_all(valid, B, gb, [&](int t, int cat) {
+            auto write_entries = [&](int ph, int B, int t, int cat) {
                     const uint64_t bm = __ballot(cat == 0), bp = __ballot(cat == 1), bs = __ballot(cat == 2);
                     const uint64_t bx = __ballot(cat == 4);
                     const uint32_t cm = __builtin_amdgcn_readlane(ph == 0 ? c0m : c1m, t);
@@ -464,8 +504,23 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
                             c1m += __popcll(bm); c1x += __popcll(bx); c1p += __popcll(bp); c1s += __popcll(bs);
                         }
                     }
+            };
+            if (!solo && ncall <= CAT_CALLS) {
+                // the categories of pass 1, without the children's geometry or the MAC
+                int k = 0;
+                if (lo < hi)
+                    enumerate_slice_ids(cache, lo, hi, [&](int ph, bool valid, int B) {
+                        const uint32_t word = cat_s[wv][k][lane];
+                        ++k;
+#pragma unroll
+                        for (int t = 0; t < MAXT; ++t)
+                            if (t < nt) write_entries(ph, B, t, valid ? (int)((word >> (3 * t)) & 7u) : 3);
+                    });
+            } else {
+                each([&](int ph, bool valid, int B, int4 gb) {
+                    classify_all(valid, B, gb, [&](int t, int cat) { write_entries(ph, B, t, cat); });
                 });
-            });
+            }
         }
     }
     if (threadIdx.x == 0) {
