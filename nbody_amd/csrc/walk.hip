@@ -389,9 +389,12 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
                         const int64_t dy = cby - __builtin_amdgcn_readlane(cay_l, t);
                         const int64_t dz = cbz - __builtin_amdgcn_readlane(caz_l, t);
                         const uint64_t dsq = (uint64_t)(dx * dx + dy * dy + dz * dz);
-                        const bool approx = B != A0 + t && lhs < num4 * dsq;
-                        const bool both_leaf = __builtin_amdgcn_readlane(leaf_l, t) && b_leaf;
-                        const int cat = !valid ? 3 : approx ? (lb == lt_level ? 0 : 4) : both_leaf ? 1 : 2;
+                        // evaluated without short cuts: selects, not branches
+                        const bool approx = (B != A0 + t) & (lhs < num4 * dsq);
+                        const bool both_leaf = (__builtin_amdgcn_readlane(leaf_l, t) != 0) & b_leaf;
+                        const int near = both_leaf ? 1 : 2;
+                        const int far = lb == lt_level ? 0 : 4;
+                        const int cat = valid ? (approx ? far : near) : 3;
                         per_target(t, cat);
                     }
                 }
