@@ -1,7 +1,7 @@
 // naive_ref_shim.cpp -- C entry point around the reference's NaiveSimulation,
 // compiled together with /root/reference/src/naive_simulation.cpp into
-// oracle/_ref/libnaive_ref.so (see oracle/Makefile).  TEST/BASELINE ONLY: used
-// to write golden vectors (tests/golden/) and as the timed CPU baseline.
+// oracle/_ref/libnaive_ref.so (see oracle/Makefile).  Test infrastructure only:
+// used to write golden vectors (tests/golden/) and as the timed CPU baseline.
 //
 // Particle layout of the flat float array (12 floats per particle) mirrors
 // include/nbody/simulation.h:16-32 with vector_t = cl_float4
