@@ -194,25 +194,36 @@ def workload_uniform4096(nb, args):
 def workload_clustered(nb, args):
     """Two Plummer spheres of 4,194,304 particles (seeds 2, 3) at (+-5, 0, 0)
     approaching at 0.3 each; eps=5e-4, theta=0.4, p=6 (M2L |k|+|n| <= 6),
-    leaf capacity 32, depth <= 21; 1 force evaluation (error on 8192 targets)
-    plus 5 KDK steps at dt=5e-4."""
+    leaf capacity 32, depth <= 21; per M2L precision (FP32 on the VALU, FP32
+    and bf16x3 products on MFMA for the same-level pairs), 1 force evaluation
+    (error on 8192 targets) plus 5 KDK steps at dt=5e-4.  `value` is the
+    precision chosen by --m2l (default: FP32 VALU)."""
     n_each = args.n // 2 if args.n != 1 << 20 else 4194304
     x, v, m = nb.ics.clustered_pair(n_each)
     n = len(m)
     m = m / m.sum()
-    cfg = nb.default_config(G=1.0, eps=5e-4, theta=0.4, dt=5e-4, leaf_capacity=32, order=6, m2l_order=6,
-                            max_depth=21)
-    sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
-    err = force_error(sim, n, 8192, 8192)
     K = 5
-    agg, wall, inter, m2l_pairs, split_pairs, p2p_pairs, st = time_steps(sim, 1, K)
-    sim.close()
+    precs = [("fp32_valu", nb.M2L_GROUPED), ("fp32_mfma", nb.M2L_MFMA_F32), ("bf16x3_mfma", nb.M2L_BF16X3)]
+    sel = {"mfma_f32": "fp32_mfma", "bf16x3": "bf16x3_mfma"}.get(args.m2l, "fp32_valu")
+    errs, ms_by = {}, {}
+    for name, kind in precs:
+        print(f"[clustered] M2L {name}", file=sys.stderr, flush=True)
+        cfg = nb.default_config(G=1.0, eps=5e-4, theta=0.4, dt=5e-4, leaf_capacity=32, order=6, m2l_order=6,
+                                max_depth=21, m2l=kind)
+        sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
+        errs[name] = force_error(sim, n, 8192, 8192)
+        res = time_steps(sim, 1, K)
+        ms_by[name] = res[1] / K * 1e3
+        if name == sel:
+            agg, wall, inter, m2l_pairs, split_pairs, p2p_pairs, st = res
+        sim.close()
     return {"metric": "particle-steps/s, N=8,388,608 clustered pair, theta=0.4, p=6; force error vs FP64 direct",
             "value": n * K / wall, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": 1,
             "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "fp32", "data": "synthetic",
             "config": {"workload": f"clustered_pair_2x{n_each}_seeds2_3_eps5e-4_theta0.4_cap32_p6_q6_kdk_dt5e-4"},
-            "force_err": {"fp32": err}, "phases_ms": {k[3:]: agg[k] / K for k in agg},
+            "m2l_precision": sel, "force_err": errs, "ms_per_step_by_m2l": ms_by,
+            "phases_ms": {k[3:]: agg[k] / K for k in agg},
             "tree": {"nodes": st["num_nodes"], "leaves": st["num_leaves"], "depth": st["depth"]},
             "m2l_pairs_per_step": m2l_pairs / K, "p2p_interactions_per_step": inter / K}
 
@@ -335,7 +346,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--m2l", default="grouped", choices=["plain", "grouped", "bf16x3"])
+    ap.add_argument("--m2l", default="grouped", choices=["plain", "grouped", "bf16x3", "mfma_f32"])
     ap.add_argument("--p2p", default="lane", choices=["leaf", "lane"])
     ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--m2l-order", type=int, default=0, help="M2L truncation |k|+|n| <= q (0: order + 1)")
@@ -369,7 +380,7 @@ def main():
     if args.workload in others:
         print(json.dumps(others[args.workload](nb, args)), flush=True)
         return
-    m2l = {"plain": nb.M2L_PLAIN, "grouped": nb.M2L_GROUPED, "bf16x3": nb.M2L_BF16X3}[args.m2l]
+    m2l = {"plain": nb.M2L_PLAIN, "grouped": nb.M2L_GROUPED, "bf16x3": nb.M2L_BF16X3, "mfma_f32": nb.M2L_MFMA_F32}[args.m2l]
     p2p = {"leaf": nb.P2P_LEAF, "lane": nb.P2P_LANE}[args.p2p]
     n = args.n
     p = args.order

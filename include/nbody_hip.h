@@ -43,8 +43,9 @@ extern "C" {
 
 /* far-field (M2L) kernel */
 #define NBH_M2L_PLAIN 0        /* per-pair FP32 VALU, derivative tensor per pair */
-#define NBH_M2L_GROUPED 1      /* FP32, grouped by translation offset */
-#define NBH_M2L_BF16X3 2       /* grouped, MFMA bf16x3 split products, FP32 accumulate */
+#define NBH_M2L_GROUPED 1      /* FP32 VALU, grouped by translation offset */
+#define NBH_M2L_BF16X3 2       /* grouped; same-level pairs on MFMA, bf16x3 split products, FP32 accumulate */
+#define NBH_M2L_MFMA_F32 3     /* grouped; same-level pairs on MFMA, FP32 products */
 
 /* near-field (P2P) kernel */
 #define NBH_P2P_LEAF 0         /* one wavefront per target leaf (plain) */

@@ -19,7 +19,7 @@ NBH_E_NONFINITE = -4
 NBH_E_INTERNAL = -5
 
 SOLVER_FMM, SOLVER_DIRECT = 0, 1
-M2L_PLAIN, M2L_GROUPED, M2L_BF16X3 = 0, 1, 2
+M2L_PLAIN, M2L_GROUPED, M2L_BF16X3, M2L_MFMA_F32 = 0, 1, 2, 3
 P2P_LEAF, P2P_LANE = 0, 1
 INT_KDK, INT_EULER, INT_KICK_DRIFT = 0, 1, 2
 LAW_SOFTENED, LAW_COMPAT_NAIVE = 0, 1

@@ -147,6 +147,7 @@ void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, double ki
 // terms of degree p + 1 (the expansions themselves stay of order p).
 struct M2LPlan {
     int mode;  // 0: plain (one thread per target, tensor per pair); 1: grouped
+    int mfma;  // grouped, same-level pairs: 0 FP32 VALU; 1 MFMA bf16x3; 2 MFMA FP32
     uint32_t num_nodes;
     uint32_t level_off[MAX_LEVELS + 2];
     uint32_t level_count[MAX_LEVELS + 1];

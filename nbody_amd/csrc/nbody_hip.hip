@@ -207,7 +207,7 @@ int validate_config(const nbh_config* c, std::string& why) {
             return NBH_E_INVALID;
         }
         if (c->max_depth > 21) { why = "max_depth must be <= 21"; return NBH_E_INVALID; }
-        if (c->m2l > NBH_M2L_BF16X3) { why = "unknown m2l kernel"; return NBH_E_INVALID; }
+        if (c->m2l > NBH_M2L_MFMA_F32) { why = "unknown m2l kernel"; return NBH_E_INVALID; }
         if (c->p2p > NBH_P2P_LANE) { why = "unknown p2p kernel"; return NBH_E_INVALID; }
     }
     if (c->fixed_root && !(c->root_size > 0)) { why = "fixed_root needs root_size > 0"; return NBH_E_INVALID; }
@@ -491,6 +491,7 @@ static int m2l_q(const nbh_handle* h) {
 static M2LPlan m2l_plan(const nbh_handle* h) {
     M2LPlan pl{};
     pl.mode = h->cfg.m2l == NBH_M2L_PLAIN ? 0 : 1;
+    pl.mfma = h->cfg.m2l == NBH_M2L_BF16X3 ? 1 : (h->cfg.m2l == NBH_M2L_MFMA_F32 ? 2 : 0);
     pl.num_nodes = h->num_nodes;
     pl.max_lds = (size_t)h->max_lds;
     std::copy(h->hc->level_off, h->hc->level_off + MAX_LEVELS + 2, pl.level_off);
