@@ -663,7 +663,7 @@ template <int P, int Q>
 constexpr size_t mfma_g_bytes() { return (size_t)(ncoef(P) + 15) / 16 * 16 * mfma_g_stride<P, Q>() * 4; }
 
 template <int P, int Q, bool BF3>
-__global__ __launch_bounds__(256) void m2l_mfma_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 2 : 1, 2))) void m2l_mfma_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
     constexpr int NC = ncoef(P), NCQ = ncoef(Q);
     constexpr int TN = (NC + 15) / 16, TM = (NCQ + 15) / 16;
     constexpr int GS = mfma_g_stride<P, Q>();
@@ -691,17 +691,27 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(FmmParams F, uint32_t off
         const uint32_t row = F.lists.first[A];
         const uint32_t s0 = F.lists.m_start[row];
         const uint32_t e1 = s0 + F.lists.m_cnt[row] - F.lists.mx_cnt[row];
+        // lane l's pair of each batch of 64: source B and (tensor row offset
+        // << 3 | octant) rn, -1 past the end.  Loaded ahead: the sources two
+        // batches ahead and the geometry one batch ahead, so the dependent
+        // list -> geometry loads overlap the products of the current batch.
+        auto pair_code = [&](int Bc, const int4& gb) {
+            const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
+            const int neg = (ox < 0) | ((oy < 0) << 1) | ((oz < 0) << 2);
+            return ((((abs(ox) * K1 + abs(oy)) * K1 + abs(oz)) * F.dcp) << 3) | neg;
+        };
+        int B = 0, rn = -1, Bn = 0;
+        if (s0 + lane < e1) {
+            B = F.lists.m_pool[s0 + lane];
+            rn = pair_code(B, F.nodes.geo[B]);
+        }
+        if (s0 + WAVE + lane < e1) Bn = F.lists.m_pool[s0 + WAVE + lane];
         for (uint32_t base = s0; base < e1; base += WAVE) {
-            // this lane's pair of the batch: source and (tensor row, octant)
-            const uint32_t e = base + lane;
-            int B = 0, rn = -1;
-            if (e < e1) {
-                B = F.lists.m_pool[e];
-                const int4 gb = F.nodes.geo[B];
-                const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
-                const int neg = (ox < 0) | ((oy < 0) << 1) | ((oz < 0) << 2);
-                rn = ((((abs(ox) * K1 + abs(oy)) * K1 + abs(oz)) * F.dcp) << 3) | neg;
-            }
+            const bool has_n = base + WAVE + lane < e1;
+            int4 gbn = make_int4(0, 0, 0, 0);
+            if (has_n) gbn = F.nodes.geo[Bn];
+            int Bnn = 0;
+            if (base + 2 * WAVE + lane < e1) Bnn = F.lists.m_pool[base + 2 * WAVE + lane];
             const uint32_t nb = min(e1 - base, (uint32_t)WAVE);
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
@@ -765,6 +775,9 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(FmmParams F, uint32_t off
                                                                                   0, 0);
                 }
             }
+            rn = has_n ? pair_code(Bn, gbn) : -1;
+            B = Bn;
+            Bn = Bnn;
         }
         // G[n][m] to LDS (rows n, stride GS), then L_k for k = lane
 #pragma unroll
