@@ -46,14 +46,14 @@ def _ics(kind, n, seed=0):
 
 # name, M2L kernel, P2P kernel, M2L truncation (0: |k|+|n| <= p+1, the default; -1: <= p),
 # RMS tolerances (acc, pot) against the FP64 oracle FMM.  The MFMA kernels take
-# the same-level pairs: FP32 products accumulate in another order (FP32
-# tolerances); bf16x3 drops the lo*lo term and rounds both halves to bf16, about
-# 2^-16 relative per product, which is the scale of its tolerances.
+# the same-level pairs and are held to the FP32 tolerances: bf16x3 drops the
+# lo*lo term and rounds both halves to bf16 (about 2^-16 relative per product),
+# which the far field's share of each force keeps below them.
 KERNELS = [("tuned", nb.M2L_GROUPED, nb.P2P_LANE, 0, 2e-5, 2e-6),
            ("tuned_qp", nb.M2L_GROUPED, nb.P2P_LANE, -1, 2e-5, 2e-6),
            ("plain", nb.M2L_PLAIN, nb.P2P_LEAF, 0, 2e-5, 2e-6),
            ("mfma_f32", nb.M2L_MFMA_F32, nb.P2P_LANE, 0, 2e-5, 2e-6),
-           ("mfma_bf16x3", nb.M2L_BF16X3, nb.P2P_LANE, 0, 4e-5, 1.5e-5)]
+           ("mfma_bf16x3", nb.M2L_BF16X3, nb.P2P_LANE, 0, 2e-5, 2e-6)]
 
 
 @pytest.mark.parametrize("kern,m2l,p2p,qsel,tol_acc,tol_pot", KERNELS)
