@@ -663,7 +663,7 @@ template <int P, int Q>
 constexpr size_t mfma_g_bytes() { return (size_t)(ncoef(P) + 15) / 16 * 16 * mfma_g_stride<P, Q>() * 4; }
 
 template <int P, int Q, bool BF3>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 2 : 1, 2))) void m2l_mfma_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 3 : 1, 3))) void m2l_mfma_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
     constexpr int NC = ncoef(P), NCQ = ncoef(Q);
     constexpr int TN = (NC + 15) / 16, TM = (NCQ + 15) / 16;
     constexpr int GS = mfma_g_stride<P, Q>();

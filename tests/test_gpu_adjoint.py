@@ -23,12 +23,16 @@ def _case(name):
     print(f"[case] {name}", flush=True)
 
 
-@pytest.mark.parametrize("kind,n,cap,p", [("plummer", 6000, 16, 4), ("uniform", 3000, 8, 3)])
-def test_force_vjp_matches_oracle_transpose(kind, n, cap, p):
-    _case(f"force_vjp_{kind}{n}_p{p}")
+@pytest.mark.parametrize("kind,n,cap,p,m2l", [("plummer", 6000, 16, 4, nb.M2L_GROUPED),
+                                              ("uniform", 3000, 8, 3, nb.M2L_GROUPED),
+                                              ("plummer", 6000, 16, 4, nb.M2L_BF16X3)])
+def test_force_vjp_matches_oracle_transpose(kind, n, cap, p, m2l):
+    """The transposed far-field chain runs the handle's M2L kernels too (with
+    M2L_BF16X3 its same-level pairs on the matrix cores)."""
+    _case(f"force_vjp_{kind}{n}_p{p}_m2l{m2l}")
     x, v, m = nb.ics.plummer(n, 5) if kind == "plummer" else nb.ics.uniform(n, 5)
     eps = 1e-2
-    sim = make_sim(x, v, m, leaf_capacity=cap, order=p, eps=eps)
+    sim = make_sim(x, v, m, leaf_capacity=cap, order=p, eps=eps, m2l=m2l)
     w = np.random.default_rng(6).standard_normal((n, 3))
     gx, gm = sim.force_vjp(w)
     keys, ids, box = sim.debug_keys()
