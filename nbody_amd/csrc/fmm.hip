@@ -662,6 +662,8 @@ constexpr int mfma_g_stride() { return (ncoef(Q) + 15) / 16 * 16 + 1; }
 template <int P, int Q>
 constexpr size_t mfma_g_bytes() { return (size_t)(ncoef(P) + 15) / 16 * 16 * mfma_g_stride<P, Q>() * 4; }
 
+// The kernel waits on its loads more than on either pipe: at p <= 4 three
+// wavefronts per SIMD (168 registers, a few spilled) beat two without spills.
 template <int P, int Q, bool BF3>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 3 : 1, 3))) void m2l_mfma_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
     constexpr int NC = ncoef(P), NCQ = ncoef(Q);
