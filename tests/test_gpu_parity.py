@@ -53,7 +53,8 @@ KERNELS = [("tuned", nb.M2L_GROUPED, nb.P2P_LANE, 0, 2e-5, 2e-6),
            ("tuned_qp", nb.M2L_GROUPED, nb.P2P_LANE, -1, 2e-5, 2e-6),
            ("plain", nb.M2L_PLAIN, nb.P2P_LEAF, 0, 2e-5, 2e-6),
            ("mfma_f32", nb.M2L_MFMA_F32, nb.P2P_LANE, 0, 2e-5, 2e-6),
-           ("mfma_bf16x3", nb.M2L_BF16X3, nb.P2P_LANE, 0, 2e-5, 2e-6)]
+           ("mfma_bf16x3", nb.M2L_BF16X3, nb.P2P_LANE, 0, 2e-5, 2e-6),
+           ("mfma_bf16x3_qp", nb.M2L_BF16X3, nb.P2P_LANE, -1, 2e-5, 2e-6)]
 
 
 @pytest.mark.parametrize("kern,m2l,p2p,qsel,tol_acc,tol_pot", KERNELS)
