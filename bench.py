@@ -34,10 +34,12 @@ P2P_FLOP = 20              # per directed particle-particle interaction
 
 
 def m2l_flop(p, q):
-    """2 x #{(k, n): |k|, |n| <= p, |k| + |n| <= q} per directed cell pair
-    (420 for p = q = 4, 840 for p = 4, q = 5)."""
+    """2 x #{kept (k, n): |k|, |n| <= p and |k| + |n| <= p, or |k| + |n| <= q
+    with min(|k|, |n|) <= q - p} per directed cell pair (420 for p = q = 4,
+    600 for p = 4, q = 5)."""
     tri = lambda d: (d + 1) * (d + 2) // 2
-    return 2 * sum(tri(a) * tri(b) for a in range(p + 1) for b in range(p + 1) if a + b <= q)
+    keep = lambda a, b: a + b <= p or (a + b <= q and min(a, b) <= q - p)
+    return 2 * sum(tri(a) * tri(b) for a in range(p + 1) for b in range(p + 1) if keep(a, b))
 
 
 def ncoef(p):

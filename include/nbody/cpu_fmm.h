@@ -13,7 +13,8 @@
 //     (interaction.cl:63-79, NODE_APPROX_RATIO interaction.cl:3-5) evaluated
 //     exactly in integers for theta = num/den;
 //   * Cartesian Taylor expansions of order p about the cell centres: P2M
-//     (moment.cl), M2M, M2L truncated at |k| + |n| <= q, L2L, L2P, and the
+//     (moment.cl), M2M, M2L (terms of total degree <= p, and up to degree q
+//     those with min(|k|, |n|) <= q - p), L2L, L2P, and the
 //     softened pair sum of field.cl:17-32 for the near field.
 // Everything is FP64.  The tree and the interaction lists are the HIP
 // library's for the same positions; forces differ from it by its FP32
@@ -44,7 +45,8 @@ namespace nbody {
 
 struct FmmOptions {
 	int order = 4;           // Taylor order p, 1..8
-	int m2l_order = 0;       // M2L truncation q, p <= q <= 2p (0: p + 1)
+	int m2l_order = 0;       // M2L truncation q, p <= q <= 2p (0: p + 1): terms of degree <= p,
+	                         // and up to degree q those with min(|k|, |n|) <= q - p
 	double theta = 0.5;      // opening ratio; must equal num/den with num, den <= 64
 	int leaf_capacity = 32;  // particles per leaf cell
 };
@@ -138,7 +140,7 @@ struct Tables {
 		for (int k = 0; k < nc; ++k) {
 			m2l_kb.push_back((int)m2l_k.size());
 			for (int n = 0; n < nc; ++n)
-				if (deg[k] + deg[n] <= q) {
+				if (deg[k] + deg[n] <= p || (deg[k] + deg[n] <= q && std::min(deg[k], deg[n]) <= q - p)) {
 					m2l_k.push_back(k);
 					m2l_n.push_back(n);
 					m2l_s.push_back(mindex(mx[k] + mx[n], my[k] + my[n], mz[k] + mz[n]));

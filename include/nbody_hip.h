@@ -80,8 +80,9 @@ typedef struct nbh_config {
     double root_lo[3];
     double root_size;
     uint32_t p2p;           /* NBH_P2P_* */
-    uint32_t m2l_order;     /* M2L keeps |k| + |n| <= m2l_order (k, n of order <= p): p, or
-                               p + 1 to add the cross terms of degree p + 1; 0 = p + 1 */
+    uint32_t m2l_order;     /* M2L truncation q (k, n of order <= p): p keeps |k| + |n| <= p;
+                               p + 1 adds the degree-(p + 1) terms with |n| = 1 or |k| = 1;
+                               0 = p + 1 */
 } nbh_config;
 
 typedef struct nbh_handle nbh_handle;

@@ -638,12 +638,12 @@ __device__ __forceinline__ int coef_index(int x, int y, int z) {
     const int d = x + y + z, a = d - x;
     return d * (d + 1) * (d + 2) / 6 + a * (a + 1) / 2 + z;
 }
-// L_k += sum over the order-P n with |k| + |n| <= Q of (-1)^|n| G[n][k + n]
+// L_k += sum over the order-P n of the kept terms (m2l_keeps) of (-1)^|n| G[n][k + n]
 template <int P, int Q, int N>
 __device__ __forceinline__ void gather_step(const float* G, int gs, int kx, int ky, int kz, int kd, float& L) {
     constexpr auto& T = Tab<P>::t;
     constexpr int nx = T.mx[N], ny = T.my[N], nz = T.mz[N], nd = T.deg[N];
-    if (kd + nd <= Q) {
+    if (m2l_keeps(P, Q, kd, nd)) {
         const float v = G[N * gs + coef_index(kx + nx, ky + ny, kz + nz)];
         L += (nd & 1) ? -v : v;
     }

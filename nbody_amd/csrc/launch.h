@@ -143,8 +143,8 @@ void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
 void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
 void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st);
-// M2L of order (p, q): q = p keeps |k| + |n| <= p, q = p + 1 also the cross
-// terms of degree p + 1 (the expansions themselves stay of order p).
+// M2L of order (p, q): q = p keeps |k| + |n| <= p, q = p + 1 also the terms
+// of degree p + 1 with |n| = 1 or |k| = 1 (the expansions stay of order p).
 struct M2LPlan {
     int mode;  // 0: plain (one thread per target, tensor per pair); 1: grouped
     int mfma;  // grouped, same-level pairs: 0 FP32 VALU; 1 MFMA bf16x3; 2 MFMA FP32

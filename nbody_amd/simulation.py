@@ -90,7 +90,8 @@ class HipSimulation(Simulation):
         mass = _c64(mass)
         charge = None if charge is None else _c64(charge)
         self.config = cfg
-        # M2L truncation |k| + |n| <= m2l_order (0 selects order + 1)
+        # M2L truncation q = m2l_order (0 selects order + 1): terms of degree <= p,
+        # and up to degree q those with min(|k|, |n|) <= q - p
         self.m2l_order = int(cfg.m2l_order) or int(cfg.order) + 1
         h = ctypes.c_void_p()
         L = lib()

@@ -480,9 +480,12 @@ void orc_m2l(int p, int q, const double* zA, const double* zB, double eps, const
         double acc = 0;
         for (int n = 0; n < T.size(); ++n) {
             const auto& nn = T.mi[n];
+            const int dk = kk[0] + kk[1] + kk[2], dn = nn[0] + nn[1] + nn[2];
+            // beyond total degree p only the terms with min(|k|, |n|) <= q - p
+            if (dk + dn > p && std::min(dk, dn) > q - p) continue;
             int s = TQ.at(kk[0] + nn[0], kk[1] + nn[1], kk[2] + nn[2]);
             if (s < 0) continue;
-            double sign = ((nn[0] + nn[1] + nn[2]) & 1) ? -1.0 : 1.0;
+            double sign = (dn & 1) ? -1.0 : 1.0;
             acc += sign * D[s] * M[n];
         }
         L[k] += acc;

@@ -91,8 +91,10 @@ void orc_derivs(int p, const double* R, double eps, double* D);
 void orc_p2m(int p, const double* z, size_t n, const double* x, const double* q, double* M);
 /* shift multipole from child centre zc to parent centre zp */
 void orc_m2m(int p, const double* zc, const double* zp, const double* Mc, double* Mp);
-/* L_k += sum_{|k|,|n|<=p, |k|+|n|<=q} (-1)^|n| D_{k+n}(zA - zB) M_n  (q >= p;
- * q == p is the usual total-degree truncation) */
+/* L_k += sum (-1)^|n| D_{k+n}(zA - zB) M_n over |k|,|n| <= p with |k|+|n| <= p,
+ * or |k|+|n| <= q and min(|k|,|n|) <= q-p (q >= p).  q == p is the usual
+ * total-degree truncation; q == p+1 adds the degree-(p+1) terms with |n| = 1 or
+ * |k| = 1 (a term set symmetric in (k, n)). */
 void orc_m2l(int p, int q, const double* zA, const double* zB, double eps, const double* M, double* L);
 /* shift local expansion from parent zp to child zc (accumulates into Lc) */
 void orc_l2l(int p, const double* zp, const double* zc, const double* Lp, double* Lc);

@@ -213,7 +213,8 @@ def m2m(p, zc, zp, Mc):
 
 
 def m2l(p, zA, zB, M, eps=0.0, m2l_order=None):
-    """m2l_order q: keep |k| + |n| <= q (default p)."""
+    """m2l_order q (default p): keep |k| + |n| <= p, and up to |k| + |n| <= q
+    the terms with min(|k|, |n|) <= q - p."""
     zA = c64(zA); zB = c64(zB); M = c64(M)
     L = np.zeros(ncoef(p))
     q = p if m2l_order is None else int(m2l_order)

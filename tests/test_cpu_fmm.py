@@ -85,7 +85,7 @@ def test_converges_to_direct_sum():
         a, ph = sim.forces()
         errs.append(np.sqrt(np.mean((np.linalg.norm(a - ad, axis=1) / np.linalg.norm(ad, axis=1)) ** 2)))
         sim.close()
-    assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-5, errs
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 5e-5, errs
 
 
 @pytest.mark.parametrize("n", [1, 2, 20])

@@ -397,21 +397,38 @@ def test_euler_and_kick_drift_one_step_closed_form():
 
 
 def test_m2l_extended_truncation_is_the_higher_order_operator_restricted():
-    """orc_m2l(p, q=p+1) keeps |k|, |n| <= p with |k|+|n| <= p+1: it must equal
-    the order-(p+1) M2L applied to M padded with zeros above order p, read at
-    |k| <= p (the order-(p+1) operator is pinned by the sympy derivative test
-    and the convergence tests above)."""
+    """orc_m2l(p, q=p+1) keeps the terms of total degree <= p and those of
+    degree p+1 with |n| = 1 or |k| = 1.  So, with the order-(p+1) M2L applied to
+    M padded with zeros (read at |k| <= p): local orders |k| <= 1 receive every
+    moment and equal it; on a monopole + dipole every local order equals it; and
+    on moments of order >= 2 the orders |k| >= 2 equal the total-degree
+    truncation q = p.  (The order-(p+1) operator is pinned by the sympy
+    derivative test and the convergence tests above.)"""
     rng = np.random.default_rng(21)
     zA, zB = np.array([0.3, -0.2, 0.1]), np.array([2.1, 0.7, -1.4])
     for p in [1, 2, 3, 4, 5]:
-        M = rng.standard_normal(po.ncoef(p))
-        Mpad = np.zeros(po.ncoef(p + 1))
-        Mpad[:po.ncoef(p)] = M
+        nc = po.ncoef(p)
+        M = rng.standard_normal(nc)
+        low = np.zeros(nc)
+        low[:4] = M[:4]            # monopole and dipole
+        high = M - low             # orders >= 2
+
+        def full(Mx):
+            pad = np.zeros(po.ncoef(p + 1))
+            pad[:nc] = Mx
+            return po.m2l(p + 1, zA, zB, pad, eps=0.03)[:nc]
+
         ext = po.m2l(p, zA, zB, M, eps=0.03, m2l_order=p + 1)
-        ref = po.m2l(p + 1, zA, zB, Mpad, eps=0.03)[:po.ncoef(p)]
-        assert np.allclose(ext, ref, rtol=1e-13, atol=1e-15 * np.abs(ref).max())
-        # and it differs from the total-degree truncation exactly by the
-        # degree-(p+1) cross terms (nonzero whenever p >= 1)
+        ref = full(M)
+        tol = 1e-13 * np.abs(ref).max()
+        assert np.allclose(ext[:4], ref[:4], rtol=1e-13, atol=tol)
+        ext_low = po.m2l(p, zA, zB, low, eps=0.03, m2l_order=p + 1)
+        assert np.allclose(ext_low, full(low), rtol=1e-13, atol=tol)
+        ext_high = po.m2l(p, zA, zB, high, eps=0.03, m2l_order=p + 1)
+        tot_high = po.m2l(p, zA, zB, high, eps=0.03)
+        assert np.allclose(ext_high[4:], tot_high[4:], rtol=1e-13, atol=tol)
+        assert np.allclose(ext, ext_low + ext_high, rtol=1e-12, atol=tol)
+        # and it differs from the total-degree truncation (p >= 1)
         assert not np.allclose(ext, po.m2l(p, zA, zB, M, eps=0.03))
 
 
