@@ -44,7 +44,7 @@ def _ics(kind, n, seed=0):
     return nb.ics.plummer(n, seed + 1)
 
 
-# name, M2L kernel, P2P kernel, M2L truncation (0: |k|+|n| <= p+1, the default; -1: <= p),
+# name, M2L kernel, P2P kernel, M2L truncation (0: q = p+1, the default; -1: q = p),
 # RMS tolerances (acc, pot) against the FP64 oracle FMM.  The MFMA kernels take
 # the same-level pairs and are held to the FP32 tolerances: bf16x3 drops the
 # lo*lo term and rounds both halves to bf16 (about 2^-16 relative per product),
