@@ -483,3 +483,29 @@ def test_fmm_vjp_is_the_exact_transpose_of_the_fmm(ext):
     gxd, gmd = po.direct_vjp(xs, ms, w, eps=eps)
     assert np.sqrt(np.sum((gx - gxd) ** 2) / np.sum(gxd ** 2)) < 2e-3
     assert np.sqrt(np.sum((gm - gmd) ** 2) / np.sum(gmd ** 2)) < 2e-3
+
+
+def test_m2l_term_set_is_symmetric_with_the_counted_size():
+    """The kept (k, n) terms of orc_m2l, probed column by column with unit
+    moments: 210 at p = q = 4 and 300 at p = 4, q = 5 (the count bench.py turns
+    into FLOP per pair), and the pattern is symmetric in (k, n), which is what
+    lets the adjoint reuse the same operator."""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        "bench_mod", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    zA, zB = np.array([0.3, -0.2, 0.1]), np.array([2.1, 0.7, -1.4])
+    p = 4
+    nc = po.ncoef(p)
+    for q, count in [(4, 210), (5, 300)]:
+        T = np.zeros((nc, nc))
+        for n in range(nc):
+            e = np.zeros(nc)
+            e[n] = 1.0
+            T[:, n] = po.m2l(p, zA, zB, e, eps=0.03, m2l_order=q)
+        nz = np.abs(T) > 0
+        assert int(nz.sum()) == count
+        assert np.array_equal(nz, nz.T)
+        assert bench.m2l_flop(p, q) == 2 * count
