@@ -72,7 +72,11 @@ public:
 		_log << "HipSimulation: " << _n << " particles on HIP device " << config.device << ".\n";
 	}
 
-	~HipSimulation() override { nbh_destroy(_h); }
+	// No `override`: when the reference's include/ comes first on the include
+	// path, "nbody/simulation.h" is the reference's header, whose Simulation
+	// has no virtual destructor (simulation.h:6-37).  Over this repository's
+	// header the destructor is virtual all the same.
+	~HipSimulation() { nbh_destroy(_h); }
 	HipSimulation(HipSimulation const&) = delete;
 	HipSimulation& operator=(HipSimulation const&) = delete;
 

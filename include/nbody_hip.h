@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define NBH_ABI_VERSION 1
+#define NBH_ABI_VERSION 2
 
 /* status codes */
 #define NBH_OK 0
@@ -36,6 +36,7 @@ extern "C" {
 #define NBH_E_NOMEM (-3)       /* device allocation failed */
 #define NBH_E_NONFINITE (-4)   /* NaN/Inf in the particle state */
 #define NBH_E_INTERNAL (-5)    /* an internal consistency check failed */
+#define NBH_E_OUTSIDE (-6)     /* fixed_root: a particle lies outside the root box */
 
 /* solver */
 #define NBH_SOLVER_FMM 0       /* replaces OpenClSimulation's pipeline */
@@ -76,7 +77,8 @@ typedef struct nbh_config {
     uint32_t integrator;    /* NBH_INT_* */
     uint32_t force_law;     /* NBH_LAW_* (direct solver) */
     int32_t device;         /* HIP device ordinal */
-    uint32_t fixed_root;    /* 0: root cube = bounding cube each step; 1: [root_lo, root_lo+root_size) */
+    uint32_t fixed_root;    /* 0: root cube = bounding cube each step; 1: [root_lo, root_lo+root_size]
+                               (a particle outside it fails with NBH_E_OUTSIDE) */
     double root_lo[3];
     double root_size;
     uint32_t p2p;           /* NBH_P2P_* */

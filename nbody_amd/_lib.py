@@ -17,6 +17,7 @@ NBH_E_HIP = -2
 NBH_E_NOMEM = -3
 NBH_E_NONFINITE = -4
 NBH_E_INTERNAL = -5
+NBH_E_OUTSIDE = -6
 
 SOLVER_FMM, SOLVER_DIRECT = 0, 1
 M2L_PLAIN, M2L_GROUPED, M2L_BF16X3, M2L_MFMA_F32 = 0, 1, 2, 3

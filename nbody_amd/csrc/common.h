@@ -39,6 +39,7 @@ struct Counters {
     unsigned long long m2lx_total;     // walk: M2L entries between different levels
     uint32_t far_blocks, far_pairs;    // M2L: deferred pairs (level gap >= 2)
     uint32_t nonfinite;
+    uint32_t outside;                  // fixed root box: a particle lies outside it
 };
 
 enum : uint32_t { OVF_NODES = 1, OVF_SL = 2, OVF_ROWS = 4, OVF_M2L = 8, OVF_P2P = 16 };

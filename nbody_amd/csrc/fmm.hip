@@ -1222,7 +1222,11 @@ void launch_integrate(const State& S, uint32_t n, int mode, double dt_k, double 
 // Direct sum on the GPU (replaces NaiveSimulation::step's O(N^2) loop,
 // naive_simulation.cpp:8-35).  Source tiles staged in LDS; FP32 positions
 // relative to the system's box centre.  compat_law reproduces the reference
-// naive force law: separation (dx, dx, dx), no softening (:10-17).
+// naive force law: separation (dx, dx, dx), no softening (:10-17), with each
+// pair's term formed as the reference forms it in FP32 (uncentred x
+// difference, r = sqrt(d d + d d + d d), force = (k q_i q_j) d / (r r r),
+// divided by m_i; no contraction into FMAs) and summed in FP64, so one
+// evaluation differs from the reference only by its FP32 summation order.
 constexpr int DS_THREADS = 256;
 
 __global__ __launch_bounds__(DS_THREADS) void direct_f32_kernel(State S, uint32_t n, const uint32_t* sys_off,
@@ -1235,11 +1239,14 @@ __global__ __launch_bounds__(DS_THREADS) void direct_f32_kernel(State S, uint32_
     if (b + blockIdx.x * DS_THREADS >= e) return;
     const double4 bx = box[s];
     const double h = 0.5 * bx.w;
-    const double cx = bx.x + h, cy = bx.y + h, cz = bx.z + h;
+    // compat: absolute FP32 coordinates, as the reference holds them
+    const double cx = compat ? 0.0 : bx.x + h, cy = compat ? 0.0 : bx.y + h, cz = compat ? 0.0 : bx.z + h;
     const bool valid = i < e;
     float px = 0, py = 0, pz = 0;
     if (valid) { px = (float)(S.x[i] - cx); py = (float)(S.y[i] - cy); pz = (float)(S.z[i] - cz); }
     float ax = 0, ay = 0, az = 0, ph = 0;
+    double acd = 0.0;  // compat: sum of the reference's per-pair kicks / dt
+    const float qi = valid ? (float)S.charge[i] : 0.f, mi = valid ? (float)S.mass[i] : 1.f;
     for (uint32_t j0 = b; j0 < e; j0 += DS_THREADS) {
         const uint32_t j = j0 + threadIdx.x;
         __syncthreads();
@@ -1263,17 +1270,31 @@ __global__ __launch_bounds__(DS_THREADS) void direct_f32_kernel(State S, uint32_
             }
         } else {
             for (uint32_t k = 0; k < jn; ++k) {
+                // IEEE single operations in the reference's order: no FMA
+                // contraction; sqrtf and '/' are correctly rounded in HIP
+#pragma clang fp contract(off)
                 if (j0 + k == i) continue;
                 const float4 q = tile[k];
                 const float d = q.x - px;
-                const float r = sqrtf(3.0f * d * d);
-                const float w = q.w * d / (r * r * r);
-                ax += w; ay += w; az += w;
+                const float r = sqrtf(d * d + d * d + d * d);
+                const float f = G * (qi * q.w) * d / (r * r * r);
+                acd += (double)(f / mi);
                 ph += q.w / r;
             }
         }
     }
     if (!valid) return;
+    if (compat) {
+        const float a = (float)acd;
+        S.ax[i] = a; S.ay[i] = a; S.az[i] = a;
+        S.pot[i] = -G * ph;
+        if (kick_dt != 0.0) {
+            S.vx[i] += (double)a * kick_dt;
+            S.vy[i] += (double)a * kick_dt;
+            S.vz[i] += (double)a * kick_dt;
+        }
+        return;
+    }
     const float fac = (float)(S.charge[i] / S.mass[i]) * G;
     S.ax[i] = fac * ax; S.ay[i] = fac * ay; S.az[i] = fac * az;
     S.pot[i] = -G * ph;

@@ -23,7 +23,7 @@ void launch_bbox(const double* x, const double* y, const double* z, const uint32
                  double4* box, double* inv, float* unit, uint32_t* nonfinite, hipStream_t st);
 void launch_keys(const double* x, const double* y, const double* z, uint32_t n, const uint32_t* sys_off,
                  int nsys, int bits, const double4* box, const double* inv, uint64_t* keys, uint32_t* perm,
-                 hipStream_t st);
+                 uint32_t* fixed_flags, hipStream_t st);
 size_t tree_tmp_bytes(uint32_t n, int max_depth);
 void launch_tree_build(const uint64_t* keys, uint32_t n, const uint32_t* sys_off, int nsys, int bits, int cap,
                        int max_depth, NodeView nodes, uint32_t* node_begin, uint32_t* leaf_ids, uint32_t node_cap,

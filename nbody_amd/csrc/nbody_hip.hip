@@ -260,7 +260,8 @@ static int sort_particles(nbh_handle* h) {
                 h->cfg.root_lo, h->cfg.root_size, h->bbox_part.as<double>(), h->box.as<double4>(),
                 h->inv.as<double>(), h->unit.as<float>(), &dC->nonfinite, h->stream);
     launch_keys(a.x, a.y, a.z, n, h->sys_off.as<uint32_t>(), h->nsys, h->bits, h->box.as<double4>(),
-                h->inv.as<double>(), h->keys.as<uint64_t>(), h->perm.as<uint32_t>(), h->stream);
+                h->inv.as<double>(), h->keys.as<uint64_t>(), h->perm.as<uint32_t>(),
+                h->cfg.fixed_root ? &dC->nonfinite : nullptr, h->stream);
     int keybits = 3 * h->bits;
     if (h->nsys > 1) {
         int sb = 0;
@@ -285,6 +286,10 @@ static int build_tree(nbh_handle* h) {
                       h->leaf_ids.as<uint32_t>(), h->node_cap, dC, h->tree_tmp.p, h->stream);
     CHK(read_counters(h));
     if (h->hc->nonfinite) return h->fail(NBH_E_NONFINITE, "non-finite particle position");
+    if (h->hc->outside)
+        return h->fail(NBH_E_OUTSIDE, "a particle lies outside the fixed root box [%g, %g) x [%g, %g) x [%g, %g)",
+                       h->cfg.root_lo[0], h->cfg.root_lo[0] + h->cfg.root_size, h->cfg.root_lo[1],
+                       h->cfg.root_lo[1] + h->cfg.root_size, h->cfg.root_lo[2], h->cfg.root_lo[2] + h->cfg.root_size);
     h->num_nodes = h->hc->num_nodes;
     h->num_leaves = h->hc->num_leaves;
     h->depth = h->hc->depth;
@@ -422,11 +427,6 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
         M2LPlan plan{};
         bool started = false;
         if (early) {
-            // same-level kernels of an earlier attempt read the rows this one rewrites
-            if (attempt > 0) {
-                HIPCHK(h, hipStreamSynchronize(h->stream3));
-                HIPCHK(h, hipStreamSynchronize(h->stream4));
-            }
             Fe = fmm_params(h);  // this attempt's rows and pools
             copy_tables(Fe, em->T);
             plan = m2l_plan(h);
@@ -450,6 +450,12 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
         CHK(read_counters(h));
         const uint32_t ovf = h->hc->overflow;
         const uint32_t walk_bits = OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P;
+        if ((ovf & walk_bits) && early) {
+            // this attempt's same-level kernels may still be reading the rows
+            // and pools about to be re-allocated: drain their streams first
+            HIPCHK(h, hipStreamSynchronize(h->stream3));
+            HIPCHK(h, hipStreamSynchronize(h->stream4));
+        }
         if (ovf & OVF_SL) {
             h->sl_cap = (uint32_t)std::min<unsigned long long>(
                 std::max<unsigned long long>(2ull * h->sl_cap, h->hc->sl_used + h->hc->sl_used / 4 + waste_s),
@@ -739,6 +745,21 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
             }
         if (!(mass[i] > 0) || !std::isfinite(mass[i])) { g_create_error = "masses must be finite and > 0"; return NBH_E_INVALID; }
         if (charge && !std::isfinite(charge[i])) { g_create_error = "non-finite charge"; return NBH_E_NONFINITE; }
+    }
+    if (cfg->fixed_root) {
+        // the reference's octree spans [0, bounds) (open_cl_simulation.cpp:20,43-47); the
+        // closed upper face is accepted (main.cpp:118-122 may produce it)
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const double t = pos[3 * i + k] - cfg->root_lo[k];
+                if (!(t >= 0.0 && t <= cfg->root_size)) {
+                    char b[160];
+                    snprintf(b, sizeof b, "particle %zu lies outside the fixed root box (axis %d: %g not in [%g, %g])",
+                             i, k, pos[3 * i + k], cfg->root_lo[k], cfg->root_lo[k] + cfg->root_size);
+                    g_create_error = b;
+                    return NBH_E_OUTSIDE;
+                }
+            }
     }
     bool gravity = true;
     if (charge)

@@ -135,24 +135,37 @@ __global__ __launch_bounds__(WAVE) void bbox_final_kernel(const double* part, in
 }
 
 // key = sys << 3B | interleave(qx, qy, qz), q = clamp(floor((x - lo) * 2^B / size))
+// With a fixed root box (flags != nullptr) the bounding-cube pass does not
+// run, so this kernel reports what it would have caught: a non-finite
+// coordinate (flags[0]) and a particle outside the closed box [lo, lo + size]
+// (flags[1]), which the clamp would otherwise file into a boundary cell whose
+// centre-relative offsets and opening criterion no longer hold for it.  A
+// coordinate equal to lo + size (the reference's inputs may reach the upper
+// bound, main.cpp:118-122) stays inside and lands in the last cell.
 __global__ void keys_kernel(const double* __restrict__ x, const double* __restrict__ y,
                             const double* __restrict__ z, uint32_t n, const uint32_t* sys_off, int nsys,
-                            int bits, const double4* box, const double* inv, uint64_t* keys, uint32_t* perm) {
+                            int bits, const double4* box, const double* inv, uint64_t* keys, uint32_t* perm,
+                            uint32_t* flags) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = nsys == 1 ? 0 : sys_of(sys_off, nsys, i);
     const double4 b = box[s];
     const double iv = inv[s];
-    const double qmax = ldexp(1.0, bits) - 1.0;
+    const double qn = ldexp(1.0, bits), qmax = qn - 1.0;
     const double v[3] = {x[i], y[i], z[i]};
     const double lo[3] = {b.x, b.y, b.z};
     uint64_t q[3];
+    bool bad = false, out = false;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        double f = floor((v[a] - lo[a]) * iv);
+        const double t = (v[a] - lo[a]) * iv;
+        bad |= !isfinite(v[a]);
+        out |= !(t >= 0.0 && t <= qn);
+        double f = floor(t);
         f = fmin(fmax(f, 0.0), qmax);
         q[a] = (uint64_t)f;
     }
+    if (flags && (bad || out)) atomicOr(&flags[bad ? 0 : 1], 1u);
     keys[i] = ((uint64_t)s << (3 * bits)) | (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
     perm[i] = i;
 }
@@ -356,9 +369,9 @@ void launch_bbox(const double* x, const double* y, const double* z, const uint32
 
 void launch_keys(const double* x, const double* y, const double* z, uint32_t n, const uint32_t* sys_off,
                  int nsys, int bits, const double4* box, const double* inv, uint64_t* keys, uint32_t* perm,
-                 hipStream_t st) {
+                 uint32_t* fixed_flags, hipStream_t st) {
     hipLaunchKernelGGL(keys_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, x, y, z, n, sys_off, nsys, bits,
-                       box, inv, keys, perm);
+                       box, inv, keys, perm, fixed_flags);
 }
 
 size_t tree_tmp_bytes(uint32_t n, int max_depth) {

@@ -7,7 +7,7 @@ import pytest
 
 import nbody_amd as nb
 import pyoracle as po
-from helpers import make_sim, rms_rel
+from helpers import make_sim, max_rel, max_rel_floor, rms_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -58,14 +58,21 @@ def test_ensemble_matches_oracle_and_direct():
     sim.close()
 
 
-def test_clustered_pair_theta04_p6():
+CLUSTER_M2L = [("grouped", nb.M2L_GROUPED), ("mfma_f32", nb.M2L_MFMA_F32), ("mfma_bf16x3", nb.M2L_BF16X3)]
+
+
+@pytest.mark.parametrize("kern,m2l", CLUSTER_M2L)
+def test_clustered_pair_theta04_p6(kern, m2l):
     """Clustered pair, 2 x 32768: two Plummer spheres (seeds 2, 3) at +-5 on x,
-    approaching at 0.3 each; eps=5e-4, theta=0.4, p=6."""
-    _case("clustered_pair_2x32768_p6")
+    approaching at 0.3 each; eps=5e-4, theta=0.4, p=6, for each M2L precision.
+    Lists bit-exact; forces against the FP64 oracle FMM on the same lists
+    (FP32 tolerance, RMS and per particle) and against the FP64 direct sum on
+    2048 sampled targets (RMS and max relative)."""
+    _case(f"clustered_pair_2x32768_p6_{kern}")
     x, v, m = nb.ics.clustered_pair(32768)
     n = len(m)
     m = m / m.sum()
-    sim = make_sim(x, v, m, leaf_capacity=16, order=6, theta=0.4, eps=5e-4)
+    sim = make_sim(x, v, m, leaf_capacity=16, order=6, theta=0.4, eps=5e-4, m2l=m2l)
     acc, _ = sim.forces()
     keys, ids, box = sim.debug_keys()
     tree = po.build_tree(keys, cap=16)
@@ -74,12 +81,17 @@ def test_clustered_pair_theta04_p6():
     lists = po.walk(tree, 2, 5)
     mo, ms, pofs, ps = sim.debug_lists()
     assert np.array_equal(ms, lists.m2l_src) and np.array_equal(ps, lists.p2p_src)
+    acc_o, _ = po.fmm(tree, lists, 6, x[ids], m[ids], box, eps=5e-4, m2l_order=sim.m2l_order)
+    e_o = rms_rel(acc[ids], acc_o)
+    m_o = max_rel_floor(acc[ids], acc_o)
     tg = np.random.default_rng(8).choice(n, 2048, replace=False)
     ad, _ = po.direct(x, m, xt=x[tg], self_idx=tg, eps=5e-4)
     e = rms_rel(acc[tg], ad)
-    emax = float(np.max(np.linalg.norm(acc[tg] - ad, axis=1) / np.linalg.norm(ad, axis=1)))
-    print(f"  rms err {e:.2e} max {emax:.2e} depth {sim.debug_sizes()['depth']}", flush=True)
-    assert e < 1e-5
+    emax = max_rel(acc[tg], ad)
+    print(f"  vs fp64 fmm: rms {e_o:.2e} max {m_o:.2e}; vs fp64 direct: rms {e:.2e} max {emax:.2e} "
+          f"depth {sim.debug_sizes()['depth']}", flush=True)
+    assert e_o < 2e-5 and m_o < 1e-3
+    assert e < 1e-5 and emax < 1e-3
     sim.close()
 
 

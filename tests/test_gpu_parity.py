@@ -10,7 +10,7 @@ import pytest
 
 import nbody_amd as nb
 import pyoracle as po
-from helpers import make_sim, max_rel, oracle_pipeline, rms_rel
+from helpers import make_sim, max_rel, max_rel_floor, oracle_pipeline, rms_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -92,9 +92,15 @@ def test_tree_lists_forces_match_oracle(name, kind, n, cap, p, th, eps, kern, m2
     pot_s = pot[ids]
     e_acc = rms_rel(acc_s, acc_o)
     e_pot = rms_rel(pot_s, pot_o)
-    print(f"  fp32 vs fp64-fmm: acc rms {e_acc:.2e} pot rms {e_pot:.2e}", flush=True)
+    # per particle: an indexing slip that damages a few particles (a chunk or
+    # leaf edge) would pass an RMS bound at these sizes
+    m_acc = max_rel_floor(acc_s, acc_o)
+    m_pot = max_rel_floor(pot_s, pot_o)
+    print(f"  fp32 vs fp64-fmm: acc rms {e_acc:.2e} max {m_acc:.2e}; pot rms {e_pot:.2e} max {m_pot:.2e}",
+          flush=True)
     assert e_acc < tol_acc
     assert e_pot < tol_pot
+    assert m_acc < 1e-3 and m_pot < 1e-4
     st = sim.stats()
     assert st["m2l_pairs"] == len(lists.m2l_src)
     assert st["p2p_pairs"] == len(lists.p2p_src)
@@ -135,8 +141,10 @@ def test_direct_fp64_matches_oracle():
     sim.close()
 
 
-@pytest.mark.parametrize("p,err", [(2, 5e-3), (4, 2e-4), (6, 3e-5)])
+@pytest.mark.parametrize("p,err", [(2, 5e-3), (4, 1e-4), (6, 3e-5)])
 def test_fmm_converges_to_direct(p, err):
+    """Uniform cube N=4096 (seed 0), eps=1e-2, theta=0.5, cap 16, the default
+    M2L: RMS error against the FP64 direct sum within the 1e-4 gate at p=4."""
     _case(f"convergence_uniform4096_p{p}")
     n = 4096
     x, v, m = nb.ics.uniform(n, 0)
