@@ -150,6 +150,18 @@ int nbh_compute_forces(nbh_handle* h, float* acc, float* pot);
  * include the q_i/m_i factor like nbh_compute_forces. */
 int nbh_direct_fp64(nbh_handle* h, size_t nt, const int64_t* targets, double* acc, double* pot);
 
+/* nbh_direct_fp64 without device allocation (no counterpart in the
+ * reference).  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * per-call target list, slots and FP64 results live in a fixed scratch array
+ * of the library on each GPU (44 bytes per target, 2.75 MiB, held until the process
+ * ends), so nt is at most NBH_NOALLOC_MAX_TARGETS (larger nt fails with
+ * NBH_E_INVALID).  Same targets, same kernel and results as nbh_direct_fp64.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (from several threads or handles) race. */
+#define NBH_NOALLOC_MAX_TARGETS 65536
+int nbh_direct_fp64_noalloc(nbh_handle* h, size_t nt, const int64_t* targets, double* acc, double* pot);
+
 /* Energy and momentum of the current state (FP64, on the GPU):
  * out[0] kinetic, out[1] potential (uses the last force evaluation),
  * out[2..4] linear momentum, out[5..7] angular momentum. */

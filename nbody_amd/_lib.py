@@ -18,6 +18,7 @@ NBH_E_NOMEM = -3
 NBH_E_NONFINITE = -4
 NBH_E_INTERNAL = -5
 NBH_E_OUTSIDE = -6
+NOALLOC_MAX_TARGETS = 65536
 
 SOLVER_FMM, SOLVER_DIRECT = 0, 1
 M2L_PLAIN, M2L_GROUPED, M2L_BF16X3, M2L_MFMA_F32 = 0, 1, 2, 3
@@ -96,6 +97,7 @@ SIGNATURES = {
     "nbh_set_state": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_double]),
     "nbh_compute_forces": (ctypes.c_int, [_vp, _fp, _fp]),
     "nbh_direct_fp64": (ctypes.c_int, [_vp, _sz, _i64p, _dp, _dp]),
+    "nbh_direct_fp64_noalloc": (ctypes.c_int, [_vp, _sz, _i64p, _dp, _dp]),
     "nbh_diagnostics": (ctypes.c_int, [_vp, _dp]),
     "nbh_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
     "nbh_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -1319,6 +1319,13 @@ __global__ __launch_bounds__(256) void direct_fp64_kernel(State S, const uint32_
     const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     if (w >= nt) return;
     const uint32_t i = targets[w];
+    if (i >= sys_off[nsys]) {  // no such particle: report NaN rather than read out of bounds
+        if (lane_id() == 0) {
+            acc[3 * w] = acc[3 * w + 1] = acc[3 * w + 2] = NAN;
+            pot[w] = NAN;
+        }
+        return;
+    }
     int s = 0;
     {
         int lo = 0, hi = nsys - 1;
@@ -1350,6 +1357,29 @@ void launch_direct_fp64(const State& S, const uint32_t* sys_off, int nsys, const
                         double G, double eps2, double* acc, double* pot, hipStream_t st) {
     hipLaunchKernelGGL(direct_fp64_kernel, dim3(ceil_div(nt, 4)), dim3(256), 0, st, S, sys_off, nsys, targets, nt, G,
                        eps2, acc, pot);
+}
+
+// Sorted slot of each target given in input order, by a scan of the id array
+// (one wavefront per target; an id array is a permutation, so every index in
+// [0, n) is found once; anything else gets 0xffffffff).  Costs a read of the
+// ids per target, an eighth of what the FP64 sum then reads per target.
+__global__ __launch_bounds__(256) void find_slots_kernel(const uint32_t* __restrict__ id, uint32_t n,
+                                                         const int64_t* __restrict__ targets, uint32_t nt,
+                                                         uint32_t* slots) {
+    const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    if (w >= nt) return;
+    const int64_t t = targets[w];
+    uint32_t found = 0xffffffffu;
+    for (uint32_t j0 = 0; j0 < n && found == 0xffffffffu; j0 += WAVE) {
+        const uint32_t j = j0 + lane_id();
+        const uint64_t hit = __ballot(j < n && (int64_t)id[j] == t);
+        if (hit) found = j0 + (uint32_t)__ffsll((unsigned long long)hit) - 1;
+    }
+    if (lane_id() == 0) slots[w] = found;
+}
+void launch_find_slots(const uint32_t* id, uint32_t n, const int64_t* targets, uint32_t nt, uint32_t* slots,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(find_slots_kernel, dim3(ceil_div(nt, 4)), dim3(256), 0, st, id, n, targets, nt, slots);
 }
 
 // ---------------------------------------------------------------------------

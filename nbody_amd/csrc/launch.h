@@ -173,6 +173,8 @@ void launch_direct_f32(const State& S, uint32_t n, const uint32_t* sys_off, int 
                        float G, float eps2, int compat_law, double kick_dt, hipStream_t st);
 void launch_direct_fp64(const State& S, const uint32_t* sys_off, int nsys, const uint32_t* targets, uint32_t nt,
                         double G, double eps2, double* acc, double* pot, hipStream_t st);
+void launch_find_slots(const uint32_t* id, uint32_t n, const int64_t* targets, uint32_t nt, uint32_t* slots,
+                       hipStream_t st);
 
 void launch_diagnostics(const State& S, uint32_t n, double* partial, uint32_t nblk, hipStream_t st);
 uint32_t diagnostics_blocks(uint32_t n);

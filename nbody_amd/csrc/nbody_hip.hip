@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1041,6 +1042,73 @@ int nbh_direct_fp64(nbh_handle* h, size_t nt, const int64_t* targets, double* ac
     HIPCHK(h, hipMemcpyAsync(pot, dp.p, nt * 8, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     dt.release(); da.release(); dp.release();
+    return NBH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Fixed device scratch of nbh_direct_fp64_noalloc: the input-order targets,
+// their sorted slots and the FP64 results of up to NBH_NOALLOC_MAX_TARGETS
+// targets, 44 bytes per target (2.75 MiB), held by every GPU the library runs
+// on until the process ends.  Host code uses only the per-device address from
+// hipGetSymbolAddress, never the array's name.
+namespace {
+constexpr size_t kNoallocTg = 0;                                                 // int64[max]
+constexpr size_t kNoallocSlot = kNoallocTg + 8 * (size_t)NBH_NOALLOC_MAX_TARGETS;  // uint32[max]
+constexpr size_t kNoallocAcc = kNoallocSlot + 4 * (size_t)NBH_NOALLOC_MAX_TARGETS; // double[3 max]
+constexpr size_t kNoallocPot = kNoallocAcc + 24 * (size_t)NBH_NOALLOC_MAX_TARGETS; // double[max]
+constexpr size_t kNoallocBytes = kNoallocPot + 8 * (size_t)NBH_NOALLOC_MAX_TARGETS;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) char scratch[kNoallocBytes];
+
+// The device address of this GPU's scratch (the current device must be
+// h->device), looked up once per device.
+static int noalloc_scratch(nbh_handle* h, char** out) {
+    if (h->device < 0 || h->device >= kMaxDevices) return h->fail(NBH_E_INVALID, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[h->device]) {
+        void* p = nullptr;
+        HIPCHK(h, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return h->fail(NBH_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[h->device] = (char*)p;
+    }
+    *out = g_scratch_ptr[h->device];
+    return NBH_OK;
+}
+
+extern "C" {
+
+int nbh_direct_fp64_noalloc(nbh_handle* h, size_t nt, const int64_t* targets, double* acc, double* pot) {
+    if (!h) return NBH_E_INVALID;
+    if (!targets || !acc || !pot) return h->fail(NBH_E_INVALID, "targets, acc and pot are required");
+    if (nt > (size_t)NBH_NOALLOC_MAX_TARGETS)
+        return h->fail(NBH_E_INVALID, "nbh_direct_fp64_noalloc: %zu targets exceed its fixed scratch of %d targets",
+                       nt, NBH_NOALLOC_MAX_TARGETS);
+    HIPCHK(h, hipSetDevice(h->device));
+    if (!nt) return NBH_OK;
+    const size_t n = h->n;
+    for (size_t k = 0; k < nt; ++k)
+        if (targets[k] < 0 || (size_t)targets[k] >= n) return h->fail(NBH_E_INVALID, "target index out of range");
+    char* base = nullptr;
+    CHK(noalloc_scratch(h, &base));
+    int64_t* dtg = (int64_t*)(base + kNoallocTg);
+    uint32_t* dslot = (uint32_t*)(base + kNoallocSlot);
+    double* dacc = (double*)(base + kNoallocAcc);
+    double* dpot = (double*)(base + kNoallocPot);
+    State S = h->st[h->cur].view();
+    HIPCHK(h, hipMemcpyAsync(dtg, targets, nt * 8, hipMemcpyHostToDevice, h->stream));
+    launch_find_slots(S.id, (uint32_t)n, dtg, (uint32_t)nt, dslot, h->stream);
+    launch_direct_fp64(S, h->sys_off.as<uint32_t>(), h->nsys, dslot, (uint32_t)nt, h->cfg.G,
+                       h->cfg.eps * h->cfg.eps, dacc, dpot, h->stream);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(acc, dacc, nt * 24, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(pot, dpot, nt * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return NBH_OK;
 }
 

@@ -181,6 +181,16 @@ class HipSimulation(Simulation):
                "nbh_direct_fp64")
         return a, p
 
+    def direct_fp64_noalloc(self, targets):
+        """nbh_direct_fp64_noalloc: the same FP64 sums with no device
+        allocation (at most NOALLOC_MAX_TARGETS targets per call)."""
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        a = np.zeros((len(t), 3))
+        p = np.zeros(len(t))
+        _check(self._h, lib().nbh_direct_fp64_noalloc(self._h, len(t), ptr(t, _i64p), ptr(a, _dp), ptr(p, _dp)),
+               "nbh_direct_fp64_noalloc")
+        return a, p
+
     def diagnostics(self):
         out = np.zeros(8)
         _check(self._h, lib().nbh_diagnostics(self._h, ptr(out, _dp)), "nbh_diagnostics")
