@@ -45,8 +45,9 @@ extern "C" {
 /* far-field (M2L) kernel */
 #define NBH_M2L_PLAIN 0        /* per-pair FP32 VALU, derivative tensor per pair */
 #define NBH_M2L_GROUPED 1      /* FP32 VALU, grouped by translation offset */
-#define NBH_M2L_BF16X3 2       /* grouped; same-level pairs on MFMA, bf16x3 split products, FP32 accumulate */
-#define NBH_M2L_MFMA_F32 3     /* grouped; same-level pairs on MFMA, FP32 products */
+#define NBH_M2L_BF16X3 2       /* grouped; same-level pairs by translation offset on MFMA with the dense
+                                  operator (every |k|, |n| <= p), bf16x3 split products, FP32 accumulate */
+#define NBH_M2L_MFMA_F32 3     /* the same with FP32 products */
 
 /* near-field (P2P) kernel */
 #define NBH_P2P_LEAF 0         /* one wavefront per target leaf (plain) */
@@ -97,6 +98,11 @@ typedef struct nbh_stats {
     uint64_t p2p_interactions; /* directed particle-particle interactions (self excluded) */
     uint64_t split_pairs;    /* pairs opened by the walk */
     double ms_tree, ms_moments, ms_walk, ms_m2l, ms_l2l, ms_p2p, ms_total; /* if timing enabled */
+    uint64_t m2l_same_order; /* M2L truncation q of the pairs of one level: 2p (every term
+                                |k|, |n| <= p) when they run on the matrix cores, else m2l_order */
+    uint64_t m2l_same_pairs;  /* directed M2L entries between cells of one level */
+    uint64_t m2l_same_launches; /* timing: same-level M2L launches bracketed by events */
+    double ms_m2l_same;       /* timing: sum of their durations, each on its own stream */
 } nbh_stats;
 
 int nbh_abi_version(void);
@@ -177,6 +183,20 @@ int nbh_set_timing(nbh_handle* h, int enable);
  * gm[n] = (da/dm)^T w, input order.  Uses the tree and M2L/P2P lists of a
  * force evaluation at the current positions (FP32 kernels). */
 int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm);
+
+/* Reverse-mode gradient of a KDK trajectory, kept on the device (no
+ * counterpart in the reference).  nbh_adjoint_forward runs nsteps steps from
+ * the current state (like nbh_step) and checkpoints the full state every
+ * ckpt_every steps on the device.  nbh_adjoint_backward then takes the
+ * cotangents xbar[3n], vbar[3n] of the final positions and velocities and
+ * returns the gradient of that scalar with respect to the initial positions
+ * gx[3n], velocities gv[3n] and masses gm[n] (gravity, charge == mass; input
+ * order): it recomputes each segment from its checkpoint and runs the
+ * transposed leapfrog with one force VJP (nbh_force_vjp's kernels) per state.
+ * The handle ends at the final state.  Needs the FMM solver and KDK. */
+int nbh_adjoint_forward(nbh_handle* h, int nsteps, int ckpt_every);
+int nbh_adjoint_backward(nbh_handle* h, const double* xbar, const double* vbar, double* gx, double* gv,
+                         double* gm);
 
 /* ---- tree / list export for parity tests (read-only views) -------------- */
 /* sizes[0]=nodes, [1]=leaves, [2]=depth, [3]=m2l entries, [4]=p2p entries,

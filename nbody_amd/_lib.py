@@ -66,6 +66,10 @@ class Stats(ctypes.Structure):
         ("ms_l2l", ctypes.c_double),
         ("ms_p2p", ctypes.c_double),
         ("ms_total", ctypes.c_double),
+        ("m2l_same_order", ctypes.c_uint64),
+        ("m2l_same_pairs", ctypes.c_uint64),
+        ("m2l_same_launches", ctypes.c_uint64),
+        ("ms_m2l_same", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -102,6 +106,8 @@ SIGNATURES = {
     "nbh_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
     "nbh_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "nbh_force_vjp": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "nbh_adjoint_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "nbh_adjoint_backward": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "nbh_debug_sizes": (ctypes.c_int, [_vp, _i64p]),
     "nbh_debug_keys": (ctypes.c_int, [_vp, _u64p, _u32p, _dp]),
     "nbh_debug_tree": (ctypes.c_int, [_vp] + [_i32p] * 10),

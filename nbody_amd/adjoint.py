@@ -22,8 +22,8 @@ from typing import Callable, Optional
 
 import numpy as np
 
-from ._lib import default_config
-from .simulation import HipSimulation
+from ._lib import default_config, lib, ptr
+from .simulation import HipSimulation, _check, _dp
 
 
 class HipForceProvider:
@@ -108,28 +108,65 @@ class TrajectoryAdjoint:
         return xbar, vbar, mbar
 
 
+class DeviceTrajectory:
+    """The same gradient with the whole reverse pass on the GPU
+    (nbh_adjoint_forward / nbh_adjoint_backward): checkpoints, recomputed
+    states and cotangents stay in device memory; the host sees only the
+    initial state, the cotangents of the final state and the gradient."""
+
+    def __init__(self, x0, v0, m, nsteps: int, ckpt_every: int = 20, config=None):
+        cfg = config if config is not None else default_config()
+        self.sim = HipSimulation.from_arrays(x0, v0, m, m, config=cfg)
+        self.n = len(m)
+        self.nsteps = int(nsteps)
+        self.k = max(1, int(ckpt_every))
+        self.seconds = {"forward": 0.0, "backward": 0.0}
+
+    def forward(self):
+        t0 = time.perf_counter()
+        rc = lib().nbh_adjoint_forward(self.sim._h, self.nsteps, self.k)
+        self.seconds["forward"] = time.perf_counter() - t0
+        _check(self.sim._h, rc, "nbh_adjoint_forward")
+        x, v, _, _, _ = self.sim.state()
+        return x, v
+
+    def backward(self, xbar, vbar):
+        xb = np.ascontiguousarray(xbar, dtype=np.float64).reshape(self.n, 3)
+        vb = np.ascontiguousarray(vbar, dtype=np.float64).reshape(self.n, 3)
+        gx = np.zeros((self.n, 3))
+        gv = np.zeros((self.n, 3))
+        gm = np.zeros(self.n)
+        t0 = time.perf_counter()
+        rc = lib().nbh_adjoint_backward(self.sim._h, ptr(xb, _dp), ptr(vb, _dp), ptr(gx, _dp), ptr(gv, _dp),
+                                        ptr(gm, _dp))
+        self.seconds["backward"] = time.perf_counter() - t0
+        _check(self.sim._h, rc, "nbh_adjoint_backward")
+        return gx, gv, gm
+
+    def close(self):
+        self.sim.close()
+
+
 def trajectory_function():
-    """torch.autograd.Function computing x_N from (x0, v0, m) on the GPU; the
-    backward pass is TrajectoryAdjoint.  Import torch lazily (only the
-    harness needs it)."""
+    """torch.autograd.Function computing (x_N, v_N) from (x0, v0, m) on the GPU;
+    the backward pass is DeviceTrajectory's reverse leapfrog.  Import torch
+    lazily (only the harness needs it)."""
     import torch
 
     class Trajectory(torch.autograd.Function):
         @staticmethod
         def forward(ctx, x0, v0, m, nsteps: int, ckpt_every: int, config):
-            prov = HipForceProvider(x0.detach().cpu().double().numpy(), v0.detach().cpu().double().numpy(),
-                                    m.detach().cpu().double().numpy(), config)
-            adj = TrajectoryAdjoint(prov, prov.dt, nsteps, ckpt_every)
-            xN, vN = adj.forward(x0.detach().cpu().double().numpy(), v0.detach().cpu().double().numpy())
+            adj = DeviceTrajectory(x0.detach().cpu().double().numpy(), v0.detach().cpu().double().numpy(),
+                                   m.detach().cpu().double().numpy(), nsteps, ckpt_every, config)
+            xN, vN = adj.forward()
             ctx.adj = adj
-            ctx.prov = prov
             return (torch.from_numpy(xN).to(x0.device, x0.dtype), torch.from_numpy(vN).to(v0.device, v0.dtype))
 
         @staticmethod
         def backward(ctx, gxN, gvN):
             adj = ctx.adj
             gx0, gv0, gm = adj.backward(gxN.detach().cpu().double().numpy(), gvN.detach().cpu().double().numpy())
-            ctx.prov.close()
+            adj.close()
             dev, dt_ = gxN.device, gxN.dtype
             return (torch.from_numpy(gx0).to(dev, dt_), torch.from_numpy(gv0).to(dev, dt_),
                     torch.from_numpy(gm).to(dev, dt_), None, None, None)

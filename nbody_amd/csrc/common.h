@@ -42,7 +42,9 @@ struct Counters {
     uint32_t outside;                  // fixed root box: a particle lies outside it
 };
 
-enum : uint32_t { OVF_NODES = 1, OVF_SL = 2, OVF_ROWS = 4, OVF_M2L = 8, OVF_P2P = 16 };
+// OVF_GEOM: a same-level M2L block found a pair count or a source position
+// that differs from its list (an internal consistency check)
+enum : uint32_t { OVF_NODES = 1, OVF_SL = 2, OVF_ROWS = 4, OVF_M2L = 8, OVF_P2P = 16, OVF_GEOM = 32 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 

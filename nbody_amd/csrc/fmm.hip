@@ -596,207 +596,6 @@ __device__ __forceinline__ void load_tensor(const float4* row, uint32_t neg, flo
     octant_signs<Q>(neg, D);
 }
 
-// ---------------------------------------------------------------------------
-// Same-level M2L on the matrix cores.  For a target A with same-level sources
-// B_j,
-//   L_k = sum_n (-1)^|n| G[k+n][n],   G[m][n] = sum_j D_m(R_j) M_n(B_j),
-// so G is a matrix product whose inner dimension is A's pair list: rows n
-// (the NC moment coefficients, TN tiles of 16), columns m (the NCQ tensor
-// coefficients, TM tiles), 32 pairs per k-block.  The wavefront accumulates G
-// in 16x16 MFMA tiles, stages it in LDS, and lane k < NC gathers L_k.
-//   BF3:  bf16x3 split products with FP32 accumulation, x = hi + lo with
-//         hi = bf16(x), lo = bf16(x - hi) (round to nearest even), G += lo_M
-//         hi_D + hi_M lo_D + hi_M hi_D (v_mfma_f32_16x16x32_bf16; each product
-//         to about 2^-16 relative).
-//   !BF3: FP32 products (v_mfma_f32_16x16x4_f32).
-// Tensors come from the level's table in global memory (L2-resident); LDS
-// holds G only.  Operand layouts (lane l, r = l % 16, h = l / 16): the A
-// operand row r and B operand column r; the k-slots of a lane are pairs
-// 8h..8h+7 of the block (16x16x32) or pair 4s + h at step s (16x16x4); the
-// accumulator's element i is C[4h + i][r].
-typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
-typedef __attribute__((ext_vector_type(4))) float f32x4_t;
-
-// octant parity bits (x, y, z exponents odd) of order-Q coefficient m
-template <int Q, size_t... I>
-__device__ __forceinline__ uint32_t coef_parity(int m, std::index_sequence<I...>) {
-    constexpr auto& T = Tab<Q>::t;
-    uint32_t v = 0;
-    ((v = m == (int)I ? (uint32_t)((T.mx[I] & 1) | ((T.my[I] & 1) << 1) | ((T.mz[I] & 1) << 2)) : v), ...);
-    return v;
-}
-// exponents (x | y << 8 | z << 16) of order-P coefficient k
-template <int P, size_t... I>
-__device__ __forceinline__ uint32_t coef_exponents(int k, std::index_sequence<I...>) {
-    constexpr auto& T = Tab<P>::t;
-    uint32_t v = 0;
-    ((v = k == (int)I ? (uint32_t)(T.mx[I] | (T.my[I] << 8) | (T.mz[I] << 16)) : v), ...);
-    return v;
-}
-// index of the coefficient with exponents (x, y, z) in the total-degree order
-__device__ __forceinline__ int coef_index(int x, int y, int z) {
-    const int d = x + y + z, a = d - x;
-    return d * (d + 1) * (d + 2) / 6 + a * (a + 1) / 2 + z;
-}
-// L_k += sum over the order-P n of the kept terms (m2l_keeps) of (-1)^|n| G[n][k + n]
-template <int P, int Q, int N>
-__device__ __forceinline__ void gather_step(const float* G, int gs, int kx, int ky, int kz, int kd, float& L) {
-    constexpr auto& T = Tab<P>::t;
-    constexpr int nx = T.mx[N], ny = T.my[N], nz = T.mz[N], nd = T.deg[N];
-    if (m2l_keeps(P, Q, kd, nd)) {
-        const float v = G[N * gs + coef_index(kx + nx, ky + ny, kz + nz)];
-        L += (nd & 1) ? -v : v;
-    }
-}
-template <int P, int Q, size_t... I>
-__device__ __forceinline__ float gather_local(const float* G, int gs, int kx, int ky, int kz, int kd,
-                                              std::index_sequence<I...>) {
-    float L = 0.f;
-    (gather_step<P, Q, (int)I>(G, gs, kx, ky, kz, kd, L), ...);
-    return L;
-}
-
-template <int P, int Q>
-constexpr int mfma_g_stride() { return (ncoef(Q) + 15) / 16 * 16 + 1; }
-// LDS bytes of one wavefront's G
-template <int P, int Q>
-constexpr size_t mfma_g_bytes() { return (size_t)(ncoef(P) + 15) / 16 * 16 * mfma_g_stride<P, Q>() * 4; }
-
-// The kernel waits on its loads more than on either pipe: at p <= 4 three
-// wavefronts per SIMD (168 registers, a few spilled) beat two without spills.
-template <int P, int Q, bool BF3>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 3 : 1, 3))) void m2l_mfma_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
-    constexpr int NC = ncoef(P), NCQ = ncoef(Q);
-    constexpr int TN = (NC + 15) / 16, TM = (NCQ + 15) / 16;
-    constexpr int GS = mfma_g_stride<P, Q>();
-    extern __shared__ float g_lds[];
-    const int lane = lane_id(), wave = threadIdx.x / WAVE;
-    const int lr = lane & 15, lh = lane >> 4;
-    float* G = g_lds + (size_t)wave * (mfma_g_bytes<P, Q>() / 4);
-    const int K1 = F.dK + 1;
-    const float* tab = F.dtab + (size_t)lvl * F.dsz * F.dcp;
-    uint32_t par[TM];
-#pragma unroll
-    for (int tm = 0; tm < TM; ++tm) par[tm] = coef_parity<Q>(16 * tm + lr, std::make_index_sequence<NCQ>{});
-    const uint32_t ke = coef_exponents<P>(lane, std::make_index_sequence<NC>{});
-    const int kx = ke & 255, ky = (ke >> 8) & 255, kz = ke >> 16, kd = kx + ky + kz;
-    if (F.C->overflow & (OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P)) return;
-    const uint32_t wpb = blockDim.x / WAVE;
-    for (uint32_t t = blockIdx.x * wpb + wave; t < count; t += gridDim.x * wpb) {
-        const uint32_t A = off + t;
-        const int4 ga = F.nodes.geo[A];
-        f32x4_t acc[TN][TM];
-#pragma unroll
-        for (int tn = 0; tn < TN; ++tn)
-#pragma unroll
-            for (int tm = 0; tm < TM; ++tm) acc[tn][tm] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-        const uint32_t row = F.lists.first[A];
-        const uint32_t s0 = F.lists.m_start[row];
-        const uint32_t e1 = s0 + F.lists.m_cnt[row] - F.lists.mx_cnt[row];
-        // lane l's pair of each batch of 64: source B and (tensor row offset
-        // << 3 | octant) rn, -1 past the end.  Loaded ahead: the sources two
-        // batches ahead and the geometry one batch ahead, so the dependent
-        // list -> geometry loads overlap the products of the current batch.
-        auto pair_code = [&](int Bc, const int4& gb) {
-            const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
-            const int neg = (ox < 0) | ((oy < 0) << 1) | ((oz < 0) << 2);
-            return ((((abs(ox) * K1 + abs(oy)) * K1 + abs(oz)) * F.dcp) << 3) | neg;
-        };
-        int B = 0, rn = -1, Bn = 0;
-        if (s0 + lane < e1) {
-            B = F.lists.m_pool[s0 + lane];
-            rn = pair_code(B, F.nodes.geo[B]);
-        }
-        if (s0 + WAVE + lane < e1) Bn = F.lists.m_pool[s0 + WAVE + lane];
-        for (uint32_t base = s0; base < e1; base += WAVE) {
-            const bool has_n = base + WAVE + lane < e1;
-            int4 gbn = make_int4(0, 0, 0, 0);
-            if (has_n) gbn = F.nodes.geo[Bn];
-            int Bnn = 0;
-            if (base + 2 * WAVE + lane < e1) Bnn = F.lists.m_pool[base + 2 * WAVE + lane];
-            const uint32_t nb = min(e1 - base, (uint32_t)WAVE);
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                if ((uint32_t)(32 * c) >= nb) break;
-                float av[TN][8], bv[TM][8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int src = 32 * c + (BF3 ? 8 * lh + j : 4 * j + lh);
-                    const int Bj = __shfl(B, src, WAVE);
-                    const int rj = __shfl(rn, src, WAVE);
-                    const bool ok = rj >= 0;
-                    const float* mrow = F.M + (size_t)Bj * F.ncp;
-                    const float* drow = tab + (rj >> 3);
-#pragma unroll
-                    for (int tn = 0; tn < TN; ++tn) {
-                        const int n = 16 * tn + lr;
-                        av[tn][j] = (ok && n < NC) ? mrow[n] : 0.f;
-                    }
-#pragma unroll
-                    for (int tm = 0; tm < TM; ++tm) {
-                        const int m = 16 * tm + lr;
-                        const float d = (ok && m < NCQ) ? drow[m] : 0.f;
-                        const uint32_t flip = (uint32_t)(__popc((uint32_t)rj & par[tm]) & 1) << 31;
-                        bv[tm][j] = __uint_as_float(__float_as_uint(d) ^ flip);
-                    }
-                }
-                if constexpr (BF3) {
-                    bf16x8_t ahi[TN], alo[TN], bhi[TM], blo[TM];
-#pragma unroll
-                    for (int tn = 0; tn < TN; ++tn)
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const __bf16 hi = (__bf16)av[tn][j];
-                            ahi[tn][j] = hi;
-                            alo[tn][j] = (__bf16)(av[tn][j] - (float)hi);
-                        }
-#pragma unroll
-                    for (int tm = 0; tm < TM; ++tm)
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const __bf16 hi = (__bf16)bv[tm][j];
-                            bhi[tm][j] = hi;
-                            blo[tm][j] = (__bf16)(bv[tm][j] - (float)hi);
-                        }
-#pragma unroll
-                    for (int tn = 0; tn < TN; ++tn)
-#pragma unroll
-                        for (int tm = 0; tm < TM; ++tm) {
-                            acc[tn][tm] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[tn], bhi[tm], acc[tn][tm], 0, 0, 0);
-                            acc[tn][tm] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tn], blo[tm], acc[tn][tm], 0, 0, 0);
-                            acc[tn][tm] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tn], bhi[tm], acc[tn][tm], 0, 0, 0);
-                        }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-#pragma unroll
-                        for (int tn = 0; tn < TN; ++tn)
-#pragma unroll
-                            for (int tm = 0; tm < TM; ++tm)
-                                acc[tn][tm] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tn][j], bv[tm][j], acc[tn][tm], 0,
-                                                                                  0, 0);
-                }
-            }
-            rn = has_n ? pair_code(Bn, gbn) : -1;
-            B = Bn;
-            Bn = Bnn;
-        }
-        // G[n][m] to LDS (rows n, stride GS), then L_k for k = lane
-#pragma unroll
-        for (int tn = 0; tn < TN; ++tn)
-#pragma unroll
-            for (int tm = 0; tm < TM; ++tm)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) G[(16 * tn + 4 * lh + i) * GS + 16 * tm + lr] = acc[tn][tm][i];
-        wave_lds_sync();
-        if (lane < NC) {
-            const float L = gather_local<P, Q>(G, GS, kx, ky, kz, kd, std::make_index_sequence<NC>{});
-            F.L[(size_t)A * F.ncp + lane] = L;
-        }
-        wave_lds_sync();
-    }
-}
-
 // M2L with the tensor built per pair, every entry (no tables: ensembles of
 // systems with different root sizes); writes L.  One wavefront per target,
 // lanes over its entries.
@@ -1053,19 +852,10 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
         const uint32_t cnt = plan.level_count[l];
         if (!cnt) continue;
         if (plan.mfma) {
-            // G of each wavefront in LDS: up to four wavefronts per workgroup
-            constexpr size_t gb = mfma_g_bytes<P, Q>();
-            const unsigned w = (unsigned)std::max<size_t>(1, std::min<size_t>(4, plan.max_lds / gb));
-            const unsigned g = std::min<unsigned>(ceil_div(cnt, w), 65535u);
-            const void* fn = plan.mfma == 1 ? reinterpret_cast<const void*>(&m2l_mfma_kernel<P, Q, true>)
-                                            : reinterpret_cast<const void*>(&m2l_mfma_kernel<P, Q, false>);
-            (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(w * gb));
-            if (plan.mfma == 1)
-                hipLaunchKernelGGL((m2l_mfma_kernel<P, Q, true>), dim3(g), dim3(w * WAVE), w * gb, st, F,
-                                   plan.level_off[l], cnt, l - F.dlevel0);
-            else
-                hipLaunchKernelGGL((m2l_mfma_kernel<P, Q, false>), dim3(g), dim3(w * WAVE), w * gb, st, F,
-                                   plan.level_off[l], cnt, l - F.dlevel0);
+            // by translation offset on the matrix cores: one workgroup per
+            // level-(l-2) cell, its level-l descendants as the targets
+            launch_m2l_offset_level(P, plan.mfma == 1, F, plan.off, plan.level_off[l - 2], plan.level_count[l - 2],
+                                    l - 2, st);
             continue;
         }
         constexpr int T = m2l_table_threads(P, Q);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "common.h"
 
 namespace nbh {
@@ -145,9 +147,38 @@ void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_l
 void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st);
 // M2L of order (p, q): q = p keeps |k| + |n| <= p, q = p + 1 also the terms
 // of degree p + 1 with |n| = 1 or |k| = 1 (the expansions stay of order p).
+// ---- same-level M2L by translation offset on the matrix cores (m2l_offset.hip)
+constexpr int OFF_J = 4;      // translation offsets per batch (along the contraction)
+constexpr int OFF_WAVES = 4;  // wavefronts per workgroup (one 4x4x4 block of targets)
+struct OffsetParams {
+    const void* mb;         // signed split moments per node, zero row at zero_node
+    uint32_t zero_node;
+    const uint32_t* tab;    // dense tensors, per level 2.. : [E^3][off_table_words(p)]
+    int E, K, W;            // E = K + 1 (octant table edge), K = max |o_i|, W = 4 + 2K (window edge)
+    const uint32_t* cand;   // candidate offsets: (ox+16) | (oy+16) << 5 | (oz+16) << 10 | class mask << 16
+    int ncand;
+    uint32_t* flags;        // Counters::overflow (OVF_GEOM)
+    const uint16_t* hank;   // off_tables: Hankel byte offsets of the A operand
+    const uint8_t* par;     // off_tables: octant parity bits of the tensor words
+};
+// the host tables of the kernel (uploaded once per handle)
+void off_tables(int p, bool bf3, std::vector<uint16_t>& hank, std::vector<uint8_t>& par);
+bool off_supported(int p);
+size_t off_lds_bytes(int p, bool bf3, int W);
+size_t off_moment_bytes(int p, bool bf3, uint32_t nn);
+int off_table_words(int p);
+void launch_off_moments(int p, bool bf3, const float* M, int ncp, uint32_t nn, const int4* geo, const double4* box,
+                        void* out, hipStream_t st);
+void launch_off_dtable(int p, bool bf3, const double4* box, double eps2, int level0, int nlev, int E, uint32_t* out,
+                       hipStream_t st);
+// the level-l targets below the level-(l-2) cells [goff, goff + gcount); lvl = l - 2
+void launch_m2l_offset_level(int p, bool bf3, const FmmParams& F, const OffsetParams& O, uint32_t goff,
+                             uint32_t gcount, int lvl, hipStream_t st);
+
 struct M2LPlan {
     int mode;  // 0: plain (one thread per target, tensor per pair); 1: grouped
-    int mfma;  // grouped, same-level pairs: 0 FP32 VALU; 1 MFMA bf16x3; 2 MFMA FP32
+    int mfma;  // grouped, same-level pairs: 0 FP32 VALU; 1 MFMA bf16x3; 2 MFMA FP32 (by offset, dense operator)
+    OffsetParams off;  // with mfma != 0
     uint32_t num_nodes;
     uint32_t level_off[MAX_LEVELS + 2];
     uint32_t level_count[MAX_LEVELS + 1];
@@ -184,6 +215,11 @@ uint32_t diagnostics_blocks(uint32_t n);
 // transposed far-field chain (dipole P2M, M2M, M2L, L2L into Fw) are
 // independent; the far-field join adds the rest to g.
 void launch_vjp_gather(const double* w_in, const uint32_t* id, float4* pw, uint32_t n, hipStream_t st);
+// reverse leapfrog: w = c vbar (after vbar += dt xbar when update); xbar[id] +=
+// g, mbar[id] += gm from a VJP in sorted order
+void launch_adj_kick(double* xb, double* vb, double* w, size_t m, double dt, double c, bool update, hipStream_t st);
+void launch_adj_accum(const double* g, const uint32_t* id, uint32_t n, double* xb, double* mb, hipStream_t st);
+void launch_fill_f64(double* p, size_t m, double v, hipStream_t st);
 void launch_vjp_near(int p, const FmmParams& F, const float4* pw, uint32_t n, uint32_t num_leaves, double* g,
                      hipStream_t st);
 void launch_vjp_chain(int p, int q, const M2LPlan& plan, const FmmParams& F, const FmmParams& Fw, const float4* pw,

@@ -103,6 +103,18 @@ struct nbh_handle {
     DevBuf counters, diag, tmp_small;
     Counters* hc = nullptr;  // pinned host copy
     DevBuf Mw, Lw, pw, wbuf, gbuf;   // adjoint
+    // same-level M2L by translation offset (matrix cores): split moments
+    // (forward and adjoint), dense tensor tables, candidate offsets
+    DevBuf mb, mbw, d2tab, cand, hank, opar;
+    // trajectory adjoint: checkpoints of the full state (every adj_k steps),
+    // the final state, the recomputed states of one segment, the cotangents
+    std::vector<DevBuf> adj_ckpt, adj_seg;
+    std::vector<double> adj_ckpt_t, adj_seg_t;
+    DevBuf adj_final, adj_xb, adj_vb, adj_mb, adj_w;
+    double adj_final_t = 0.0;
+    int adj_nsteps = 0, adj_k = 0;
+    bool adj_ready = false;
+    int ncand = 0, offK = 0;
     bool gravity = true;             // charge == mass for every particle
     FmmParams lastF{};               // parameters of the last FMM force evaluation
     uint32_t node_cap = 0, row_cap = 0, sl_cap = 0;
@@ -122,6 +134,9 @@ struct nbh_handle {
     hipEvent_t ev_depth[MAX_LEVELS + 1] = {};
     hipEvent_t ev_same = nullptr, ev_same4 = nullptr;  // same-level M2L done
     hipEvent_t evc0 = nullptr;     // timing: first same-level launch
+    // timing: each level's same-level M2L launch, bracketed on its own stream
+    hipEvent_t ev_lvl[2 * (MAX_LEVELS + 1)] = {};
+    bool lvl_timed[MAX_LEVELS + 1] = {};
 
     int fail(int code, const char* fmt, ...) {
         char buf[512];
@@ -209,6 +224,10 @@ int validate_config(const nbh_config* c, std::string& why) {
         }
         if (c->max_depth > 21) { why = "max_depth must be <= 21"; return NBH_E_INVALID; }
         if (c->m2l > NBH_M2L_MFMA_F32) { why = "unknown m2l kernel"; return NBH_E_INVALID; }
+        if ((c->m2l == NBH_M2L_BF16X3 || c->m2l == NBH_M2L_MFMA_F32) && c->order < 2) {
+            why = "the matrix-core M2L needs order >= 2";
+            return NBH_E_INVALID;
+        }
         if (c->p2p > NBH_P2P_LANE) { why = "unknown p2p kernel"; return NBH_E_INVALID; }
     }
     if (c->fixed_root && !(c->root_size > 0)) { why = "fixed_root needs root_size > 0"; return NBH_E_INVALID; }
@@ -445,12 +464,19 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
                 HIPCHK(h, hipStreamWaitEvent(sl, h->ev_depth[s], 0));
                 if (!started && h->timing) (void)hipEventRecord(h->evc0, sl);
                 started = true;
+                if (h->timing) (void)hipEventRecord(h->ev_lvl[2 * s], sl);
                 launch_m2l_parts(em->p, em->q, Fe, plan, M2L_LEVELS, (int)s, sl);
+                if (h->timing) {
+                    (void)hipEventRecord(h->ev_lvl[2 * s + 1], sl);
+                    h->lvl_timed[s] = true;
+                }
             }
         }
         CHK(read_counters(h));
         const uint32_t ovf = h->hc->overflow;
         const uint32_t walk_bits = OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P;
+        if (ovf & walk_bits)
+            for (auto& t : h->lvl_timed) t = false;  // the levels are launched again
         if ((ovf & walk_bits) && early) {
             // this attempt's same-level kernels may still be reading the rows
             // and pools about to be re-allocated: drain their streams first
@@ -495,10 +521,29 @@ static int m2l_q(const nbh_handle* h) {
     return h->cfg.m2l_order ? (int)h->cfg.m2l_order : (int)h->cfg.order + 1;
 }
 
+// the same-level pairs go to the matrix cores (one system; the window and
+// the offset encoding bound the opening ratio)
+static bool use_offset_m2l(const nbh_handle* h) {
+    return (h->cfg.m2l == NBH_M2L_BF16X3 || h->cfg.m2l == NBH_M2L_MFMA_F32) && h->nsys == 1 && h->ncand > 0;
+}
+
 static M2LPlan m2l_plan(const nbh_handle* h) {
     M2LPlan pl{};
     pl.mode = h->cfg.m2l == NBH_M2L_PLAIN ? 0 : 1;
-    pl.mfma = h->cfg.m2l == NBH_M2L_BF16X3 ? 1 : (h->cfg.m2l == NBH_M2L_MFMA_F32 ? 2 : 0);
+    pl.mfma = use_offset_m2l(h) ? (h->cfg.m2l == NBH_M2L_BF16X3 ? 1 : 2) : 0;
+    if (pl.mfma) {
+        pl.off.mb = h->mb.p;
+        pl.off.zero_node = h->num_nodes;
+        pl.off.tab = h->d2tab.as<uint32_t>();
+        pl.off.K = h->offK;
+        pl.off.E = h->offK + 1;
+        pl.off.W = 4 + 2 * h->offK;
+        pl.off.cand = h->cand.as<uint32_t>();
+        pl.off.ncand = h->ncand;
+        pl.off.flags = &h->counters.as<Counters>()->overflow;
+        pl.off.hank = h->hank.as<uint16_t>();
+        pl.off.par = h->opar.as<uint8_t>();
+    }
     pl.num_nodes = h->num_nodes;
     pl.max_lds = (size_t)h->max_lds;
     std::copy(h->hc->level_off, h->hc->level_off + MAX_LEVELS + 2, pl.level_off);
@@ -555,6 +600,16 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         T.xlevel0 = 1;
         T.xnlev = xn;
         launch_dtable(q, T, T.xtab, 1, xn, E, 1, 2, 1, h->stream2);
+        if (use_offset_m2l(h)) {
+            // the matrix-core same-level M2L: signed split moments and the
+            // dense tensors (orders <= 2p) of the offsets [0, K]^3, levels 2..depth
+            const bool bf3 = h->cfg.m2l == NBH_M2L_BF16X3;
+            HIPCHK(h, h->mb.ensure(off_moment_bytes(p, bf3, h->num_nodes)));
+            HIPCHK(h, h->d2tab.ensure((size_t)dsz * nlev * off_table_words(p) * 4));
+            launch_off_moments(p, bf3, F.M, F.ncp, h->num_nodes, h->geo.as<int4>(), h->box.as<double4>(), h->mb.p, h->stream2);
+            launch_off_dtable(p, bf3, h->box.as<double4>(), h->cfg.eps * h->cfg.eps, 2, nlev, K + 1,
+                              h->d2tab.as<uint32_t>(), h->stream2);
+        }
     }
     HIPCHK(h, hipEventRecord(h->ev_side, h->stream2));
     EarlyM2L em;
@@ -625,6 +680,18 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     h->stats.p2p_pairs = h->hc->p2p_total;
     h->stats.p2p_interactions = h->hc->p2p_inter;
     h->stats.split_pairs = h->hc->split_total;
+    h->stats.m2l_same_pairs = h->hc->m2l_total - h->hc->m2lx_total;
+    h->stats.ms_m2l_same = 0.0;
+    h->stats.m2l_same_launches = 0;
+    for (uint32_t l = 0; l <= (uint32_t)MAX_LEVELS; ++l) {
+        if (!h->lvl_timed[l]) continue;
+        float m = 0;
+        if (hipEventElapsedTime(&m, h->ev_lvl[2 * l], h->ev_lvl[2 * l + 1]) == hipSuccess) {
+            h->stats.ms_m2l_same += m;
+            h->stats.m2l_same_launches += 1;
+        }
+        h->lvl_timed[l] = false;
+    }
     if (h->timing) {
         // phases on the side stream overlap the others: moments run beside
         // the walk, P2P beside M2L and L2L; the total is the main stream's
@@ -813,6 +880,7 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
     CCHK(hipEventCreate(&h->evc0));
     CCHK(hipEventCreateWithFlags(&h->ev_same, hipEventDisableTiming));
     for (auto& e : h->ev_depth) CCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : h->ev_lvl) CCHK(hipEventCreate(&e));
     for (auto& e : h->ev) CCHK(hipEventCreate(&e));
     for (auto& e : h->evb) CCHK(hipEventCreate(&e));
     for (hipEvent_t* e : {&h->ev_fork, &h->ev_side, &h->ev_walk, &h->ev_near})
@@ -841,7 +909,50 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
     if (cfg->solver == NBH_SOLVER_FMM) {
         int r = alloc_nodes(h, (uint32_t)std::min<size_t>(n / 2 + 64, 0x7fffffff));
         if (r) return bail(r);
+        // candidate same-level offsets o of the matrix-core M2L, with the mask of
+        // the target classes c (position in the parent) that take them: the
+        // criterion accepts o and rejects the parent offset floor((c + o) / 2)
+        const int num = h->theta_num, den = h->theta_den;
+        const int K = 2 * (int)std::floor(std::sqrt(3.0) * den / num) + 1;
+        int dev_lds = 0;
+        CCHK(hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+        if (K <= 15 && h->nsys == 1 && off_supported((int)cfg->order) &&
+            off_lds_bytes((int)cfg->order, cfg->m2l == NBH_M2L_BF16X3, 4 + 2 * K) <= (size_t)dev_lds) {
+            auto accepts = [&](int x, int y, int z) {
+                return (long long)num * num * (x * x + y * y + z * z) > 3ll * den * den;
+            };
+            auto fdiv2 = [](int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); };
+            std::vector<uint32_t> cand;
+            for (int ox = -K; ox <= K; ++ox)
+                for (int oy = -K; oy <= K; ++oy)
+                    for (int oz = -K; oz <= K; ++oz) {
+                        if (!accepts(ox, oy, oz)) continue;
+                        uint32_t mask = 0;
+                        for (int c = 0; c < 8; ++c) {
+                            const int px = fdiv2(((c >> 2) & 1) + ox), py = fdiv2(((c >> 1) & 1) + oy),
+                                      pz = fdiv2((c & 1) + oz);
+                            if (!accepts(px, py, pz)) mask |= 1u << c;
+                        }
+                        if (mask)
+                            cand.push_back((uint32_t)(ox + 16) | (uint32_t)(oy + 16) << 5 | (uint32_t)(oz + 16) << 10 |
+                                           mask << 16);
+                    }
+            CCHK(h->cand.ensure(cand.size() * 4));
+            CCHK(hipMemcpy(h->cand.p, cand.data(), cand.size() * 4, hipMemcpyHostToDevice));
+            std::vector<uint16_t> hk;
+            std::vector<uint8_t> pr;
+            off_tables((int)cfg->order, cfg->m2l == NBH_M2L_BF16X3, hk, pr);
+            CCHK(h->hank.ensure(hk.size() * 2));
+            CCHK(hipMemcpy(h->hank.p, hk.data(), hk.size() * 2, hipMemcpyHostToDevice));
+            CCHK(h->opar.ensure(pr.size()));
+            CCHK(hipMemcpy(h->opar.p, pr.data(), pr.size(), hipMemcpyHostToDevice));
+            h->ncand = (int)cand.size();
+            h->offK = K;
+        }
     }
+    h->stats.m2l_same_order = cfg->solver != NBH_SOLVER_FMM ? 0
+                              : use_offset_m2l(h) ? 2 * cfg->order
+                                                  : (cfg->m2l_order ? cfg->m2l_order : cfg->order + 1);
     CCHK(hipMemcpy(h->sys_off.p, sys_off, (nsys + 1) * 4, hipMemcpyHostToDevice));
     {
         std::vector<double> buf(n);
@@ -900,8 +1011,13 @@ void nbh_destroy(nbh_handle* h) {
                      &h->node_begin, &h->leaf_ids, &h->M, &h->L, &h->Lx, &h->pf, &h->pleaf, &h->dtab, &h->xtab, &h->far_src, &h->far_tgt, &h->row_node, &h->row_first,
                      &h->row_next, &h->row_sl_start, &h->row_sl_cnt, &h->row_m_start, &h->row_m_cnt,
                      &h->row_mx_cnt, &h->row_p_start, &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->counters, &h->diag,
-                     &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf};
+                     &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf, &h->mb, &h->mbw, &h->d2tab,
+                     &h->cand, &h->hank, &h->opar};
     for (DevBuf* b : all) b->release();
+    for (DevBuf& b : h->adj_ckpt) b.release();
+    for (DevBuf& b : h->adj_seg) b.release();
+    DevBuf* adj[] = {&h->adj_final, &h->adj_xb, &h->adj_vb, &h->adj_mb, &h->adj_w};
+    for (DevBuf* b : adj) b->release();
     if (h->hc) (void)hipHostFree(h->hc);
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -910,6 +1026,8 @@ void nbh_destroy(nbh_handle* h) {
     for (hipEvent_t e : {h->ev_fork, h->ev_side, h->ev_walk, h->ev_near, h->ev_same, h->ev_same4, h->evc0})
         if (e) (void)hipEventDestroy(e);
     for (auto& e : h->ev_depth)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : h->ev_lvl)
         if (e) (void)hipEventDestroy(e);
     if (h->stream4) (void)hipStreamDestroy(h->stream4);
     if (h->stream3) (void)hipStreamDestroy(h->stream3);
@@ -1140,12 +1258,12 @@ int nbh_set_timing(nbh_handle* h, int enable) {
     return NBH_OK;
 }
 
-int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
-    if (!h) return NBH_E_INVALID;
-    if (!w || !gx || !gm) return h->fail(NBH_E_INVALID, "w, gx and gm are required");
-    if (h->cfg.solver != NBH_SOLVER_FMM) return h->fail(NBH_E_INVALID, "nbh_force_vjp needs the FMM solver");
-    if (!h->gravity) return h->fail(NBH_E_INVALID, "nbh_force_vjp needs charge == mass (gravity)");
-    HIPCHK(h, hipSetDevice(h->device));
+}  // extern "C"
+
+// VJP of the accelerations at the current state with the cotangent w_dev
+// (device, input order, 3n): g = (da/dx)^T w, (da/dm)^T w into h->gbuf in the
+// current (sorted) order, as four arrays of n doubles (gx, gy, gz, gm).
+static int vjp_device(nbh_handle* h, const double* w_dev) {
     const size_t n = h->n;
     // tree, lists and mass expansions at the current positions
     CHK(force_fmm(h, 0.0, true));
@@ -1155,9 +1273,7 @@ int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
     HIPCHK(h, h->Mw.ensure((size_t)h->num_nodes * ncp * 4));
     HIPCHK(h, h->Lw.ensure((size_t)h->num_nodes * ncp * 4));
     HIPCHK(h, h->pw.ensure(n * 16));
-    HIPCHK(h, h->wbuf.ensure(n * 24));
     HIPCHK(h, h->gbuf.ensure(n * 32));
-    HIPCHK(h, hipMemcpyAsync(h->wbuf.p, w, n * 24, hipMemcpyHostToDevice, h->stream));
     FmmParams F = h->lastF;
     FmmParams Fw = F;
     Fw.M = h->Mw.as<float>();
@@ -1165,17 +1281,73 @@ int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
     State S = h->st[h->cur].view();
     float4* pw = h->pw.as<float4>();
     double* gdev = h->gbuf.as<double>();
-    launch_vjp_gather(h->wbuf.as<double>(), S.id, pw, (uint32_t)n, h->stream);
+    launch_vjp_gather(w_dev, S.id, pw, (uint32_t)n, h->stream);
     // the near field on the side stream, beside the transposed far-field chain
     HIPCHK(h, hipEventRecord(h->ev_fork, h->stream));
     HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
     launch_vjp_near(p, F, pw, (uint32_t)n, h->num_leaves, gdev, h->stream2);
     HIPCHK(h, hipEventRecord(h->ev_near, h->stream2));
-    launch_vjp_chain(p, m2l_q(h), m2l_plan(h), F, Fw, pw, h->num_leaves, h->hc->level_count, h->hc->level_off,
+    M2LPlan plan = m2l_plan(h);
+    if (plan.mfma) {
+        HIPCHK(h, h->mbw.ensure(off_moment_bytes(p, plan.mfma == 1, h->num_nodes)));
+        plan.off.mb = h->mbw.p;  // the chain splits the dipole moments into it
+    }
+    launch_vjp_chain(p, m2l_q(h), plan, F, Fw, pw, h->num_leaves, h->hc->level_count, h->hc->level_off,
                      (int)h->depth, h->stream);
     HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_near, 0));
     launch_vjp_far(p, F, Fw, pw, (uint32_t)n, gdev, h->stream);
     HIPCHK(h, hipGetLastError());
+    return NBH_OK;
+}
+
+static int vjp_checks(nbh_handle* h, const char* what) {
+    if (h->cfg.solver != NBH_SOLVER_FMM) return h->fail(NBH_E_INVALID, "%s needs the FMM solver", what);
+    if (!h->gravity) return h->fail(NBH_E_INVALID, "%s needs charge == mass (gravity)", what);
+    return NBH_OK;
+}
+
+// ---- trajectory adjoint ---------------------------------------------------
+// A state snapshot: x, y, z, vx, vy, vz, mass, charge (FP64), ax, ay, az, pot
+// (FP32) and id, in the current order (84 bytes per particle).
+static size_t snapshot_bytes(size_t n) { return n * (8 * 8 + 4 * 4 + 4); }
+
+static int snapshot_save(nbh_handle* h, DevBuf& dst) {
+    const size_t n = h->n;
+    HIPCHK(h, dst.ensure(snapshot_bytes(n)));
+    const StateBufs& sb = h->st[h->cur];
+    const DevBuf* d8[] = {&sb.x, &sb.y, &sb.z, &sb.vx, &sb.vy, &sb.vz, &sb.mass, &sb.charge};
+    const DevBuf* d4[] = {&sb.ax, &sb.ay, &sb.az, &sb.pot, &sb.id};
+    char* o = (char*)dst.p;
+    for (const DevBuf* b : d8) { HIPCHK(h, hipMemcpyAsync(o, b->p, n * 8, hipMemcpyDeviceToDevice, h->stream)); o += n * 8; }
+    for (const DevBuf* b : d4) { HIPCHK(h, hipMemcpyAsync(o, b->p, n * 4, hipMemcpyDeviceToDevice, h->stream)); o += n * 4; }
+    return NBH_OK;
+}
+
+static int snapshot_restore(nbh_handle* h, const DevBuf& src, double t) {
+    const size_t n = h->n;
+    StateBufs& sb = h->st[h->cur];
+    DevBuf* d8[] = {&sb.x, &sb.y, &sb.z, &sb.vx, &sb.vy, &sb.vz, &sb.mass, &sb.charge};
+    DevBuf* d4[] = {&sb.ax, &sb.ay, &sb.az, &sb.pot, &sb.id};
+    const char* o = (const char*)src.p;
+    for (DevBuf* b : d8) { HIPCHK(h, hipMemcpyAsync(b->p, o, n * 8, hipMemcpyDeviceToDevice, h->stream)); o += n * 8; }
+    for (DevBuf* b : d4) { HIPCHK(h, hipMemcpyAsync(b->p, o, n * 4, hipMemcpyDeviceToDevice, h->stream)); o += n * 4; }
+    h->time = t;
+    h->acc_valid = true;  // the snapshot carries the accelerations of its positions
+    return NBH_OK;
+}
+
+extern "C" {
+
+int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
+    if (!h) return NBH_E_INVALID;
+    if (!w || !gx || !gm) return h->fail(NBH_E_INVALID, "w, gx and gm are required");
+    CHK(vjp_checks(h, "nbh_force_vjp"));
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t n = h->n;
+    HIPCHK(h, h->wbuf.ensure(n * 24));
+    HIPCHK(h, hipMemcpyAsync(h->wbuf.p, w, n * 24, hipMemcpyHostToDevice, h->stream));
+    CHK(vjp_device(h, h->wbuf.as<double>()));
+    State S = h->st[h->cur].view();
     std::vector<double> g(4 * n);
     std::vector<uint32_t> ids(n);
     HIPCHK(h, hipMemcpyAsync(g.data(), h->gbuf.p, 4 * n * 8, hipMemcpyDeviceToHost, h->stream));
@@ -1188,6 +1360,88 @@ int nbh_force_vjp(nbh_handle* h, const double* w, double* gx, double* gm) {
         gx[3 * j + 2] = g[2 * n + i];
         gm[j] = g[3 * n + i];
     }
+    return NBH_OK;
+}
+
+int nbh_adjoint_forward(nbh_handle* h, int nsteps, int ckpt_every) {
+    if (!h) return NBH_E_INVALID;
+    CHK(vjp_checks(h, "nbh_adjoint_forward"));
+    if (h->cfg.integrator != NBH_INT_KDK) return h->fail(NBH_E_INVALID, "nbh_adjoint_forward needs the KDK integrator");
+    if (nsteps < 1 || ckpt_every < 1) return h->fail(NBH_E_INVALID, "nsteps and ckpt_every must be >= 1");
+    HIPCHK(h, hipSetDevice(h->device));
+    const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
+    if ((int)h->adj_ckpt.size() < nck) h->adj_ckpt.resize(nck);
+    h->adj_ckpt_t.assign(nck, 0.0);
+    h->adj_ready = false;
+    if (!h->acc_valid) { CHK(force_eval(h, 0.0)); h->acc_valid = true; }
+    for (int s = 0; s < nsteps; ++s) {
+        if (s % ckpt_every == 0) {
+            CHK(snapshot_save(h, h->adj_ckpt[s / ckpt_every]));
+            h->adj_ckpt_t[s / ckpt_every] = h->time;
+        }
+        CHK(do_step(h));
+    }
+    CHK(snapshot_save(h, h->adj_final));
+    h->adj_final_t = h->time;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->adj_nsteps = nsteps;
+    h->adj_k = ckpt_every;
+    h->adj_ready = true;
+    return NBH_OK;
+}
+
+int nbh_adjoint_backward(nbh_handle* h, const double* xbar, const double* vbar, double* gx, double* gv, double* gm) {
+    if (!h) return NBH_E_INVALID;
+    if (!xbar || !vbar || !gx || !gv || !gm) return h->fail(NBH_E_INVALID, "xbar, vbar, gx, gv and gm are required");
+    if (!h->adj_ready) return h->fail(NBH_E_INVALID, "nbh_adjoint_backward needs a preceding nbh_adjoint_forward");
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t n = h->n;
+    const double dt = h->cfg.dt;
+    HIPCHK(h, h->adj_xb.ensure(n * 24));
+    HIPCHK(h, h->adj_vb.ensure(n * 24));
+    HIPCHK(h, h->adj_mb.ensure(n * 8));
+    HIPCHK(h, h->adj_w.ensure(n * 24));
+    double* xb = h->adj_xb.as<double>();
+    double* vb = h->adj_vb.as<double>();
+    double* mbar = h->adj_mb.as<double>();
+    double* w = h->adj_w.as<double>();
+    HIPCHK(h, hipMemcpyAsync(xb, xbar, n * 24, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(vb, vbar, n * 24, hipMemcpyHostToDevice, h->stream));
+    launch_fill_f64(mbar, n, 0.0, h->stream);
+    // the last state: second half kick of the last step, cotangent dt/2 vbar
+    CHK(snapshot_restore(h, h->adj_final, h->adj_final_t));
+    launch_adj_kick(xb, vb, w, 3 * n, dt, 0.5 * dt, false, h->stream);
+    CHK(vjp_device(h, w));
+    launch_adj_accum(h->gbuf.as<double>(), h->st[h->cur].id.as<uint32_t>(), (uint32_t)n, xb, mbar, h->stream);
+    const int nseg = (int)h->adj_ckpt_t.size();
+    if ((int)h->adj_seg.size() < h->adj_k) h->adj_seg.resize(h->adj_k);
+    h->adj_seg_t.assign(h->adj_k, 0.0);
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int start = sg * h->adj_k, len = std::min(h->adj_k, h->adj_nsteps - start);
+        // recompute the segment from its checkpoint, keeping each state
+        CHK(snapshot_restore(h, h->adj_ckpt[sg], h->adj_ckpt_t[sg]));
+        for (int j = 0; j < len; ++j) {
+            CHK(snapshot_save(h, h->adj_seg[j]));
+            h->adj_seg_t[j] = h->time;
+            if (j + 1 < len) CHK(do_step(h));
+        }
+        // reverse it: vbar_h = vbar + dt xbar; xbar += VJP(x_j, c vbar_h); vbar = vbar_h
+        for (int j = len - 1; j >= 0; --j) {
+            const double c = (start + j == 0 ? 0.5 : 1.0) * dt;
+            CHK(snapshot_restore(h, h->adj_seg[j], h->adj_seg_t[j]));
+            launch_adj_kick(xb, vb, w, 3 * n, dt, c, true, h->stream);
+            CHK(vjp_device(h, w));
+            launch_adj_accum(h->gbuf.as<double>(), h->st[h->cur].id.as<uint32_t>(), (uint32_t)n, xb, mbar,
+                             h->stream);
+        }
+    }
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(gx, xb, n * 24, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(gv, vb, n * 24, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(gm, mbar, n * 8, hipMemcpyDeviceToHost, h->stream));
+    // the handle ends at the trajectory's last state
+    CHK(snapshot_restore(h, h->adj_final, h->adj_final_t));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return NBH_OK;
 }
 

@@ -266,6 +266,9 @@ static void launch_vjp_chain_t(int q, const M2LPlan& plan, const FmmParams& F, c
     hipLaunchKernelGGL((dipole_p2m_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, pw, Fw.M,
                        num_leaves);
     for (int l = depth - 1; l >= 0; --l) launch_m2m_level(P, Fw, l, level_count[l], level_off[l], st);
+    // the matrix-core same-level kernels read the dipole moments split (into plan.off.mb)
+    if (plan.mfma)
+        launch_off_moments(P, plan.mfma == 1, Fw.M, Fw.ncp, plan.num_nodes, Fw.nodes.geo, Fw.box, (void*)plan.off.mb, st);
     launch_m2l(P, q, Fw, plan, st);
     for (int l = 1; l <= depth; ++l) launch_l2l_level(P, Fw, l, level_count[l], level_off[l], st);
 }
@@ -280,6 +283,46 @@ static void launch_vjp_chain_t(int q, const M2LPlan& plan, const FmmParams& F, c
         case 6: { constexpr int PP = 6; call; } break; \
         default: break;                                \
     }
+
+// ---- reverse leapfrog on the device (trajectory adjoint) --------------------
+// cotangents in input order: update: vbar += dt xbar; then w = c vbar
+__global__ void adj_kick_kernel(double* __restrict__ xb, double* __restrict__ vb, double* __restrict__ w, size_t m,
+                                double dt, double c, int update) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double v = vb[i];
+    if (update) {
+        v += dt * xb[i];
+        vb[i] = v;
+    }
+    w[i] = c * v;
+}
+// xbar[id] += g (sorted order, gx | gy | gz | gm arrays of n), mbar[id] += gm
+__global__ void adj_accum_kernel(const double* __restrict__ g, const uint32_t* __restrict__ id, uint32_t n,
+                                 double* __restrict__ xb, double* __restrict__ mb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t j = id[i];
+    xb[3 * j] += g[i];
+    xb[3 * j + 1] += g[(size_t)n + i];
+    xb[3 * j + 2] += g[2 * (size_t)n + i];
+    mb[j] += g[3 * (size_t)n + i];
+}
+__global__ void fill_f64_kernel(double* p, size_t m, double v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) p[i] = v;
+}
+void launch_adj_kick(double* xb, double* vb, double* w, size_t m, double dt, double c, bool update, hipStream_t st) {
+    hipLaunchKernelGGL(adj_kick_kernel, dim3((unsigned)ceil_div(m, (size_t)256)), dim3(256), 0, st, xb, vb, w, m, dt, c,
+                       update ? 1 : 0);
+}
+void launch_adj_accum(const double* g, const uint32_t* id, uint32_t n, double* xb, double* mb, hipStream_t st) {
+    hipLaunchKernelGGL(adj_accum_kernel, dim3(ceil_div(n, 256u)), dim3(256), 0, st, g, id, n, xb, mb);
+}
+void launch_fill_f64(double* p, size_t m, double v, hipStream_t st) {
+    if (!m) return;
+    hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)ceil_div(m, (size_t)256)), dim3(256), 0, st, p, m, v);
+}
 
 void launch_vjp_gather(const double* w_in, const uint32_t* id, float4* pw, uint32_t n, hipStream_t st) {
     hipLaunchKernelGGL(vjp_gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, w_in, id, pw, n);

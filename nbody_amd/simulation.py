@@ -108,6 +108,8 @@ class HipSimulation(Simulation):
             raise RuntimeError(f"nbh_create failed ({rc}): {L.nbh_last_error(None).decode()}")
         self._h = h
         self._del = L.nbh_destroy
+        # truncation of the pairs of one level (2p on the matrix cores: every term |k|, |n| <= p)
+        self.same_level_order = int(self.stats()["m2l_same_order"]) or None
         if self._log is not None:
             self._log.write(f"HipSimulation: {self.n} particles, solver {cfg.solver}, p={cfg.order}\n")
 

@@ -539,6 +539,14 @@ static void cell_centre(const Tree& t, const double* box, int node, double* c) {
     c[2] = box[4 * s + 2] + (t.iz[node] + 0.5) * h;
 }
 
+// qm2l = q | q_same << 8: the M2L truncation of pairs of different levels (q)
+// and of pairs of one level (q_same; 0 = q).  q_same = 2p keeps every term
+// |k|, |n| <= p, the dense operator of the matrix-core same-level kernel.
+static int pair_q(const Tree& t, int qm2l, int a, int b) {
+    const int q = qm2l & 0xff, qs = (qm2l >> 8) & 0xff;
+    return (qs && t.level[a] == t.level[b]) ? qs : q;
+}
+
 void orc_fmm(const orc_tree* tr, const orc_lists* li, int p, int qm2l, size_t n, const double* x,
              const double* q, const double* box, double G, double eps,
              double* acc, double* pot) {
@@ -566,7 +574,8 @@ void orc_fmm(const orc_tree* tr, const orc_lists* li, int p, int qm2l, size_t n,
     for (long long a = 0; a < (long long)nn; ++a)
         for (int64_t e = L.m2l_off[a]; e < L.m2l_off[a + 1]; ++e) {
             int b = L.m2l_src[e];
-            orc_m2l(p, qm2l, &cen[3 * a], &cen[3 * b], eps, &M[(size_t)b * nc], &Lc[(size_t)a * nc]);
+            orc_m2l(p, pair_q(t, qm2l, (int)a, b), &cen[3 * a], &cen[3 * b], eps, &M[(size_t)b * nc],
+                    &Lc[(size_t)a * nc]);
         }
     // L2L downwards (BFS order: parents before children)
     for (size_t i = 0; i < nn; ++i)
@@ -647,8 +656,9 @@ void orc_fmm_vjp(const orc_tree* tr, const orc_lists* li, int p, int qm2l, size_
     for (long long a = 0; a < (long long)nn; ++a)
         for (int64_t e = L.m2l_off[a]; e < L.m2l_off[a + 1]; ++e) {
             int b = L.m2l_src[e];
-            orc_m2l(p, qm2l, &cen[3 * a], &cen[3 * b], eps, &M[(size_t)b * nc], &Lc[(size_t)a * nc]);
-            orc_m2l(p, qm2l, &cen[3 * a], &cen[3 * b], eps, &Mw[(size_t)b * nc], &Lw[(size_t)a * nc]);
+            const int qp = pair_q(t, qm2l, (int)a, b);
+            orc_m2l(p, qp, &cen[3 * a], &cen[3 * b], eps, &M[(size_t)b * nc], &Lc[(size_t)a * nc]);
+            orc_m2l(p, qp, &cen[3 * a], &cen[3 * b], eps, &Mw[(size_t)b * nc], &Lw[(size_t)a * nc]);
         }
     for (size_t a = 0; a < nn; ++a)
         for (int c = 0; c < t.nchild[a]; ++c) {

@@ -103,7 +103,9 @@ void orc_l2p(int p, const double* z, const double* L, const double* x, double* p
 
 /* ---- full FMM force evaluation (FP64), same tree/lists as the product ---- */
 /* x[3n] in sorted order, q[n]; returns acc (without the q_i/m_i factor) and
- * potential per particle, in the same order.  box[4*nsys] from orc_morton. */
+ * potential per particle, in the same order.  box[4*nsys] from orc_morton.
+ * qm2l = q | q_same << 8: M2L truncation (orc_m2l's q) of pairs of different
+ * levels, and of pairs of one level (q_same; 0 = the same q). */
 void orc_fmm(const orc_tree*, const orc_lists*, int p, int qm2l, size_t n, const double* x,
              const double* q, const double* box, double G, double eps,
              double* acc, double* pot);

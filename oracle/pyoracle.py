@@ -237,19 +237,28 @@ def l2p(p, z, L, x):
     return phi.value, g
 
 
-def fmm(tree: Tree, lists: Lists, p, x_sorted, q_sorted, box, G=1.0, eps=0.0, m2l_order=None):
+def _qcode(p, m2l_order, same_level_order):
+    q = p if m2l_order is None else int(m2l_order)
+    return q | ((int(same_level_order) if same_level_order else 0) << 8)
+
+
+def fmm(tree: Tree, lists: Lists, p, x_sorted, q_sorted, box, G=1.0, eps=0.0, m2l_order=None,
+        same_level_order=None):
+    """same_level_order: M2L truncation of the pairs of one level (None: m2l_order);
+    2p keeps every term |k|, |n| <= p (the dense operator)."""
     x = c64(x_sorted).reshape(-1, 3)
     q = c64(q_sorted)
     box = c64(box).reshape(-1)
     acc = np.zeros_like(x)
     pot = np.zeros(len(x))
     lib().orc_fmm(ctypes.c_void_p(tree.handle), ctypes.c_void_p(lists.handle), p,
-                  p if m2l_order is None else int(m2l_order), len(x),
+                  _qcode(p, m2l_order, same_level_order), len(x),
                   _ptr(x, _dp), _ptr(q, _dp), _ptr(box, _dp), G, eps, _ptr(acc, _dp), _ptr(pot, _dp))
     return acc, pot
 
 
-def fmm_vjp(tree: Tree, lists: Lists, p, x_sorted, m_sorted, w_sorted, box, G=1.0, eps=0.0, m2l_order=None):
+def fmm_vjp(tree: Tree, lists: Lists, p, x_sorted, m_sorted, w_sorted, box, G=1.0, eps=0.0, m2l_order=None,
+            same_level_order=None):
     x = c64(x_sorted).reshape(-1, 3)
     m = c64(m_sorted)
     w = c64(w_sorted).reshape(-1, 3)
@@ -260,7 +269,7 @@ def fmm_vjp(tree: Tree, lists: Lists, p, x_sorted, m_sorted, w_sorted, box, G=1.
     L.orc_fmm_vjp.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
                               _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
     L.orc_fmm_vjp(ctypes.c_void_p(tree.handle), ctypes.c_void_p(lists.handle), p,
-                  p if m2l_order is None else int(m2l_order), len(x), _ptr(x, _dp),
+                  _qcode(p, m2l_order, same_level_order), len(x), _ptr(x, _dp),
                   _ptr(m, _dp), _ptr(w, _dp), _ptr(box, _dp), G, eps, _ptr(gx, _dp), _ptr(gm, _dp))
     return gx, gm
 

@@ -52,3 +52,41 @@ def test_reverse_leapfrog_matches_central_differences():
         e = np.zeros(n); e[i] = h * 1e-2
         fdm = (loss(x0, v0, m + e) - loss(x0, v0, m - e)) / (2 * h * 1e-2)
         assert abs(fdm - gm[i]) < 1e-5 * max(1.0, abs(fdm))
+
+
+def test_fmm_trajectory_gradient_matches_central_differences_with_frozen_lists():
+    """The FP64 oracle of the FMM trajectory gradient (tests/fmm_trajectory.py:
+    orc_fmm forces, two orc_fmm_vjp per step) is the exact derivative of the
+    FMM trajectory with every state's tree, lists and root cube held fixed:
+    central differences (h = 1e-5) of that trajectory agree to <= 1e-5 relative.
+    Plummer N=512 (seed 4), eps=1e-2, theta=0.5, p=4, cap 16, 16 KDK steps at
+    dt=1e-3, loss = mean |x_N - x*|^2 with x* from velocities perturbed by 1e-3."""
+    import nbody_amd as nb
+    from fmm_trajectory import OracleFMMTrajectory
+
+    n, steps = 512, 16
+    x0, v0, m = nb.ics.plummer(n, 4)
+    orc = OracleFMMTrajectory(m, p=4, cap=16, eps=1e-2, dt=1e-3)
+    rng = np.random.default_rng(5)
+    xt, _, _, _ = orc.forward(x0, v0 + 1e-3 * rng.standard_normal((n, 3)), steps)
+    xN, vN, xs, sts = orc.forward(x0, v0, steps)
+    gx, gv, gm = orc.gradient(xs, sts, 2.0 * (xN - xt) / (3 * n), np.zeros_like(vN))
+
+    def loss(x, v, mm=None):
+        xe, _, _, _ = orc.forward(x, v, steps, frozen=sts, m=mm)
+        return float(np.sum((xe - xt) ** 2) / (3 * n))
+
+    h = 1e-5
+    worst = 0.0
+    for s in rng.choice(3 * n, 8, replace=False):
+        i, k = divmod(int(s), 3)
+        e = np.zeros((n, 3)); e[i, k] = h
+        fdx = (loss(x0 + e, v0) - loss(x0 - e, v0)) / (2 * h)
+        fdv = (loss(x0, v0 + e) - loss(x0, v0 - e)) / (2 * h)
+        worst = max(worst, abs(fdx - gx[i, k]) / np.abs(gx).max(), abs(fdv - gv[i, k]) / np.abs(gv).max())
+    for i in rng.choice(n, 4, replace=False):
+        e = np.zeros(n); e[i] = 1e-3 * m[i]
+        fdm = (loss(x0, v0, m + e) - loss(x0, v0, m - e)) / (2e-3 * m[i])
+        worst = max(worst, abs(fdm - gm[i]) / np.abs(gm).max())
+    print(f"oracle FMM trajectory gradient vs central differences: worst {worst:.2e}")
+    assert worst < 1e-5

@@ -46,9 +46,11 @@ def _ics(kind, n, seed=0):
 
 # name, M2L kernel, P2P kernel, M2L truncation (0: q = p+1, the default; -1: q = p),
 # RMS tolerances (acc, pot) against the FP64 oracle FMM.  The MFMA kernels take
-# the same-level pairs and are held to the FP32 tolerances: bf16x3 drops the
-# lo*lo term and rounds both halves to bf16 (about 2^-16 relative per product),
-# which the far field's share of each force keeps below them.
+# the same-level pairs by translation offset with the dense operator (every
+# term |k|, |n| <= p; the oracle is run with that truncation for those pairs)
+# and are held to the FP32 tolerances: bf16x3 drops the lo*lo term and rounds
+# both halves to bf16 (about 2^-16 relative per product), which the far
+# field's share of each force keeps below them.
 KERNELS = [("tuned", nb.M2L_GROUPED, nb.P2P_LANE, 0, 2e-5, 2e-6),
            ("tuned_qp", nb.M2L_GROUPED, nb.P2P_LANE, -1, 2e-5, 2e-6),
            ("plain", nb.M2L_PLAIN, nb.P2P_LEAF, 0, 2e-5, 2e-6),
@@ -85,7 +87,8 @@ def test_tree_lists_forces_match_oracle(name, kind, n, cap, p, th, eps, kern, m2
     assert np.array_equal(pofs, lists.p2p_off)
     assert np.array_equal(ps, lists.p2p_src)
     # forces: FP32 kernels vs FP64 oracle FMM on the same tree and lists
-    acc_o, pot_o = po.fmm(tree, lists, p, xs, qs, box_o, G=1.0, eps=eps, m2l_order=sim.m2l_order)
+    acc_o, pot_o = po.fmm(tree, lists, p, xs, qs, box_o, G=1.0, eps=eps, m2l_order=sim.m2l_order,
+                          same_level_order=sim.same_level_order)
     inv = np.empty_like(ids)
     inv[ids] = np.arange(n, dtype=np.uint32)
     acc_s = acc[ids]
