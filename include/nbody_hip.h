@@ -174,7 +174,10 @@ int nbh_direct_fp64_noalloc(nbh_handle* h, size_t nt, const int64_t* targets, do
 int nbh_diagnostics(nbh_handle* h, double* out);
 
 int nbh_get_stats(nbh_handle* h, nbh_stats* s);
-/* Enables per-phase hipEvent timing of force evaluations (adds syncs). */
+/* Per-phase hipEvent timing of force evaluations: 0 off, 1 on, 2 on with the
+ * whole evaluation serialized on the handle's main stream (each kernel then
+ * runs alone, for standalone kernel durations; results are unchanged up to
+ * the FP32 summation order of the far-field parts). */
 int nbh_set_timing(nbh_handle* h, int enable);
 
 /* ---- adjoint (no counterpart in the reference) --------------------------- */

@@ -164,7 +164,7 @@ struct OffsetParams {
 // the host tables of the kernel (uploaded once per handle)
 void off_tables(int p, bool bf3, std::vector<uint16_t>& hank, std::vector<uint8_t>& par);
 bool off_supported(int p);
-size_t off_lds_bytes(int p, bool bf3, int W);
+size_t off_lds_bytes(int p, bool bf3, int W, int ncand);
 size_t off_moment_bytes(int p, bool bf3, uint32_t nn);
 int off_table_words(int p);
 void launch_off_moments(int p, bool bf3, const float* M, int ncp, uint32_t nn, const int4* geo, const double4* box,

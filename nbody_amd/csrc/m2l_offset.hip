@@ -91,6 +91,16 @@ constexpr uint32_t OFF_PAD = 0xffffffffu;  // batch slot without an offset
 // the workgroup's dynamic LDS (no static LDS in this file, so it starts at address 0)
 extern __shared__ __align__(16) unsigned char off_lds[];
 
+// LDS writes of one lane made visible to the other lanes of its wavefront,
+// ordering LDS only: the tensor rows of the next batch stay in flight from
+// global memory across it (a fence over every address space would wait for
+// them)
+__device__ __forceinline__ void off_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 template <int P>
 struct OffDims {
     static constexpr int NC = ncoef(P);
@@ -115,8 +125,10 @@ struct OffLayout {
     static constexpr size_t SB = DB + (size_t)OFF_WAVES * OFF_J * D::DCP * 4;  // int [wave][2][OFF_J][64] sources
     static constexpr size_t HK = SB + (size_t)OFF_WAVES * 2 * OFF_J * WAVE * 4;  // u16 [OFF_J][KR][KC] A indices
     static constexpr size_t WIN = HK + (((size_t)OFF_J * D::KR * KC * 2 + 15) & ~(size_t)15);  // int [W^3]
-    static __host__ __device__ size_t total(int W) {
-        const size_t main = WIN + (size_t)W * W * W * 4;
+    // int [wave][ceil(ncand / 4)]: the wavefront's candidates that some target takes
+    static __host__ __device__ size_t CL(int W) { return WIN + (size_t)W * W * W * 4; }
+    static __host__ __device__ size_t total(int W, int ncand) {
+        const size_t main = CL(W) + (size_t)OFF_WAVES * ((ncand + OFF_WAVES - 1) / OFF_WAVES) * 4;
         const size_t red = HDR + (size_t)(OFF_WAVES - 1) * D::KR * WAVE * 4;  // partial sums of waves 1..3
         return main > red ? main : red;
     }
@@ -251,17 +263,19 @@ __device__ __forceinline__ void off_batch(of32x4_t (&acc)[OffDims<P>::MT][4], co
     const uint16_t* hk = (const uint16_t*)(off_lds + Lay::HK) + ((size_t)hh * D::KR + r) * KC;
     if constexpr (BF3) {
         const unsigned char* mb = (const unsigned char*)O.mb;
+        // the moments of the four target tiles, empty slots and tiles past
+        // ntile reading the zero row: the loads carry no branches, so that a
+        // K-step's are issued one step ahead of its products
         const unsigned char* bp[4];
-        bool bv[4];  // the slot has a source: empty slots load nothing (zero operands)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int src = t < ntile ? sb[hh * WAVE + 16 * t + r] : (int)O.zero_node;
-            bv[t] = src != (int)O.zero_node;
             bp[t] = mb + (size_t)src * (D::NG * 32);
         }
-#pragma unroll
-        for (int s = 0; s < D::NG; ++s) {
-            obf16x8_t ahi[MT], alo[MT];
+        uint4 bq[2][4][2];
+        obf16x8_t ahi[2][MT], alo[2][MT];
+        // A fragments of K-step s (LDS gathers of the Hankel-indexed tensor words)
+        auto build_a = [&](int s, obf16x8_t (&hi)[MT], obf16x8_t (&lo)[MT]) {
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const uint4 hx = *(const uint4*)(hk + (size_t)16 * mt * KC + 8 * s);
@@ -274,35 +288,49 @@ __device__ __forceinline__ void off_batch(of32x4_t (&acc)[OffDims<P>::MT][4], co
                     hv[q] = __builtin_amdgcn_perm(w1, w0, 0x07060302u);  // hi halves: w0.hi | w1.hi << 16
                     lv[q] = __builtin_amdgcn_perm(w1, w0, 0x05040100u);  // lo halves: w0.lo | w1.lo << 16
                 }
-                ahi[mt] = __builtin_bit_cast(obf16x8_t, hv);
-                alo[mt] = __builtin_bit_cast(obf16x8_t, lv);
+                hi[mt] = __builtin_bit_cast(obf16x8_t, hv);
+                lo[mt] = __builtin_bit_cast(obf16x8_t, lv);
             }
+        };
+        auto load_b = [&](int s, uint4 (&bv)[4][2]) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                if (t >= ntile) break;
-                uint4 uh = make_uint4(0, 0, 0, 0), ul = make_uint4(0, 0, 0, 0);
-                if (bv[t]) {
-                    uh = *(const uint4*)(bp[t] + 32 * s);
-                    ul = *(const uint4*)(bp[t] + 32 * s + 16);
-                }
-                const obf16x8_t bhi = __builtin_bit_cast(obf16x8_t, uh);
-                const obf16x8_t blo = __builtin_bit_cast(obf16x8_t, ul);
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[mt], bhi, acc[mt][t], 0, 0, 0);
-                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[mt], blo, acc[mt][t], 0, 0, 0);
-                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[mt], bhi, acc[mt][t], 0, 0, 0);
-                }
+                bv[t][0] = *(const uint4*)(bp[t] + 32 * s);
+                bv[t][1] = *(const uint4*)(bp[t] + 32 * s + 16);
             }
+        };
+        // software pipeline: the operands of K-step s + 1 (moment loads, tensor
+        // gathers) are issued ahead of the products of step s
+        load_b(0, bq[0]);
+        build_a(0, ahi[0], alo[0]);
+#pragma unroll
+        for (int s = 0; s < D::NG; ++s) {
+            if (s + 1 < D::NG) {
+                load_b(s + 1, bq[(s + 1) & 1]);
+                build_a(s + 1, ahi[(s + 1) & 1], alo[(s + 1) & 1]);
+            }
+            // the three split products in turn over all tiles, so that
+            // consecutive MFMAs write different accumulators
+#pragma unroll
+            for (int pr = 0; pr < 3; ++pr)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (t < ntile) {
+                        const obf16x8_t bhi = __builtin_bit_cast(obf16x8_t, bq[s & 1][t][0]);
+                        const obf16x8_t blo = __builtin_bit_cast(obf16x8_t, bq[s & 1][t][1]);
+#pragma unroll
+                        for (int mt = 0; mt < MT; ++mt)
+                            acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                pr == 0 ? alo[s & 1][mt] : ahi[s & 1][mt], pr == 1 ? blo : bhi, acc[mt][t], 0, 0, 0);
+                    }
+                }
         }
     } else {
         const float* mf = (const float*)O.mb;
         const float* bp[4];
-        bool bv[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int src = t < ntile ? sb[hh * WAVE + 16 * t + r] : (int)O.zero_node;
-            bv[t] = src != (int)O.zero_node;
             bp[t] = mf + (size_t)src * KC;
         }
         // K-step s: coefficient n = s of the offset of lane group h
@@ -310,13 +338,16 @@ __device__ __forceinline__ void off_batch(of32x4_t (&acc)[OffDims<P>::MT][4], co
             float a[MT];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) a[mt] = *(const float*)(off_lds + DBW + hk[(size_t)16 * mt * KC + s]);
+            float bb[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bb[t] = bp[t][s];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                if (t >= ntile) break;
-                const float bb = bv[t] ? bp[t][s] : 0.f;
+                if (t < ntile) {
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mt], bb, acc[mt][t], 0, 0, 0);
+                    for (int mt = 0; mt < MT; ++mt)
+                        acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mt], bb[t], acc[mt][t], 0, 0, 0);
+                }
             }
         }
     }
@@ -408,21 +439,36 @@ void m2l_offset_kernel(FmmParams F, OffsetParams O, uint32_t goff, int lvl) {
     }
     __syncthreads();
     // window: every same-level source of the block's targets at its position,
-    // the entries of all targets taken flat across the workgroup
+    // the entries of all targets taken flat across the workgroup, eight per
+    // thread at a time so that their list and geometry loads overlap
     const int total = lpre[WAVE];
-    for (int e = tid; e < total; e += OFF_WAVES * WAVE) {
-        int lo = 0, hi = nt - 1;  // the target whose entries hold e
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (lpre[mid] <= e) lo = mid; else hi = mid - 1;
+    {
+        constexpr int U = 8;
+        int tcur = 0;  // target of this thread's current entry (entries only increase)
+        for (int base = 0; base < total; base += U * OFF_WAVES * WAVE) {
+            int Bv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = base + u * OFF_WAVES * WAVE + tid;
+                Bv[u] = -1;
+                if (e < total) {
+                    while (lpre[tcur + 1] <= e) ++tcur;
+                    Bv[u] = F.lists.m_pool[ls0[tcur] + (e - lpre[tcur])];
+                }
+            }
+            int4 gv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) gv[u] = F.nodes.geo[Bv[u] >= 0 ? Bv[u] : 0];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (Bv[u] < 0) continue;
+                const int wx = gv[u].x - 4 * gg.x + K, wy = gv[u].y - 4 * gg.y + K, wz = gv[u].z - 4 * gg.z + K;
+                if ((unsigned)wx < (unsigned)W && (unsigned)wy < (unsigned)W && (unsigned)wz < (unsigned)W)
+                    win[(wx * W + wy) * W + wz] = Bv[u];
+                else
+                    atomicOr(O.flags, OVF_GEOM);
+            }
         }
-        const int B = F.lists.m_pool[ls0[lo] + (e - lpre[lo])];
-        const int4 gb = F.nodes.geo[B];
-        const int wx = gb.x - 4 * gg.x + K, wy = gb.y - 4 * gg.y + K, wz = gb.z - 4 * gg.z + K;
-        if ((unsigned)wx < (unsigned)W && (unsigned)wy < (unsigned)W && (unsigned)wz < (unsigned)W)
-            win[(wx * W + wy) * W + wz] = B;
-        else
-            atomicOr(O.flags, OVF_GEOM);
     }
     __syncthreads();
 
@@ -436,40 +482,48 @@ void m2l_offset_kernel(FmmParams F, OffsetParams O, uint32_t goff, int lvl) {
     const uint32_t* tab = O.tab + (size_t)lvl * E3 * DCP;
     int myc = 0;  // pairs of this lane's target taken by this wavefront
 
-    // this wavefront's candidates: wave, wave + 4, ...; 64 of them prefetched
-    // into a register, one per lane
+    // this wavefront's candidates (wave, wave + 4, ...) that some target of
+    // the block takes, found 64 at a time (lane = candidate, independent
+    // window reads over the targets) and listed in LDS
+    uint32_t* clist = (uint32_t*)(off_lds + Lay::CL(W)) + wave * ((O.ncand + OFF_WAVES - 1) / OFF_WAVES);
     const int mine = O.ncand > wave ? (O.ncand - wave + OFF_WAVES - 1) / OFF_WAVES : 0;
-    int ci = 0, cbase = 0;
-    uint32_t creg = lane < mine ? O.cand[wave + OFF_WAVES * lane] : OFF_PAD;
+    int ncl = 0;
+    for (int g0 = 0; g0 < mine; g0 += WAVE) {
+        const uint32_t c = g0 + lane < mine ? O.cand[wave + OFF_WAVES * (g0 + lane)] : OFF_PAD;
+        const int ox = (int)(c & 31) - 16, oy = (int)((c >> 5) & 31) - 16, oz = (int)((c >> 10) & 31) - 16;
+        const int lin = (ox * W + oy) * W + oz;
+        const uint32_t cm = c == OFF_PAD ? 0u : c >> 16;
+        bool any = false;
+        for (int t = 0; t < nt; ++t) {
+            const int tpl = __builtin_amdgcn_readlane(pl, t), tcl = __builtin_amdgcn_readlane(cls, t);
+            if (((cm >> tcl) & 1u) && win[tpl + lin] >= 0) any = true;
+        }
+        const uint64_t bal = __ballot(any);
+        if (any) clist[ncl + __popcll(bal & lanemask_lt())] = c;
+        ncl += __popcll(bal);
+    }
+    int ci = 0;
     uint32_t ccur[OFF_J], cnxt[OFF_J];  // offset codes of the current and the next batch (uniform)
-    // gathers the next OFF_J offsets that some target takes into batch buffer
-    // b (sources; empty slots padded) and cd (codes); returns how many
+    // the next OFF_J listed offsets into batch buffer b: each target's source
+    // (or the zero row) and the codes cd; returns how many
     auto collect = [&](int b, uint32_t (&cd)[OFF_J]) -> int {
         int* sb = sbw + b * OFF_J * WAVE;
-        int jn = 0;
+        off_lds_sync();  // the list is written
+        const int jn = ncl - ci < OFF_J ? ncl - ci : OFF_J;
 #pragma unroll
-        for (int j = 0; j < OFF_J; ++j) cd[j] = OFF_PAD;
-        while (jn < OFF_J && ci < mine) {
-            if (ci - cbase == WAVE) {
-                cbase += WAVE;
-                creg = cbase + lane < mine ? O.cand[wave + OFF_WAVES * (cbase + lane)] : OFF_PAD;
-            }
-            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)creg, ci - cbase);
-            ++ci;
-            const int ox = (int)(c & 31) - 16, oy = (int)((c >> 5) & 31) - 16, oz = (int)((c >> 10) & 31) - 16;
-            const int lin = (ox * W + oy) * W + oz;
+        for (int j = 0; j < OFF_J; ++j) {
+            uint32_t c = OFF_PAD;
             int src = -1;
-            if (tv && ((c >> (16 + cls)) & 1)) src = win[pl + lin];
-            if (__ballot(src >= 0)) {
-                myc += src >= 0;
-                sb[jn * WAVE + lane] = src >= 0 ? src : (int)O.zero_node;
-#pragma unroll
-                for (int j = 0; j < OFF_J; ++j)
-                    if (j == jn) cd[j] = c;
-                ++jn;
+            if (j < jn) {
+                c = clist[ci + j];
+                const int ox = (int)(c & 31) - 16, oy = (int)((c >> 5) & 31) - 16, oz = (int)((c >> 10) & 31) - 16;
+                if (tv && ((c >> (16 + cls)) & 1)) src = win[pl + (ox * W + oy) * W + oz];
             }
+            cd[j] = c;
+            myc += src >= 0;
+            sb[j * WAVE + lane] = src >= 0 ? src : (int)O.zero_node;
         }
-        for (int j = jn; j < OFF_J; ++j) sb[j * WAVE + lane] = (int)O.zero_node;
+        ci += jn;
         return jn;
     };
     // the batch's tensor rows from the table into registers (all loads in
@@ -480,11 +534,11 @@ void m2l_offset_kernel(FmmParams F, OffsetParams O, uint32_t goff, int lvl) {
         for (int j = 0; j < OFF_J; ++j) {
             const uint32_t c = cd[j];
             const int ox = (int)(c & 31) - 16, oy = (int)((c >> 5) & 31) - 16, oz = (int)((c >> 10) & 31) - 16;
-            const uint32_t* row = tab + (size_t)((abs(ox) * E + abs(oy)) * E + abs(oz)) * DCP;
+            const uint32_t* row = tab + (c != OFF_PAD ? (size_t)((abs(ox) * E + abs(oy)) * E + abs(oz)) * DCP : 0);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const int m = lane + WAVE * q;
-                dreg[j][q] = (c != OFF_PAD && m < DCP) ? row[m] : 0u;
+                dreg[j][q] = row[m < DCP ? m : DCP - 1];  // masked at the write
             }
         }
     };
@@ -500,7 +554,8 @@ void m2l_offset_kernel(FmmParams F, OffsetParams O, uint32_t goff, int lvl) {
                 const int m = lane + WAVE * q;
                 if (m < DCP) {
                     const uint32_t flip = (fm[q] >> neg) & 1u;
-                    db[j * DCP + m] = dreg[j][q] ^ (flip ? (BF3 ? 0x80008000u : 0x80000000u) : 0u);
+                    const uint32_t v = c != OFF_PAD ? dreg[j][q] : 0u;
+                    db[j * DCP + m] = v ^ (flip ? (BF3 ? 0x80008000u : 0x80000000u) : 0u);
                 }
             }
         }
@@ -520,16 +575,16 @@ void m2l_offset_kernel(FmmParams F, OffsetParams O, uint32_t goff, int lvl) {
     if (collect(0, ccur) > 0) {
         stage_load(ccur);
         stage_write(ccur);
-        wave_lds_sync();
+        off_lds_sync();
         for (;;) {
             const int jn = collect(cur ^ 1, cnxt);
             if (jn) stage_load(cnxt);
-            wave_lds_sync();
+            off_lds_sync();
             process(cur);
             if (!jn) break;
-            wave_lds_sync();  // every lane is done with the current tensors
+            off_lds_sync();  // every lane is done with the current tensors
             stage_write(cnxt);
-            wave_lds_sync();
+            off_lds_sync();
             cur ^= 1;
 #pragma unroll
             for (int j = 0; j < OFF_J; ++j) ccur[j] = cnxt[j];
@@ -581,7 +636,7 @@ void m2l_offset_kernel(FmmParams F, OffsetParams O, uint32_t goff, int lvl) {
 template <int P, bool BF3>
 static void launch_off_level_t(const FmmParams& F, const OffsetParams& O, uint32_t goff, uint32_t gcount, int lvl,
                                hipStream_t st) {
-    const size_t lds = OffLayout<P, BF3>::total(O.W);
+    const size_t lds = OffLayout<P, BF3>::total(O.W, O.ncand);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&m2l_offset_kernel<P, BF3>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL((m2l_offset_kernel<P, BF3>), dim3(gcount), dim3(OFF_WAVES * WAVE), lds, st, F, O, goff, lvl);
@@ -663,9 +718,9 @@ void off_tables(int p, bool bf3, std::vector<uint16_t>& hank, std::vector<uint8_
     NBH_OFF_DISPATCH(p, bf3, (off_tables_t<PP, BB>(hank, par)));
 }
 
-size_t off_lds_bytes(int p, bool bf3, int W) {
+size_t off_lds_bytes(int p, bool bf3, int W, int ncand) {
     size_t b = 0;
-    NBH_OFF_DISPATCH(p, bf3, (b = OffLayout<PP, BB>::total(W)));
+    NBH_OFF_DISPATCH(p, bf3, (b = OffLayout<PP, BB>::total(W, ncand)));
     return b;
 }
 size_t off_moment_bytes(int p, bool bf3, uint32_t nn) {

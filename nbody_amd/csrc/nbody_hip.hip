@@ -125,6 +125,9 @@ struct nbh_handle {
     // The side stream runs the moments and tables while the walk runs, then
     // the near field while M2L and L2L run; these order the two streams.
     hipStream_t stream2 = nullptr;
+    // the side streams as created (nbh_set_timing mode 2 points stream2..4 at
+    // the main stream, which serializes the evaluation)
+    hipStream_t own2 = nullptr, own3 = nullptr, own4 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_side = nullptr, ev_walk = nullptr, ev_near = nullptr;
     // A third stream runs each level's same-level M2L as soon as the walk
     // depth that emits it is done (ev_depth), while the walk goes deeper.
@@ -876,6 +879,9 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
     CCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
     CCHK(hipStreamCreateWithFlags(&h->stream3, hipStreamNonBlocking));
     CCHK(hipStreamCreateWithFlags(&h->stream4, hipStreamNonBlocking));
+    h->own2 = h->stream2;
+    h->own3 = h->stream3;
+    h->own4 = h->stream4;
     CCHK(hipEventCreateWithFlags(&h->ev_same4, hipEventDisableTiming));
     CCHK(hipEventCreate(&h->evc0));
     CCHK(hipEventCreateWithFlags(&h->ev_same, hipEventDisableTiming));
@@ -916,13 +922,12 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
         const int K = 2 * (int)std::floor(std::sqrt(3.0) * den / num) + 1;
         int dev_lds = 0;
         CCHK(hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
-        if (K <= 15 && h->nsys == 1 && off_supported((int)cfg->order) &&
-            off_lds_bytes((int)cfg->order, cfg->m2l == NBH_M2L_BF16X3, 4 + 2 * K) <= (size_t)dev_lds) {
+        std::vector<uint32_t> cand;
+        if (K <= 15 && h->nsys == 1 && off_supported((int)cfg->order)) {
             auto accepts = [&](int x, int y, int z) {
                 return (long long)num * num * (x * x + y * y + z * z) > 3ll * den * den;
             };
             auto fdiv2 = [](int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); };
-            std::vector<uint32_t> cand;
             for (int ox = -K; ox <= K; ++ox)
                 for (int oy = -K; oy <= K; ++oy)
                     for (int oz = -K; oz <= K; ++oz) {
@@ -937,6 +942,11 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
                             cand.push_back((uint32_t)(ox + 16) | (uint32_t)(oy + 16) << 5 | (uint32_t)(oz + 16) << 10 |
                                            mask << 16);
                     }
+            if (off_lds_bytes((int)cfg->order, cfg->m2l == NBH_M2L_BF16X3, 4 + 2 * K, (int)cand.size()) >
+                (size_t)dev_lds)
+                cand.clear();  // the window does not fit: the same-level pairs stay on the vector units
+        }
+        if (!cand.empty()) {
             CCHK(h->cand.ensure(cand.size() * 4));
             CCHK(hipMemcpy(h->cand.p, cand.data(), cand.size() * 4, hipMemcpyHostToDevice));
             std::vector<uint16_t> hk;
@@ -1029,9 +1039,9 @@ void nbh_destroy(nbh_handle* h) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : h->ev_lvl)
         if (e) (void)hipEventDestroy(e);
-    if (h->stream4) (void)hipStreamDestroy(h->stream4);
-    if (h->stream3) (void)hipStreamDestroy(h->stream3);
-    if (h->stream2) (void)hipStreamDestroy(h->stream2);
+    if (h->own4) (void)hipStreamDestroy(h->own4);
+    if (h->own3) (void)hipStreamDestroy(h->own3);
+    if (h->own2) (void)hipStreamDestroy(h->own2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -1254,7 +1264,14 @@ int nbh_get_stats(nbh_handle* h, nbh_stats* s) {
 
 int nbh_set_timing(nbh_handle* h, int enable) {
     if (!h) return NBH_E_INVALID;
+    if (enable < 0 || enable > 2) return h->fail(NBH_E_INVALID, "timing mode must be 0, 1 or 2");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
     h->timing = enable != 0;
+    const bool serial = enable == 2;
+    h->stream2 = serial ? h->stream : h->own2;
+    h->stream3 = serial ? h->stream : h->own3;
+    h->stream4 = serial ? h->stream : h->own4;
     return NBH_OK;
 }
 

@@ -204,8 +204,10 @@ class HipSimulation(Simulation):
         _check(self._h, lib().nbh_get_stats(self._h, ctypes.byref(s)), "nbh_get_stats")
         return s.as_dict()
 
-    def set_timing(self, on=True):
-        _check(self._h, lib().nbh_set_timing(self._h, 1 if on else 0), "nbh_set_timing")
+    def set_timing(self, on=True, serial=False):
+        """Per-phase timing; serial=True also runs each evaluation on one stream."""
+        mode = (2 if serial else 1) if on else 0
+        _check(self._h, lib().nbh_set_timing(self._h, mode), "nbh_set_timing")
 
     def force_vjp(self, w):
         w = _c64(w, (self.n, 3))
