@@ -33,12 +33,14 @@ HBM_PEAK_GBS = 8000.0
 P2P_FLOP = 20              # per directed particle-particle interaction
 
 
-def m2l_flop(p, q):
-    """2 x #{kept (k, n): |k|, |n| <= p and |k| + |n| <= p, or |k| + |n| <= q
-    with min(|k|, |n|) <= q - p} per directed cell pair (420 for p = q = 4,
-    600 for p = 4, q = 5)."""
+def m2l_flop(p, q, terms="all"):
+    """2 x #{kept (k, n): |k|, |n| <= p and |k| + |n| <= q ("all"), or
+    |k| + |n| <= p, or |k| + |n| <= q with min(|k|, |n|) <= q - p ("low")}
+    per directed cell pair (420 for p = q = 4; at p = 4, q = 5: 840 with
+    every term, 600 with the low-degree set)."""
     tri = lambda d: (d + 1) * (d + 2) // 2
-    keep = lambda a, b: a + b <= p or (a + b <= q and min(a, b) <= q - p)
+    r = q if terms == "all" else q - p
+    keep = lambda a, b: a + b <= p or (a + b <= q and min(a, b) <= r)
     return 2 * sum(tri(a) * tri(b) for a in range(p + 1) for b in range(p + 1) if keep(a, b))
 
 

@@ -45,8 +45,8 @@ namespace nbody {
 
 struct FmmOptions {
 	int order = 4;           // Taylor order p, 1..8
-	int m2l_order = 0;       // M2L truncation q, p <= q <= 2p (0: p + 1): terms of degree <= p,
-	                         // and up to degree q those with min(|k|, |n|) <= q - p
+	int m2l_order = 0;       // M2L truncation q, p <= q <= 2p (0: p + 1): terms of degree <= q
+	bool m2l_low_terms = false;  // beyond degree p only those with min(|k|, |n|) <= q - p
 	double theta = 0.5;      // opening ratio; must equal num/den with num, den <= 64
 	int leaf_capacity = 32;  // particles per leaf cell
 };
@@ -102,7 +102,8 @@ struct Tables {
 	std::vector<int> grad_s[3];             // index of k + e_axis (or -1)
 	std::vector<double> fact;               // x! y! z!
 
-	Tables(int p_, int q_) : p(p_), q(q_), nc(ncoef(p_)), nq(ncoef(q_)) {
+	// low: beyond degree p only the terms with min(|k|, |n|) <= q - p
+	Tables(int p_, int q_, bool low) : p(p_), q(q_), nc(ncoef(p_)), nq(ncoef(q_)) {
 		for (int d = 0; d <= q; ++d)
 			for (int x = d; x >= 0; --x)
 				for (int y = d - x; y >= 0; --y) {
@@ -140,7 +141,8 @@ struct Tables {
 		for (int k = 0; k < nc; ++k) {
 			m2l_kb.push_back((int)m2l_k.size());
 			for (int n = 0; n < nc; ++n)
-				if (deg[k] + deg[n] <= p || (deg[k] + deg[n] <= q && std::min(deg[k], deg[n]) <= q - p)) {
+				if (deg[k] + deg[n] <= p ||
+				    (deg[k] + deg[n] <= q && (!low || std::min(deg[k], deg[n]) <= q - p))) {
 					m2l_k.push_back(k);
 					m2l_n.push_back(n);
 					m2l_s.push_back(mindex(mx[k] + mx[n], my[k] + my[n], mz[k] + mz[n]));
@@ -316,7 +318,7 @@ public:
 		if (!(opt.theta > 0) || !fmm_detail::rational(opt.theta, &_num, &_den))
 			throw std::runtime_error("CpuFmm: theta must be > 0 and equal num/den with num, den <= 64");
 		if (opt.leaf_capacity < 1) throw std::runtime_error("CpuFmm: leaf_capacity must be >= 1");
-		_tab.reset(new fmm_detail::Tables(opt.order, _q));
+		_tab.reset(new fmm_detail::Tables(opt.order, _q, opt.m2l_low_terms));
 	}
 
 	// grad (3n, AoS) and phi (n) of Phi(x_i) = sum_{j != i} q_j / sqrt(|x_j - x_i|^2 + eps^2):

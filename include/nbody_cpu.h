@@ -31,10 +31,11 @@ int ncpu_forces(ncpu_handle* h, double* acc, double* pot);
 /* Forces from the FP64 fast multipole method instead of the direct sum (the
  * FMM of OpenClSimulation, open_cl_simulation.h:194-198, with the reference's
  * traversal and opening criterion interaction.cl:42-92): Taylor order 1..8,
- * M2L truncation (0: order + 1), opening ratio theta (NODE_APPROX_RATIO,
- * interaction.cl:3-5; num/den with num, den <= 64), leaf capacity.  order 0
- * switches back to the direct sum. */
-int ncpu_set_fmm(ncpu_handle* h, int order, int m2l_order, double theta, int leaf_capacity);
+ * M2L truncation q (0: order + 1) and term set (0: every term of degree <= q;
+ * 1: beyond degree p only those with min(|k|, |n|) <= q - p), opening ratio
+ * theta (NODE_APPROX_RATIO, interaction.cl:3-5; num/den with num, den <= 64),
+ * leaf capacity.  order 0 switches back to the direct sum. */
+int ncpu_set_fmm(ncpu_handle* h, int order, int m2l_order, int m2l_terms, double theta, int leaf_capacity);
 /* statistics of the last FMM evaluation: nodes, leaves, depth, unordered M2L
  * pairs, unordered P2P pairs, visited cell pairs */
 int ncpu_fmm_stats(ncpu_handle* h, long long* out6);

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define NBH_ABI_VERSION 2
+#define NBH_ABI_VERSION 3
 
 /* status codes */
 #define NBH_OK 0
@@ -48,6 +48,12 @@ extern "C" {
 #define NBH_M2L_BF16X3 2       /* grouped; same-level pairs by translation offset on MFMA with the dense
                                   operator (every |k|, |n| <= p), bf16x3 split products, FP32 accumulate */
 #define NBH_M2L_MFMA_F32 3     /* the same with FP32 products */
+
+/* M2L term set at truncation q = m2l_order (k, n of order <= p) */
+#define NBH_M2L_TERMS_ALL 0    /* every term with |k| + |n| <= q */
+#define NBH_M2L_TERMS_LOW 1    /* |k| + |n| <= p, and up to q only those with min(|k|, |n|) <= q - p:
+                                  at q = p + 1 the degree-(p + 1) terms of the dipole (|n| = 1) and
+                                  their mirror images (|k| = 1); 300 instead of 420 terms at p = 4 */
 
 /* near-field (P2P) kernel */
 #define NBH_P2P_LEAF 0         /* one wavefront per target leaf (plain) */
@@ -83,9 +89,10 @@ typedef struct nbh_config {
     double root_lo[3];
     double root_size;
     uint32_t p2p;           /* NBH_P2P_* */
-    uint32_t m2l_order;     /* M2L truncation q (k, n of order <= p): p keeps |k| + |n| <= p;
-                               p + 1 adds the degree-(p + 1) terms with |n| = 1 or |k| = 1;
-                               0 = p + 1 */
+    uint32_t m2l_order;     /* M2L truncation q, p or p + 1 (0 = p + 1): the terms (k, n) with
+                               |k|, |n| <= p and |k| + |n| <= q, as m2l_terms selects */
+    uint32_t m2l_terms;     /* NBH_M2L_TERMS_*: which terms of degree p + 1 < |k| + |n| <= q
+                               the M2L keeps (no effect at q = p) */
 } nbh_config;
 
 typedef struct nbh_handle nbh_handle;

@@ -22,6 +22,7 @@ NOALLOC_MAX_TARGETS = 65536
 
 SOLVER_FMM, SOLVER_DIRECT = 0, 1
 M2L_PLAIN, M2L_GROUPED, M2L_BF16X3, M2L_MFMA_F32 = 0, 1, 2, 3
+M2L_TERMS_ALL, M2L_TERMS_LOW = 0, 1
 P2P_LEAF, P2P_LANE = 0, 1
 INT_KDK, INT_EULER, INT_KICK_DRIFT = 0, 1, 2
 LAW_SOFTENED, LAW_COMPAT_NAIVE = 0, 1
@@ -47,6 +48,7 @@ class Config(ctypes.Structure):
         ("root_size", ctypes.c_double),
         ("p2p", ctypes.c_uint32),
         ("m2l_order", ctypes.c_uint32),
+        ("m2l_terms", ctypes.c_uint32),
     ]
 
 

@@ -33,7 +33,8 @@ def lib():
         L.ncpu_get_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.ncpu_set_state.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double]
         L.ncpu_forces.argtypes = [ctypes.c_void_p, _dp, _dp]
-        L.ncpu_set_fmm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.ncpu_set_fmm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_int]
         L.ncpu_fmm_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
         L.ncpu_fmm_times.argtypes = [ctypes.c_void_p, _dp]
         L.ncpu_destroy.argtypes = [ctypes.c_void_p]
@@ -50,11 +51,13 @@ class CpuSimulation(Simulation):
     """Same constructor prefix as HipSimulation / OpenClSimulation; options:
     G, eps (softening), integrator (INT_*), threads (0 = OpenMP default),
     method ("direct" or "fmm") and, for the FMM, order, m2l_order (0: order +
-    1), theta and leaf_capacity."""
+    1), m2l_terms ("all": every term of degree <= m2l_order; "low": beyond
+    degree order only those with min(|k|, |n|) <= m2l_order - order), theta
+    and leaf_capacity."""
 
     def __init__(self, bounds=None, particles: Optional[Iterable[Particle]] = None, time_step: float = 1e-3,
                  log=None, *, G=1.0, eps=0.01, integrator=INT_KDK, threads=0, arrays=None, method="direct",
-                 order=4, m2l_order=0, theta=0.5, leaf_capacity=32):
+                 order=4, m2l_order=0, m2l_terms="all", theta=0.5, leaf_capacity=32):
         if arrays is not None:
             pos, vel, mass, charge = arrays
         else:
@@ -77,8 +80,11 @@ class CpuSimulation(Simulation):
             raise RuntimeError(f"ncpu_create failed: {lib().ncpu_last_error().decode()}")
         self._h = h
         if method == "fmm":
-            self._check(lib().ncpu_set_fmm(self._h, int(order), int(m2l_order), float(theta), int(leaf_capacity)),
-                        "ncpu_set_fmm")
+            if m2l_terms not in ("all", "low"):
+                self.close()
+                raise ValueError(f"m2l_terms must be 'all' or 'low', not {m2l_terms!r}")
+            self._check(lib().ncpu_set_fmm(self._h, int(order), int(m2l_order), int(m2l_terms == "low"),
+                                           float(theta), int(leaf_capacity)), "ncpu_set_fmm")
         elif method != "direct":
             self.close()
             raise ValueError(f"method must be 'direct' or 'fmm', not {method!r}")

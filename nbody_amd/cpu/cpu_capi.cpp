@@ -76,7 +76,7 @@ int ncpu_forces(ncpu_handle* h, double* acc, double* pot) {
 	return 0;
 }
 
-int ncpu_set_fmm(ncpu_handle* h, int order, int m2l_order, double theta, int leaf_capacity) {
+int ncpu_set_fmm(ncpu_handle* h, int order, int m2l_order, int m2l_terms, double theta, int leaf_capacity) {
 	if (!h) return fail("invalid handle");
 	try {
 		if (order == 0) {
@@ -85,6 +85,8 @@ int ncpu_set_fmm(ncpu_handle* h, int order, int m2l_order, double theta, int lea
 			nbody::FmmOptions o;
 			o.order = order;
 			o.m2l_order = m2l_order;
+			if (m2l_terms != 0 && m2l_terms != 1) return fail("ncpu_set_fmm: m2l_terms must be 0 (all) or 1 (low)");
+			o.m2l_low_terms = m2l_terms == 1;
 			o.theta = theta;
 			o.leaf_capacity = leaf_capacity;
 			h->engine.set_method(1, o);

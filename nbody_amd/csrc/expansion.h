@@ -9,9 +9,11 @@
 //   P2M  M_n   = sum_j q_j b_j^n / n!                      (b = x_j - z_B)
 //   M2M  M'_n  = sum_{m<=n} M_m d^(n-m)/(n-m)!             (d = z_child - z_parent)
 //   M2L  L_k  += sum_{(k,n) kept} (-1)^|n| D_{k+n}(R) M_n   (R = z_A - z_B)
-//        kept: |k|, |n| <= P and either |k|+|n| <= P or (|k|+|n| <= Q and
-//        min(|k|, |n|) <= Q-P) (Q = P: the total-degree truncation; Q = P+1:
-//        plus the degree-(P+1) terms of the dipole and of the field gradient)
+//        kept: |k|, |n| <= P and |k|+|n| <= q (every term of total degree
+//        <= q), or, with the low-degree term set, either |k|+|n| <= P or
+//        (|k|+|n| <= q and min(|k|, |n|) <= q-P) (q = P: the total-degree
+//        truncation; the low set at q = P+1: plus the degree-(P+1) terms of
+//        the dipole and of the field gradient only)
 //   L2L  L'_j += sum_m L_{j+m} d^m / m!                    (d = z_child - z_parent)
 //   L2P  phi   = sum_k L_k a^k/k!,  g_e = sum_k L_{k+e} a^k/k!   (a = x - z_A)
 //
@@ -127,19 +129,24 @@ struct Tab {
     static constexpr Tables<P> t{};
 };
 
-// M2L truncation: the term (k, n) of orders dk = |k|, dn = |n| (<= p) is kept
-// when dk + dn <= p, or when dk + dn <= q and min(dk, dn) <= q - p.  q = p is
-// the usual total-degree truncation; q = p + 1 adds the degree-(p + 1) terms
-// with |n| = 1 (the dipole into the order-p local coefficients; about the
-// geometric cell centre the dipole is not zero) and their mirror images
-// |k| = 1, which keep the term set symmetric in (k, n) so that the same
-// operator serves the adjoint.  At p = 4 these are 90 of the 210 terms of
-// degree 5 and carry most of their accuracy.
-constexpr bool m2l_keeps(int p, int q, int dk, int dn) {
-    return dk + dn <= p || (dk + dn <= q && (dk < dn ? dk : dn) <= q - p);
+// M2L truncation code qc = q | M2L_ALL or q: the term (k, n) of orders
+// dk = |k|, dn = |n| (<= p) is kept when dk + dn <= q (qc = q | M2L_ALL:
+// every term of total degree <= q), or, with the low-degree set (qc = q),
+// when dk + dn <= p, or dk + dn <= q and min(dk, dn) <= q - p.  q = p is the
+// usual total-degree truncation either way.  The low set at q = p + 1 adds
+// only the degree-(p + 1) terms with |n| = 1 (the dipole into the order-p
+// local coefficients; about the geometric cell centre the dipole is not zero)
+// and their mirror images |k| = 1: 90 of the 210 terms of degree 5 at p = 4.
+// Both sets are symmetric in (k, n), so the same operator serves the adjoint.
+constexpr int M2L_ALL = 16;
+constexpr int m2l_deg(int qc) { return qc & (M2L_ALL - 1); }
+constexpr bool m2l_keeps(int p, int qc, int dk, int dn) {
+    const int q = m2l_deg(qc);
+    const int r = (qc & M2L_ALL) ? q : q - p;
+    return dk + dn <= p || (dk + dn <= q && (dk < dn ? dk : dn) <= r);
 }
 // M2L pair table: (k, n, k+n) for the kept terms, with k+n indexed in the
-// order-Q coefficient order (orders <= P are a prefix of it); ordered by
+// order-q coefficient order (orders <= P are a prefix of it); ordered by
 // source coefficient n.
 constexpr int m2l_npairs(int p, int q) {
     int c = 0;
@@ -154,7 +161,7 @@ struct M2LTables {
     int pk[NP], pn[NP], pkn[NP];
     constexpr M2LTables() : pk(), pn(), pkn() {
         constexpr auto& TP = Tab<P>::t;
-        constexpr auto& TQ = Tab<Q>::t;
+        constexpr auto& TQ = Tab<m2l_deg(Q)>::t;
         int t = 0;
         for (int b = 0; b < ncoef(P); ++b)
             for (int a = 0; a < ncoef(P); ++a) {

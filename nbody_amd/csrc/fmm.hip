@@ -148,8 +148,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
             const int B = F.lists.m_pool[s0 + e];
             const int4 gb = F.nodes.geo[B];
             const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
-            float D[ncoef(Q)];
-            derivs<Q>(d.x, d.y, d.z, F.eps2, D);
+            float D[ncoef(m2l_deg(Q))];
+            derivs<m2l_deg(Q)>(d.x, d.y, d.z, F.eps2, D);
             // source moments read straight from memory; the pair table is ordered
             // by source coefficient so each M_n is loaded once and used at once
             m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 // 1024 threads (16 waves at <= 128 VGPRs, one workgroup per CU beside a
 // 112 KiB table) while expansion and tensor fit that register budget (up to
 // p = 4, q = 5); 256 above, with the table read from global memory
-constexpr int m2l_table_threads(int p, int q) { return ncoef(p) + ncoef(q) <= 91 ? 1024 : 256; }
+constexpr int m2l_table_threads(int p, int qc) { return ncoef(p) + ncoef(m2l_deg(qc)) <= 91 ? 1024 : 256; }
 
 // Same-level part of the rows of the level's targets [off, off+count): one
 // wavefront per target, lanes over its entries; writes L.  IN_LDS: the
@@ -515,7 +515,7 @@ constexpr int m2l_table_threads(int p, int q) { return ncoef(p) + ncoef(q) <= 91
 template <int P, int Q, bool IN_LDS>
 __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count,
                                                                   int lvl) {
-    constexpr int NC = ncoef(P), NCQ = ncoef(Q);
+    constexpr int NC = ncoef(P), NCQ = ncoef(m2l_deg(Q));
     constexpr int NQD = (NCQ + 3) / 4;  // float4 per tensor row (the host sets dcp = 4 NQD)
     // LDS row stride in float4: odd, so that the rows that the lanes of a
     // wavefront read at once start in different bank groups (an even stride,
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
                     if (4 * q + 2 < NCQ) D[4 * q + 2] = v.z;
                     if (4 * q + 3 < NCQ) D[4 * q + 3] = v.w;
                 }
-                octant_signs<Q>(neg, D);
+                octant_signs<m2l_deg(Q)>(neg, D);
                 m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
             }
         }
@@ -620,8 +620,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
                 const int B = F.lists.m_pool[e];
                 const int4 gb = F.nodes.geo[B];
                 const float3 d = centre_diff(gb, ga, F.bits, unit);  // R = c_A - c_B
-                float D[ncoef(Q)];
-                derivs<Q>(d.x, d.y, d.z, F.eps2, D);
+                float D[ncoef(m2l_deg(Q))];
+                derivs<m2l_deg(Q)>(d.x, d.y, d.z, F.eps2, D);
                 m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
             }
         }
@@ -666,7 +666,7 @@ __device__ __forceinline__ T half_sum(T v) {
 template <int P, int Q>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 : 1))) void m2l_cross_kernel(
     FmmParams F, uint32_t num_nodes) {
-    constexpr int NC = ncoef(P), NCQ = ncoef(Q), NQD = (NCQ + 3) / 4;
+    constexpr int NC = ncoef(P), NCQ = ncoef(m2l_deg(Q)), NQD = (NCQ + 3) / 4;
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const uint32_t nw = gridDim.x * blockDim.x / WAVE;
     const int lane = lane_id();
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 
                         continue;
                     }
                     float D[NCQ];
-                    load_tensor<Q>(X4 + (size_t)t * NQD, neg, D);
+                    load_tensor<m2l_deg(Q)>(X4 + (size_t)t * NQD, neg, D);
                     m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
                 }
             }
@@ -785,8 +785,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
             const int4 ga = F.nodes.geo[A];
             const float unit = F.unit[F.nodes.sys[A]];
             const float3 d = centre_diff(F.nodes.geo[B], ga, F.bits, unit);  // R = c_A - c_B
-            float D[ncoef(Q)];
-            derivs<Q>(d.x, d.y, d.z, F.eps2, D);
+            float D[ncoef(m2l_deg(Q))];
+            derivs<m2l_deg(Q)>(d.x, d.y, d.z, F.eps2, D);
             m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
         }
 #pragma unroll
@@ -965,17 +965,22 @@ void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, 
 }
 void launch_m2l_parts(int p, int q, const FmmParams& F, const M2LPlan& plan, int parts, int level, hipStream_t st) {
     if (!plan.num_nodes) return;
-    const bool ext = q == p + 1;
+    // q: truncation code (m2l_deg(q) = p or p + 1, M2L_ALL for every term of
+    // degree p + 1; below 2p only, where the two term sets differ)
+    const int ext = m2l_deg(q) == p + 1 ? ((q & M2L_ALL) && p + 1 < 2 * p ? 2 : 1) : 0;
 #define NBH_M2L(PP, QQ) launch_m2l_pq<PP, QQ>(F, plan, parts, level, st)
+#define NBH_M2L3(PP) \
+    (ext == 2 ? NBH_M2L(PP, (PP + 1) | M2L_ALL) : ext == 1 ? NBH_M2L(PP, PP + 1) : NBH_M2L(PP, PP))
     switch (p) {
         case 1: ext ? NBH_M2L(1, 2) : NBH_M2L(1, 1); break;
-        case 2: ext ? NBH_M2L(2, 3) : NBH_M2L(2, 2); break;
-        case 3: ext ? NBH_M2L(3, 4) : NBH_M2L(3, 3); break;
-        case 4: ext ? NBH_M2L(4, 5) : NBH_M2L(4, 4); break;
-        case 5: ext ? NBH_M2L(5, 6) : NBH_M2L(5, 5); break;
-        case 6: ext ? NBH_M2L(6, 7) : NBH_M2L(6, 6); break;
+        case 2: NBH_M2L3(2); break;
+        case 3: NBH_M2L3(3); break;
+        case 4: NBH_M2L3(4); break;
+        case 5: NBH_M2L3(5); break;
+        case 6: NBH_M2L3(6); break;
         default: break;
     }
+#undef NBH_M2L3
 #undef NBH_M2L
 }
 void launch_m2l(int p, int q, const FmmParams& F, const M2LPlan& plan, hipStream_t st) {

@@ -225,6 +225,7 @@ int validate_config(const nbh_config* c, std::string& why) {
             why = "m2l_order must be 0 (= order + 1), order or order + 1";
             return NBH_E_INVALID;
         }
+        if (c->m2l_terms > NBH_M2L_TERMS_LOW) { why = "unknown m2l_terms"; return NBH_E_INVALID; }
         if (c->max_depth > 21) { why = "max_depth must be <= 21"; return NBH_E_INVALID; }
         if (c->m2l > NBH_M2L_MFMA_F32) { why = "unknown m2l kernel"; return NBH_E_INVALID; }
         if ((c->m2l == NBH_M2L_BF16X3 || c->m2l == NBH_M2L_MFMA_F32) && c->order < 2) {
@@ -519,9 +520,12 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
     return NBH_OK;
 }
 
-// M2L truncation |k| + |n| <= q
+// M2L truncation code (expansion.h): q = m2l_order (0: p + 1), with M2L_ALL
+// for every term of degree <= q where that differs from the low-degree set
 static int m2l_q(const nbh_handle* h) {
-    return h->cfg.m2l_order ? (int)h->cfg.m2l_order : (int)h->cfg.order + 1;
+    const int p = (int)h->cfg.order;
+    const int q = h->cfg.m2l_order ? (int)h->cfg.m2l_order : p + 1;
+    return (h->cfg.m2l_terms == NBH_M2L_TERMS_ALL && p < q && q < 2 * p) ? (q | M2L_ALL) : q;
 }
 
 // the same-level pairs go to the matrix cores (one system; the window and
@@ -581,7 +585,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         // system only, since the tensors depend on the root size
         const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
         const int K = 2 * Kp + 1;
-        const int dsz = (K + 1) * (K + 1) * (K + 1), dcp = ncp_of(q);
+        const int dsz = (K + 1) * (K + 1) * (K + 1), dcp = ncp_of(m2l_deg(q));
         const int nlev = (int)h->depth - 1;
         HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4));
         T = F;
@@ -591,7 +595,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         T.dnlev = nlev;
         T.dcp = dcp;
         T.dsz = dsz;
-        launch_dtable(q, T, T.dtab, 2, nlev, K + 1, 0, 1, 0, h->stream2);
+        launch_dtable(m2l_deg(q), T, T.dtab, 2, nlev, K + 1, 0, 1, 0, h->stream2);
         // pairs one level apart, by finer level 1..depth: offsets |d_i| <=
         // 4 Kp + 1 half finer cells (the coarser side's parent pair was
         // split), tabled at d_i = 1, 3, .., 4 Kp + 1
@@ -602,7 +606,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         T.xE = E;
         T.xlevel0 = 1;
         T.xnlev = xn;
-        launch_dtable(q, T, T.xtab, 1, xn, E, 1, 2, 1, h->stream2);
+        launch_dtable(m2l_deg(q), T, T.xtab, 1, xn, E, 1, 2, 1, h->stream2);
         if (use_offset_m2l(h)) {
             // the matrix-core same-level M2L: signed split moments and the
             // dense tensors (orders <= 2p) of the offsets [0, K]^3, levels 2..depth
@@ -784,6 +788,7 @@ int nbh_config_default(nbh_config* c) {
     c->force_law = NBH_LAW_SOFTENED;
     c->device = 0;
     c->p2p = NBH_P2P_LANE;
+    c->m2l_terms = NBH_M2L_TERMS_ALL;  // q = p + 1 with every degree-(p + 1) term
     return NBH_OK;
 }
 

@@ -90,9 +90,11 @@ class HipSimulation(Simulation):
         mass = _c64(mass)
         charge = None if charge is None else _c64(charge)
         self.config = cfg
-        # M2L truncation q = m2l_order (0 selects order + 1): terms of degree <= p,
-        # and up to degree q those with min(|k|, |n|) <= q - p
+        # M2L truncation q = m2l_order (0 selects order + 1) and term set:
+        # "all" keeps every term of degree <= q, "low" beyond degree p only
+        # those with min(|k|, |n|) <= q - p (the oracle's m2l_terms)
         self.m2l_order = int(cfg.m2l_order) or int(cfg.order) + 1
+        self.m2l_terms = "low" if cfg.m2l_terms == _lib.M2L_TERMS_LOW else "all"
         h = ctypes.c_void_p()
         L = lib()
         if sys_off is None:

@@ -470,7 +470,10 @@ void orc_m2m(int p, const double* zc, const double* zp, const double* Mc, double
     }
 }
 
-void orc_m2l(int p, int q, const double* zA, const double* zB, double eps, const double* M, double* L) {
+void orc_m2l(int p, int qc, const double* zA, const double* zB, double eps, const double* M, double* L) {
+    // qc = q | 16: every term of total degree <= q; qc = q: the low-degree set
+    const int q = qc & 15;
+    const bool all = (qc & 16) != 0;
     MultiIndex T(p), TQ(q);
     double R[3] = {zA[0] - zB[0], zA[1] - zB[1], zA[2] - zB[2]};
     std::vector<double> D;
@@ -481,8 +484,9 @@ void orc_m2l(int p, int q, const double* zA, const double* zB, double eps, const
         for (int n = 0; n < T.size(); ++n) {
             const auto& nn = T.mi[n];
             const int dk = kk[0] + kk[1] + kk[2], dn = nn[0] + nn[1] + nn[2];
-            // beyond total degree p only the terms with min(|k|, |n|) <= q - p
-            if (dk + dn > p && std::min(dk, dn) > q - p) continue;
+            // low-degree set: beyond total degree p only the terms with
+            // min(|k|, |n|) <= q - p (both sets stop at total degree q)
+            if (!all && dk + dn > p && std::min(dk, dn) > q - p) continue;
             int s = TQ.at(kk[0] + nn[0], kk[1] + nn[1], kk[2] + nn[2]);
             if (s < 0) continue;
             double sign = (dn & 1) ? -1.0 : 1.0;

@@ -91,11 +91,13 @@ void orc_derivs(int p, const double* R, double eps, double* D);
 void orc_p2m(int p, const double* z, size_t n, const double* x, const double* q, double* M);
 /* shift multipole from child centre zc to parent centre zp */
 void orc_m2m(int p, const double* zc, const double* zp, const double* Mc, double* Mp);
-/* L_k += sum (-1)^|n| D_{k+n}(zA - zB) M_n over |k|,|n| <= p with |k|+|n| <= p,
- * or |k|+|n| <= q and min(|k|,|n|) <= q-p (q >= p).  q == p is the usual
- * total-degree truncation; q == p+1 adds the degree-(p+1) terms with |n| = 1 or
- * |k| = 1 (a term set symmetric in (k, n)). */
-void orc_m2l(int p, int q, const double* zA, const double* zB, double eps, const double* M, double* L);
+/* L_k += sum (-1)^|n| D_{k+n}(zA - zB) M_n over |k|,|n| <= p.  qc = q | 16
+ * keeps every term with |k|+|n| <= q; qc = q (the low-degree set) keeps
+ * |k|+|n| <= p, or |k|+|n| <= q and min(|k|,|n|) <= q-p (q >= p).  q == p is
+ * the usual total-degree truncation either way; the low set at q == p+1 adds
+ * only the degree-(p+1) terms with |n| = 1 or |k| = 1.  Both sets are
+ * symmetric in (k, n). */
+void orc_m2l(int p, int qc, const double* zA, const double* zB, double eps, const double* M, double* L);
 /* shift local expansion from parent zp to child zc (accumulates into Lc) */
 void orc_l2l(int p, const double* zp, const double* zc, const double* Lp, double* Lc);
 /* evaluate: phi_L(x) = sum_k L_k a^k/k!, g = sum_k L_{k+e} a^k/k! (a = x - z) */
@@ -104,8 +106,8 @@ void orc_l2p(int p, const double* z, const double* L, const double* x, double* p
 /* ---- full FMM force evaluation (FP64), same tree/lists as the product ---- */
 /* x[3n] in sorted order, q[n]; returns acc (without the q_i/m_i factor) and
  * potential per particle, in the same order.  box[4*nsys] from orc_morton.
- * qm2l = q | q_same << 8: M2L truncation (orc_m2l's q) of pairs of different
- * levels, and of pairs of one level (q_same; 0 = the same q). */
+ * qm2l = qc | qc_same << 8: M2L truncation (orc_m2l's qc) of pairs of different
+ * levels, and of pairs of one level (qc_same; 0 = the same qc). */
 void orc_fmm(const orc_tree*, const orc_lists*, int p, int qm2l, size_t n, const double* x,
              const double* q, const double* box, double G, double eps,
              double* acc, double* pot);

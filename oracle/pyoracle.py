@@ -212,12 +212,21 @@ def m2m(p, zc, zp, Mc):
     return Mp
 
 
-def m2l(p, zA, zB, M, eps=0.0, m2l_order=None):
-    """m2l_order q (default p): keep |k| + |n| <= p, and up to |k| + |n| <= q
-    the terms with min(|k|, |n|) <= q - p."""
+def _m2l_code(q, m2l_terms):
+    """orc_m2l's truncation code: q | 16 for every term of degree <= q
+    ("all"), q for the low-degree set ("low")."""
+    if m2l_terms not in ("all", "low"):
+        raise ValueError(f"m2l_terms must be 'all' or 'low', not {m2l_terms!r}")
+    return q | (16 if m2l_terms == "all" else 0)
+
+
+def m2l(p, zA, zB, M, eps=0.0, m2l_order=None, m2l_terms="all"):
+    """m2l_order q (default p): keep every term with |k| + |n| <= q
+    (m2l_terms "all"), or |k| + |n| <= p and up to |k| + |n| <= q only the
+    terms with min(|k|, |n|) <= q - p ("low")."""
     zA = c64(zA); zB = c64(zB); M = c64(M)
     L = np.zeros(ncoef(p))
-    q = p if m2l_order is None else int(m2l_order)
+    q = _m2l_code(p if m2l_order is None else int(m2l_order), m2l_terms)
     lib().orc_m2l(ctypes.c_int(p), ctypes.c_int(q), _ptr(zA, _dp), _ptr(zB, _dp), ctypes.c_double(eps), _ptr(M, _dp), _ptr(L, _dp))
     return L
 
@@ -237,28 +246,30 @@ def l2p(p, z, L, x):
     return phi.value, g
 
 
-def _qcode(p, m2l_order, same_level_order):
-    q = p if m2l_order is None else int(m2l_order)
-    return q | ((int(same_level_order) if same_level_order else 0) << 8)
+def _qcode(p, m2l_order, same_level_order, m2l_terms="all"):
+    q = _m2l_code(p if m2l_order is None else int(m2l_order), m2l_terms)
+    qs = _m2l_code(int(same_level_order), m2l_terms) if same_level_order else 0
+    return q | (qs << 8)
 
 
 def fmm(tree: Tree, lists: Lists, p, x_sorted, q_sorted, box, G=1.0, eps=0.0, m2l_order=None,
-        same_level_order=None):
-    """same_level_order: M2L truncation of the pairs of one level (None: m2l_order);
-    2p keeps every term |k|, |n| <= p (the dense operator)."""
+        same_level_order=None, m2l_terms="all"):
+    """m2l_order, m2l_terms: the M2L truncation (m2l); same_level_order: that
+    of the pairs of one level (None: m2l_order); 2p keeps every term |k|,
+    |n| <= p (the dense operator)."""
     x = c64(x_sorted).reshape(-1, 3)
     q = c64(q_sorted)
     box = c64(box).reshape(-1)
     acc = np.zeros_like(x)
     pot = np.zeros(len(x))
     lib().orc_fmm(ctypes.c_void_p(tree.handle), ctypes.c_void_p(lists.handle), p,
-                  _qcode(p, m2l_order, same_level_order), len(x),
+                  _qcode(p, m2l_order, same_level_order, m2l_terms), len(x),
                   _ptr(x, _dp), _ptr(q, _dp), _ptr(box, _dp), G, eps, _ptr(acc, _dp), _ptr(pot, _dp))
     return acc, pot
 
 
 def fmm_vjp(tree: Tree, lists: Lists, p, x_sorted, m_sorted, w_sorted, box, G=1.0, eps=0.0, m2l_order=None,
-            same_level_order=None):
+            same_level_order=None, m2l_terms="all"):
     x = c64(x_sorted).reshape(-1, 3)
     m = c64(m_sorted)
     w = c64(w_sorted).reshape(-1, 3)
@@ -269,7 +280,7 @@ def fmm_vjp(tree: Tree, lists: Lists, p, x_sorted, m_sorted, w_sorted, box, G=1.
     L.orc_fmm_vjp.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
                               _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
     L.orc_fmm_vjp(ctypes.c_void_p(tree.handle), ctypes.c_void_p(lists.handle), p,
-                  _qcode(p, m2l_order, same_level_order), len(x), _ptr(x, _dp),
+                  _qcode(p, m2l_order, same_level_order, m2l_terms), len(x), _ptr(x, _dp),
                   _ptr(m, _dp), _ptr(w, _dp), _ptr(box, _dp), G, eps, _ptr(gx, _dp), _ptr(gm, _dp))
     return gx, gm
 

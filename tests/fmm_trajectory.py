@@ -20,11 +20,12 @@ import pyoracle as po
 
 class OracleFMMTrajectory:
     def __init__(self, m, p=4, theta=(1, 2), cap=16, eps=1e-2, dt=1e-3, G=1.0, m2l_order=None,
-                 same_level_order=None):
+                 same_level_order=None, m2l_terms="all"):
         self.m = np.asarray(m, dtype=np.float64)
         self.p, self.theta, self.cap, self.eps, self.dt, self.G = p, theta, cap, eps, dt, G
         self.q = m2l_order if m2l_order is not None else p + 1
         self.qs = same_level_order
+        self.terms = m2l_terms
 
     def structures(self, x):
         keys, box = po.morton(x)
@@ -37,7 +38,7 @@ class OracleFMMTrajectory:
         order, tree, lists, box = st
         mm = self.m if m is None else m
         a_s, _ = po.fmm(tree, lists, self.p, x[order], mm[order], box, G=self.G, eps=self.eps,
-                        m2l_order=self.q, same_level_order=self.qs)
+                        m2l_order=self.q, same_level_order=self.qs, m2l_terms=self.terms)
         a = np.empty_like(a_s)
         a[order] = a_s
         return a
@@ -45,7 +46,7 @@ class OracleFMMTrajectory:
     def vjp(self, x, w, st):
         order, tree, lists, box = st
         gx_s, gm_s = po.fmm_vjp(tree, lists, self.p, x[order], self.m[order], w[order], box, G=self.G,
-                                eps=self.eps, m2l_order=self.q, same_level_order=self.qs)
+                                eps=self.eps, m2l_order=self.q, same_level_order=self.qs, m2l_terms=self.terms)
         gx = np.empty_like(gx_s)
         gm = np.empty_like(gm_s)
         gx[order] = gx_s

@@ -38,7 +38,7 @@ def test_force_vjp_matches_oracle_transpose(kind, n, cap, p, m2l):
     keys, ids, box = sim.debug_keys()
     tree = po.build_tree(keys, cap=cap)
     lists = po.walk(tree, 1, 2)
-    gxo, gmo = po.fmm_vjp(tree, lists, p, x[ids], m[ids], w[ids], box, eps=eps, m2l_order=sim.m2l_order,
+    gxo, gmo = po.fmm_vjp(tree, lists, p, x[ids], m[ids], w[ids], box, eps=eps, m2l_order=sim.m2l_order, m2l_terms=sim.m2l_terms,
                           same_level_order=sim.same_level_order)
     inv = np.empty(n, dtype=np.int64)
     inv[ids] = np.arange(n)

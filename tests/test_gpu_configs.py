@@ -40,7 +40,7 @@ def test_ensemble_matches_oracle_and_direct():
     # no pair crosses systems
     tsys = np.repeat(tree.sys, np.diff(lists.m2l_off))
     assert np.array_equal(tsys, tree.sys[lists.m2l_src])
-    acc_o, _ = po.fmm(tree, lists, 4, x[ids], m[ids], box_o, eps=1e-2, m2l_order=sim.m2l_order,
+    acc_o, _ = po.fmm(tree, lists, 4, x[ids], m[ids], box_o, eps=1e-2, m2l_order=sim.m2l_order, m2l_terms=sim.m2l_terms,
                       same_level_order=sim.same_level_order)
     assert rms_rel(acc[ids], acc_o) < 2e-5
     # per-system error vs the FP64 direct sum on up to 256 targets each
@@ -82,7 +82,7 @@ def test_clustered_pair_theta04_p6(kern, m2l):
     lists = po.walk(tree, 2, 5)
     mo, ms, pofs, ps = sim.debug_lists()
     assert np.array_equal(ms, lists.m2l_src) and np.array_equal(ps, lists.p2p_src)
-    acc_o, _ = po.fmm(tree, lists, 6, x[ids], m[ids], box, eps=5e-4, m2l_order=sim.m2l_order,
+    acc_o, _ = po.fmm(tree, lists, 6, x[ids], m[ids], box, eps=5e-4, m2l_order=sim.m2l_order, m2l_terms=sim.m2l_terms,
                       same_level_order=sim.same_level_order)
     e_o = rms_rel(acc[ids], acc_o)
     m_o = max_rel_floor(acc[ids], acc_o)

@@ -44,7 +44,8 @@ def _ics(kind, n, seed=0):
     return nb.ics.plummer(n, seed + 1)
 
 
-# name, M2L kernel, P2P kernel, M2L truncation (0: q = p+1, the default; -1: q = p),
+# name, M2L kernel, P2P kernel, M2L truncation (0: q = p+1 with every degree-(p+1)
+# term, the default; 1: q = p+1 with the low-degree term set; -1: q = p),
 # RMS tolerances (acc, pot) against the FP64 oracle FMM.  The MFMA kernels take
 # the same-level pairs by translation offset with the dense operator (every
 # term |k|, |n| <= p; the oracle is run with that truncation for those pairs)
@@ -53,6 +54,8 @@ def _ics(kind, n, seed=0):
 # field's share of each force keeps below them.
 KERNELS = [("tuned", nb.M2L_GROUPED, nb.P2P_LANE, 0, 2e-5, 2e-6),
            ("tuned_qp", nb.M2L_GROUPED, nb.P2P_LANE, -1, 2e-5, 2e-6),
+           ("tuned_low", nb.M2L_GROUPED, nb.P2P_LANE, 1, 2e-5, 2e-6),
+           ("plain_low", nb.M2L_PLAIN, nb.P2P_LEAF, 1, 2e-5, 2e-6),
            ("plain", nb.M2L_PLAIN, nb.P2P_LEAF, 0, 2e-5, 2e-6),
            ("mfma_f32", nb.M2L_MFMA_F32, nb.P2P_LANE, 0, 2e-5, 2e-6),
            ("mfma_bf16x3", nb.M2L_BF16X3, nb.P2P_LANE, 0, 2e-5, 2e-6),
@@ -65,7 +68,7 @@ def test_tree_lists_forces_match_oracle(name, kind, n, cap, p, th, eps, kern, m2
     _case(f"{name}/{kern}")
     x, v, m = _ics(kind, n)
     sim = make_sim(x, v, m, leaf_capacity=cap, order=p, theta=th[0] / th[1], eps=eps, m2l=m2l, p2p=p2p,
-                   m2l_order=p if qsel < 0 else 0)
+                   m2l_order=p if qsel < 0 else 0, m2l_terms=nb.M2L_TERMS_LOW if qsel == 1 else nb.M2L_TERMS_ALL)
     acc, pot = sim.forces()
     keys, ids, box, xs, qs = _gpu_sorted(sim, x, m)
     # keys, box and the stable particle order
@@ -87,7 +90,7 @@ def test_tree_lists_forces_match_oracle(name, kind, n, cap, p, th, eps, kern, m2
     assert np.array_equal(pofs, lists.p2p_off)
     assert np.array_equal(ps, lists.p2p_src)
     # forces: FP32 kernels vs FP64 oracle FMM on the same tree and lists
-    acc_o, pot_o = po.fmm(tree, lists, p, xs, qs, box_o, G=1.0, eps=eps, m2l_order=sim.m2l_order,
+    acc_o, pot_o = po.fmm(tree, lists, p, xs, qs, box_o, G=1.0, eps=eps, m2l_order=sim.m2l_order, m2l_terms=sim.m2l_terms,
                           same_level_order=sim.same_level_order)
     inv = np.empty_like(ids)
     inv[ids] = np.arange(n, dtype=np.uint32)

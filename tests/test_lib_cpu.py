@@ -40,7 +40,7 @@ def test_exported_symbols_via_nm():
 
 def test_abi_and_config_layout():
     L = nb.lib()
-    assert L.nbh_abi_version() == 2
+    assert L.nbh_abi_version() == 3
     assert L.nbh_config_size() == ctypes.sizeof(nb.Config)
 
 

@@ -397,13 +397,14 @@ def test_euler_and_kick_drift_one_step_closed_form():
 
 
 def test_m2l_extended_truncation_is_the_higher_order_operator_restricted():
-    """orc_m2l(p, q=p+1) keeps the terms of total degree <= p and those of
-    degree p+1 with |n| = 1 or |k| = 1.  So, with the order-(p+1) M2L applied to
-    M padded with zeros (read at |k| <= p): local orders |k| <= 1 receive every
-    moment and equal it; on a monopole + dipole every local order equals it; and
-    on moments of order >= 2 the orders |k| >= 2 equal the total-degree
-    truncation q = p.  (The order-(p+1) operator is pinned by the sympy
-    derivative test and the convergence tests above.)"""
+    """orc_m2l(p, q=p+1) with every term keeps |k| + |n| <= p+1, which is the
+    order-(p+1) M2L applied to M padded with zeros and read at |k| <= p.  The
+    low-degree set keeps the terms of total degree <= p and those of degree
+    p+1 with |n| = 1 or |k| = 1; against the same operator: local orders
+    |k| <= 1 receive every moment and equal it; on a monopole + dipole every
+    local order equals it; and on moments of order >= 2 the orders |k| >= 2
+    equal the total-degree truncation q = p.  (The order-(p+1) operator is
+    pinned by the sympy derivative test and the convergence tests above.)"""
     rng = np.random.default_rng(21)
     zA, zB = np.array([0.3, -0.2, 0.1]), np.array([2.1, 0.7, -1.4])
     for p in [1, 2, 3, 4, 5]:
@@ -418,18 +419,23 @@ def test_m2l_extended_truncation_is_the_higher_order_operator_restricted():
             pad[:nc] = Mx
             return po.m2l(p + 1, zA, zB, pad, eps=0.03)[:nc]
 
-        ext = po.m2l(p, zA, zB, M, eps=0.03, m2l_order=p + 1)
         ref = full(M)
         tol = 1e-13 * np.abs(ref).max()
+        every = po.m2l(p, zA, zB, M, eps=0.03, m2l_order=p + 1, m2l_terms="all")
+        assert np.allclose(every, ref, rtol=1e-13, atol=tol)
+        ext = po.m2l(p, zA, zB, M, eps=0.03, m2l_order=p + 1, m2l_terms="low")
         assert np.allclose(ext[:4], ref[:4], rtol=1e-13, atol=tol)
-        ext_low = po.m2l(p, zA, zB, low, eps=0.03, m2l_order=p + 1)
+        ext_low = po.m2l(p, zA, zB, low, eps=0.03, m2l_order=p + 1, m2l_terms="low")
         assert np.allclose(ext_low, full(low), rtol=1e-13, atol=tol)
-        ext_high = po.m2l(p, zA, zB, high, eps=0.03, m2l_order=p + 1)
+        ext_high = po.m2l(p, zA, zB, high, eps=0.03, m2l_order=p + 1, m2l_terms="low")
         tot_high = po.m2l(p, zA, zB, high, eps=0.03)
         assert np.allclose(ext_high[4:], tot_high[4:], rtol=1e-13, atol=tol)
         assert np.allclose(ext, ext_low + ext_high, rtol=1e-12, atol=tol)
-        # and it differs from the total-degree truncation (p >= 1)
+        # and it differs from the total-degree truncation (p >= 1), and from
+        # the full degree-(p+1) set where that has more terms (p >= 3)
         assert not np.allclose(ext, po.m2l(p, zA, zB, M, eps=0.03))
+        if p >= 3:
+            assert not np.allclose(ext, every)
 
 
 def test_fmm_extended_m2l_is_more_accurate():
@@ -440,20 +446,20 @@ def test_fmm_extended_m2l_is_more_accurate():
     L = po.walk(t, 1, 2)
     ad, _ = po.direct(x[o], q, eps=1e-2)
 
-    def err(p, qq):
-        a, _ = po.fmm(t, L, p, x[o], q, box, eps=1e-2, m2l_order=qq)
+    def err(p, qq, terms="all"):
+        a, _ = po.fmm(t, L, p, x[o], q, box, eps=1e-2, m2l_order=qq, m2l_terms=terms)
         return np.sqrt(np.mean(np.sum((a - ad) ** 2, 1)) / np.mean(np.sum(ad ** 2, 1)))
 
     for p in [2, 3, 4]:
-        e0, e1, e2 = err(p, p), err(p, p + 1), err(p + 1, p + 1)
-        assert e2 < e1 < e0, (p, e0, e1, e2)
+        e0, e_low, e_all, e2 = err(p, p), err(p, p + 1, "low"), err(p, p + 1), err(p + 1, p + 1)
+        assert e2 < e_all < e0 and e_all <= e_low < e0, (p, e0, e_low, e_all, e2)
 
 
-@pytest.mark.parametrize("ext", [0, 1])
-def test_fmm_vjp_is_the_exact_transpose_of_the_fmm(ext):
+@pytest.mark.parametrize("ext,terms", [(0, "all"), (1, "all"), (1, "low")])
+def test_fmm_vjp_is_the_exact_transpose_of_the_fmm(ext, terms):
     """orc_fmm_vjp against FP64 central differences of orc_fmm with the tree
     and lists held fixed, and against the direct-sum adjoint (to the FMM's
-    truncation accuracy); M2L truncated at |k|+|n| <= p + ext."""
+    truncation accuracy); M2L truncated at q = p + ext with either term set."""
     rng = np.random.default_rng(13)
     n = 600
     x = np.concatenate([rng.random((500, 3)), 0.3 + 0.05 * rng.random((100, 3))])
@@ -463,10 +469,10 @@ def test_fmm_vjp_is_the_exact_transpose_of_the_fmm(ext):
     xs, ms = x[o], m[o]
     w = rng.standard_normal((n, 3))
     p, eps = 4, 1e-2
-    gx, gm = po.fmm_vjp(t, L, p, xs, ms, w, box, eps=eps, m2l_order=p + ext)
+    gx, gm = po.fmm_vjp(t, L, p, xs, ms, w, box, eps=eps, m2l_order=p + ext, m2l_terms=terms)
 
     def f(xx, mm):
-        a, _ = po.fmm(t, L, p, xx, mm, box, eps=eps, m2l_order=p + ext)
+        a, _ = po.fmm(t, L, p, xx, mm, box, eps=eps, m2l_order=p + ext, m2l_terms=terms)
         return float((a * w).sum())
 
     h = 1e-6
@@ -487,9 +493,10 @@ def test_fmm_vjp_is_the_exact_transpose_of_the_fmm(ext):
 
 def test_m2l_term_set_is_symmetric_with_the_counted_size():
     """The kept (k, n) terms of orc_m2l, probed column by column with unit
-    moments: 210 at p = q = 4 and 300 at p = 4, q = 5 (the count bench.py turns
-    into FLOP per pair), and the pattern is symmetric in (k, n), which is what
-    lets the adjoint reuse the same operator."""
+    moments: 210 at p = q = 4, 420 at p = 4, q = 5 with every term and 300 with
+    the low-degree set (the counts bench.py turns into FLOP per pair), and the
+    pattern is symmetric in (k, n), which is what lets the adjoint reuse the
+    same operator."""
     import importlib.util
     import os
     spec = importlib.util.spec_from_file_location(
@@ -499,13 +506,13 @@ def test_m2l_term_set_is_symmetric_with_the_counted_size():
     zA, zB = np.array([0.3, -0.2, 0.1]), np.array([2.1, 0.7, -1.4])
     p = 4
     nc = po.ncoef(p)
-    for q, count in [(4, 210), (5, 300)]:
+    for q, terms, count in [(4, "all", 210), (5, "all", 420), (5, "low", 300)]:
         T = np.zeros((nc, nc))
         for n in range(nc):
             e = np.zeros(nc)
             e[n] = 1.0
-            T[:, n] = po.m2l(p, zA, zB, e, eps=0.03, m2l_order=q)
+            T[:, n] = po.m2l(p, zA, zB, e, eps=0.03, m2l_order=q, m2l_terms=terms)
         nz = np.abs(T) > 0
         assert int(nz.sum()) == count
         assert np.array_equal(nz, nz.T)
-        assert bench.m2l_flop(p, q) == 2 * count
+        assert bench.m2l_flop(p, q, terms) == 2 * count
