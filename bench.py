@@ -3,19 +3,22 @@
 Workload (one "step" = one KDK leapfrog step = tree rebuild + moments + walk +
 M2L/L2L + P2P/L2P + integration): N = 1,048,576 Plummer (seed 1, a = 1, M = 1,
 G = 1, truncated at r = 20a), eps = 1e-3, theta = 0.5, leaf capacity 32,
-Cartesian order p = 4 (M2L keeping |k| + |n| <= 5), FP32 force kernels, FP64
-positions and velocities, dt = 1e-3.  One force evaluation is checked against
-the FP64 direct sum on 4096 random targets; then W warm-up steps and K timed
-steps.  The plain FP32 port (same operator, untuned kernels) is timed beside it
-for 3 steps.
+Cartesian order p = 4 (M2L keeping every term |k| + |n| <= 5), FP32 force
+kernels, FP64 positions and velocities, dt = 1e-3.  One force evaluation is
+checked against the FP64 direct sum on 4096 random targets; then W warm-up
+steps and K timed steps.  The plain FP32 port (same operator, untuned kernels)
+is timed beside it for 3 steps.
 
-Prints one JSON line.  The roofline entry is for the P2P kernel, timed with
-HIP events on the stream it runs on inside the timed region; its `traffic`
-comes from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.  The library
-runs three streams: the moments and the same-level M2L run beside the walk, P2P
-beside M2L and L2L, so `phases_ms` overlap (`overlapped` names what each ran
-beside) and only `total` is wall time; kernels that share the GPU take longer
-than alone.
+Prints one JSON line.  The library runs three streams: the moments and the
+same-level M2L run beside the walk, P2P beside M2L and L2L, so `phases_ms`
+overlap (`overlapped` names what each ran beside) and only `total` is wall
+time.  The roofline entry is for the dominant kernel, the same-level M2L
+(most of the step's FLOP), with the P2P kernel as `second`: both timed with
+the library's HIP events on the stream each runs on, in a roofline pass of
+--roofline-steps steps after the timed ones with every kernel of an
+evaluation on one stream (kernels that share the GPU take longer than
+alone; `in_timed_steps` gives their overlapped figures).  `traffic` comes
+from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
 """
 import argparse
 import json
@@ -62,19 +65,44 @@ def measured_traffic(kernel, workload):
     return None
 
 
+def host_cpu():
+    """Host CPU model (/proc/cpuinfo) and the number of cores this process
+    may run on (what `nproc` prints)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return model, nproc
+
+
 def cpu_baseline(sample_n, budget_s):
     """Reference NaiveSimulation (oracle/_ref, compiled from the reference's
-    sources) timed on this host, 1 core; falls back to the oracle's FP32
-    restatement of the same loop when the reference build is absent."""
+    sources with -O2 -ffp-contract=off) timed on this host, 1 core; falls back
+    to the oracle's FP32 restatement of the same loop when the reference
+    build is absent.  Also times one step of the as-shipped -O0 build at
+    N=4096 when it is there."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
-    rng = np.random.default_rng(123)
+    def sample(n, seed=123):
+        rng = np.random.default_rng(seed)
+        P = np.zeros((n, 12), dtype=np.float32)
+        P[:, 0:3] = rng.random((n, 3))
+        P[:, 8] = 1.0 / n
+        P[:, 9] = 1.0 / n
+        return P
+
     n = sample_n
-    P = np.zeros((n, 12), dtype=np.float32)
-    P[:, 0:3] = rng.random((n, 3))
-    P[:, 8] = 1.0 / n
-    P[:, 9] = 1.0 / n
+    P = sample(n)
     kind = "reference" if po.ref_lib() is not None else "port"
     run = po.naive_reference if kind == "reference" else po.naive_compat
     t0 = time.perf_counter()
@@ -88,9 +116,22 @@ def cpu_baseline(sample_n, budget_s):
     pairs_per_s = n * (n - 1) / 2 / dt
     target_n = 1 << 20
     value = 2.0 * pairs_per_s / (target_n - 1)
-    return {"value": value, "unit": "particle-steps/s", "cores": 1, "kind": kind,
-            "sample": f"NaiveSimulation::step at N={n} uniform, {steps} step(s) of {dt:.2f} s "
-                      f"({pairs_per_s:.3g} pairs/s, -O2), extrapolated to N=1,048,576 as 2*pairs_per_s/(N-1)"}
+    model, nproc = host_cpu()
+    out = {"value": value, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+           "sample": f"NaiveSimulation::step at N={n} uniform, {steps} step(s) of {dt:.2f} s "
+                     f"({pairs_per_s:.3g} pairs/s, -O2 -ffp-contract=off), extrapolated to N=1,048,576 "
+                     f"as 2*pairs_per_s/(N-1)",
+           "host_cpu": model, "nproc": nproc}
+    if kind == "reference" and po.ref_lib(as_shipped=True) is not None:
+        n0 = 4096
+        t0 = time.perf_counter()
+        po.naive_reference(sample(n0), 1.0, 1e-3, 1, as_shipped=True)
+        d0 = time.perf_counter() - t0
+        p0 = n0 * (n0 - 1) / 2 / d0
+        out["as_shipped_O0"] = {"value": 2.0 * p0 / (target_n - 1), "unit": "particle-steps/s",
+                                "sample": f"NaiveSimulation::step at N={n0} uniform, 1 step of {d0:.2f} s "
+                                          f"({p0:.3g} pairs/s, built -O0 as shipped), extrapolated the same way"}
+    return out
 
 
 def copy_calibration(nbytes):
@@ -114,14 +155,22 @@ def copy_calibration(nbytes):
     hip.hipFree(b)
 
 
-def time_steps(sim, warmup, steps):
+SUMMED = ["ms_m2l_same", "m2l_same_pairs", "m2l_same_launches", "ms_p2p", "p2p_interactions"]
+
+
+def time_steps(sim, warmup, steps, serial=False, totals=None):
     """warmup untimed steps, then `steps` timed ones (each step ends with a
-    stream synchronisation inside the library)."""
+    stream synchronisation inside the library), with the library's per-phase
+    HIP events on; serial: every kernel of an evaluation on one stream, so
+    that each event pair brackets one kernel running alone.  totals (a dict)
+    receives the sums over the timed steps of the SUMMED statistics."""
     for _ in range(warmup):
         sim.step(1)
-    sim.set_timing(True)
+    sim.set_timing(True, serial=serial)
     agg = {k: 0.0 for k in ["ms_tree", "ms_moments", "ms_walk", "ms_m2l", "ms_l2l", "ms_p2p", "ms_total"]}
     inter = m2l_pairs = split_pairs = p2p_pairs = 0
+    if totals is not None:
+        totals.update({k: 0 for k in SUMMED})
     t0 = time.perf_counter()
     for _ in range(steps):
         sim.step(1)
@@ -132,7 +181,11 @@ def time_steps(sim, warmup, steps):
         m2l_pairs += st["m2l_pairs"]
         split_pairs += st["split_pairs"]
         p2p_pairs += st["p2p_pairs"]
+        if totals is not None:
+            for k in SUMMED:
+                totals[k] += st[k]
     wall = time.perf_counter() - t0
+    sim.set_timing(False)
     return agg, wall, inter, m2l_pairs, split_pairs, p2p_pairs, sim.stats()
 
 
@@ -353,7 +406,17 @@ def main():
     ap.add_argument("--m2l", default="grouped", choices=["plain", "grouped", "bf16x3", "mfma_f32"])
     ap.add_argument("--p2p", default="lane", choices=["leaf", "lane"])
     ap.add_argument("--order", type=int, default=4)
-    ap.add_argument("--m2l-order", type=int, default=0, help="M2L truncation |k|+|n| <= q (0: order + 1)")
+    ap.add_argument("--m2l-order", type=int, default=0,
+                    help="M2L truncation q, order or order + 1 (0: order + 1): terms with |k|, |n| <= order and "
+                         "|k| + |n| <= q, as --m2l-terms selects")
+    ap.add_argument("--m2l-terms", default="all", choices=["all", "low"],
+                    help="all: every term of degree <= q; low: beyond degree order only those with "
+                         "min(|k|, |n|) <= q - order")
+    ap.add_argument("--serial", action="store_true",
+                    help="run every kernel of an evaluation on one stream in the timed steps too")
+    ap.add_argument("--roofline-steps", type=int, default=3,
+                    help="steps of the serialized roofline pass after the timed steps (0: take the roofline "
+                         "from the timed steps)")
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -390,7 +453,8 @@ def main():
     p = args.order
     x, v, m = nb.ics.plummer(n, seed=1, a=1.0, M=1.0, G=1.0, rmax=20.0)
     cfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, dt=1e-3, leaf_capacity=32, order=p, m2l=m2l, p2p=p2p,
-                            m2l_order=args.m2l_order)
+                            m2l_order=args.m2l_order,
+                            m2l_terms={"all": nb.M2L_TERMS_ALL, "low": nb.M2L_TERMS_LOW}[args.m2l_terms])
     sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
 
     # one force evaluation, checked against the FP64 direct sum
@@ -406,19 +470,69 @@ def main():
                "force_err_targets": int(len(tg))}
 
     K = args.steps
-    timed = time_steps(sim, args.warmup, K)
+    tot = {}
+    timed = time_steps(sim, args.warmup, K, serial=args.serial, totals=tot)
+    # the roofline pass: R more steps with every kernel of an evaluation on
+    # one stream, so that each event pair times one kernel running alone (in
+    # the timed steps the same-level M2L launches of successive levels, the
+    # walk and P2P share the GPU and each takes longer than alone)
+    R = args.roofline_steps
+    rl = {}
+    if R > 0:
+        time_steps(sim, 0, R, serial=True, totals=rl)
     q = sim.m2l_order
+    terms = sim.m2l_terms
+    q_same = int(sim.stats()["m2l_same_order"]) or q
     sim.close()
     agg, wall, inter, m2l_pairs, split_pairs, p2p_pairs, st = timed
     value = n * K / wall
-    p2p_ms = agg["ms_p2p"] / K
-    p2p_flop = P2P_FLOP * inter / K
-    achieved = p2p_flop / (p2p_ms * 1e-3) / 1e12
     m2l_ms = agg["ms_m2l"] / K
-    m2l_tf = m2l_flop(p, q) * m2l_pairs / K / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
+    m2l_tf = m2l_flop(p, q, terms) * m2l_pairs / K / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     workload = f"plummer_N{n}_seed1_eps1e-3_theta0.5_cap32_p{p}_kdk_dt1e-3"
-    kernel = ("p2p_staged_kernel" if args.p2p == "lane" else "p2p_kernel") + f"<{p}>"
-    tr = measured_traffic(kernel, workload)
+    p2p_kernel = ("p2p_staged_kernel" if args.p2p == "lane" else "p2p_kernel") + f"<{p}>"
+    qc = (q | 16) if (terms == "all" and p < q < 2 * p) else q
+    mfma = args.m2l in ("bf16x3", "mfma_f32")
+    same_kernel = (f"m2l_offset_kernel<{p}, {'true' if args.m2l == 'bf16x3' else 'false'}>" if mfma
+                   else f"m2l_lds_kernel<{p}, {qc}, true>")
+    # the same-level pairs take the dense operator (every |k|, |n| <= p) on
+    # the matrix cores, the truncation q with the chosen term set on the VALU
+    same_flop = m2l_flop(p, q_same, "all" if mfma else terms)
+
+    def kernel_roofline(src, steps, kernel, flop_per_step, ms_per_step, launches_per_step, bound, tr):
+        ach = flop_per_step / (ms_per_step * 1e-3) / 1e12 if ms_per_step > 0 else 0.0
+        return {"bound": bound, "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP32_PEAK_TFLOPS,
+                "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
+                "traffic_source": tr["source"] if tr else None,
+                "kernel": kernel, "timed_on": src, "steps": steps,
+                "launches_per_step": launches_per_step, "ms_per_step": ms_per_step,
+                "ms_per_launch": ms_per_step / launches_per_step if launches_per_step else None,
+                "flop_per_launch": flop_per_step / launches_per_step if launches_per_step else None,
+                "flop_per_step": flop_per_step}
+
+    # the dominant kernel is the same-level M2L (most of the step's FLOP);
+    # P2P is the second entry
+    src, S, tt = ("roofline pass (serialized, after the timed steps)", R, rl) if R > 0 else \
+                 ("timed steps" + (" (serialized)" if args.serial else " (streams overlapped)"), K, tot)
+    same_launches = tt["m2l_same_launches"] / S
+    roof = kernel_roofline(src, S, same_kernel, same_flop * tt["m2l_same_pairs"] / S, tt["ms_m2l_same"] / S,
+                           same_launches, "mfma" if mfma else "fp32-valu",
+                           measured_traffic(same_kernel, workload))
+    roof.update({"phase": "same-level M2L (cells of one level, offsets tabled per level)",
+                 "flop_per_pair": same_flop, "pairs_per_step": tt["m2l_same_pairs"] / S,
+                 "m2l_same_order": q_same})
+    roof["second"] = kernel_roofline(src, S, p2p_kernel, P2P_FLOP * tt["p2p_interactions"] / S,
+                                     tt["ms_p2p"] / S, 1, "fp32-valu", measured_traffic(p2p_kernel, workload))
+    roof["second"].update({"phase": "P2P near field", "flop_per_pair": P2P_FLOP,
+                           "pairs_per_step": tt["p2p_interactions"] / S})
+    # the same kernels inside the timed steps (streams overlapped unless --serial)
+    roof["in_timed_steps"] = {
+        "same_level_m2l_ms_per_step": tot["ms_m2l_same"] / K,
+        "same_level_m2l_achieved": same_flop * tot["m2l_same_pairs"] / K / (tot["ms_m2l_same"] / K * 1e-3) / 1e12
+        if tot["ms_m2l_same"] > 0 else None,
+        "p2p_ms_per_step": tot["ms_p2p"] / K,
+        "p2p_achieved": P2P_FLOP * tot["p2p_interactions"] / K / (tot["ms_p2p"] / K * 1e-3) / 1e12
+        if tot["ms_p2p"] > 0 else None}
     # algorithmic bytes of the HBM-bound phases (each input read once, each
     # output written once), per step
     nodes, leaves = st["num_nodes"], st["num_leaves"]
@@ -438,8 +552,10 @@ def main():
         ms = agg["ms_" + ph] / K
         gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         phase_roofline[ph] = {"bound": "hbm", "bytes": b, "ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
-    phase_roofline["m2l"] = {"bound": "fp32-valu", "flop_per_pair": m2l_flop(p, q), "ms": m2l_ms,
-                             "achieved_TFLOPs": m2l_tf, "frac": m2l_tf / FP32_PEAK_TFLOPS}
+    phase_roofline["m2l"] = {"bound": "fp32-valu", "flop_per_pair": m2l_flop(p, q, terms), "ms": m2l_ms,
+                             "achieved_TFLOPs": m2l_tf, "frac": m2l_tf / FP32_PEAK_TFLOPS,
+                             "note": "all M2L pairs over the M2L phase's wall time (from the first same-level "
+                                     "launch, beside the walk, to the last other-level kernel)"}
     out = {
         "metric": "particle-steps/s, N=1M Plummer, force rms err<=1e-4 vs FP64 direct; %roofline",
         "value": value,
@@ -454,24 +570,23 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "n": n, "order": p, "m2l_order": q, "theta": 0.5, "leaf_capacity": 32, "eps": 1e-3,
-                   "m2l": args.m2l, "p2p": args.p2p, "state_dtype": "fp64"},
-        "roofline": {"bound": "mfma", "engine": "fp32-valu (P2P kernel)", "achieved": achieved,
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
-                     "traffic_source": tr["source"] if tr else None,
-                     "kernel": kernel, "ms_per_launch": p2p_ms, "flop_per_launch": p2p_flop},
+                   "n": n, "order": p, "m2l_order": q, "m2l_terms": terms, "theta": 0.5, "leaf_capacity": 32,
+                   "eps": 1e-3, "m2l": args.m2l, "p2p": args.p2p, "state_dtype": "fp64",
+                   "streams": "serialized" if args.serial else "overlapped"},
+        "roofline": roof,
         "phases_ms": {k[3:]: agg[k] / K for k in agg},
-        "overlapped": {"moments": "walk", "m2l": "walk+p2p", "p2p": "m2l+l2l"},
+        "overlapped": {} if args.serial else {"moments": "walk", "m2l": "walk+p2p", "p2p": "m2l+l2l"},
         "phase_roofline": phase_roofline,
-        "m2l": {"pairs_per_step": m2l_pairs / K, "tflops": m2l_tf},
+        "m2l": {"pairs_per_step": m2l_pairs / K, "same_level_pairs_per_step": tot["m2l_same_pairs"] / K,
+                "flop_per_pair": m2l_flop(p, q, terms), "tflops": m2l_tf},
         "tree": {"nodes": nodes, "leaves": leaves, "depth": st["depth"]},
     }
     if not args.no_plain_compare and (args.m2l, args.p2p) != ("plain", "leaf"):
         # the plain all-FP32 port: one thread per target cell for M2L (tensor
         # built per pair), one wavefront per leaf for P2P, same operator
         pcfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, dt=1e-3, leaf_capacity=32, order=p,
-                                 m2l=nb.M2L_PLAIN, p2p=nb.P2P_LEAF, m2l_order=args.m2l_order)
+                                 m2l=nb.M2L_PLAIN, p2p=nb.P2P_LEAF, m2l_order=args.m2l_order,
+                                 m2l_terms=cfg.m2l_terms)
         psim = nb.HipSimulation.from_arrays(x, v, m, m, config=pcfg)
         _, pwall, _, _, _, _, _ = time_steps(psim, 1, 3)
         psim.close()

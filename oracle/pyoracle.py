@@ -54,15 +54,24 @@ def lib():
     return _LIB
 
 
-def ref_lib():
-    """The reference's own NaiveSimulation (oracle/_ref), or None if not built."""
-    global _REF
-    if _REF is None:
-        path = os.path.join(_HERE, "_ref", "libnaive_ref.so")
+_REF_O0 = None
+
+
+def ref_lib(as_shipped=False):
+    """The reference's own NaiveSimulation (oracle/_ref), or None if not built:
+    built -O2 -ffp-contract=off, or as_shipped (-O0, the reference's build)."""
+    global _REF, _REF_O0
+    lib_ = _REF_O0 if as_shipped else _REF
+    if lib_ is None:
+        path = os.path.join(_HERE, "_ref", "libnaive_ref_O0.so" if as_shipped else "libnaive_ref.so")
         if not os.path.exists(path):
             return None
-        _REF = ctypes.CDLL(path)
-    return _REF
+        lib_ = ctypes.CDLL(path)
+        if as_shipped:
+            _REF_O0 = lib_
+        else:
+            _REF = lib_
+    return lib_
 
 
 def c64(a):
@@ -313,9 +322,10 @@ def naive_compat(P12, force_constant, dt, nsteps):
     return P, t.value
 
 
-def naive_reference(P12, force_constant, dt, nsteps):
-    """The reference's own NaiveSimulation (compiled from /root/reference)."""
-    R = ref_lib()
+def naive_reference(P12, force_constant, dt, nsteps, as_shipped=False):
+    """The reference's own NaiveSimulation (compiled from /root/reference;
+    as_shipped: the -O0 build)."""
+    R = ref_lib(as_shipped)
     if R is None:
         return None
     P = np.ascontiguousarray(P12, dtype=np.float32).copy()
