@@ -59,7 +59,7 @@ def measured_traffic(kernel, workload):
         return None
     for line in open(path):
         fields = dict(kv.split("=", 1) for kv in line.split() if "=" in kv)
-        if fields.get("kernel") == kernel and fields.get("workload") == workload:
+        if fields.get("kernel") == kernel.replace(" ", "") and fields.get("workload") == workload:
             return {"fetch_bytes": float(fields["fetch_bytes"]), "write_bytes": float(fields["write_bytes"]),
                     "source": "profiles/hbm_traffic_plummer1M_p4.md"}
     return None
