@@ -287,21 +287,19 @@ def workload_clustered(nb, args):
 
 def workload_ensemble(nb, args):
     """256 independent Plummer systems (seeds 100-355), N log-uniform in
-    [512, 65536], eps=1e-2, theta=0.5, p=4, one handle; the leaf capacity is
-    picked by timing 16, 32 and 64 (2 steps each); then 20 timed steps and the
-    per-system force error on 256 targets each."""
+    [512, 65536], eps=1e-2, theta=0.5, p=4, one handle; the library's
+    autotuner (nbh_autotune) picks the leaf capacity from 16, 24, 32, 48, 64
+    and the P2P tile from 64 and 128 staged sources (2 force evaluations
+    each); then 20 timed steps and the per-system force error on 256 targets
+    each."""
     x, v, m, off = nb.ics.ensemble(256, 100, 512, 65536)
     n = len(m)
-    tune = {}
-    for cap in (16, 32, 64):
-        cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=cap, order=4)
-        sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg, sys_off=off)
-        _, wall, *_ = time_steps(sim, 1, 2)
-        sim.close()
-        tune[cap] = wall / 2 * 1e3
-    cap = min(tune, key=tune.get)
-    cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=cap, order=4)
+    cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=32, order=4)
     sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg, sys_off=off)
+    t0 = time.perf_counter()
+    cap, chunk, tuned_ms, table = sim.autotune([16, 24, 32, 48, 64], [64, 128], evals=2)
+    tune_s = time.perf_counter() - t0
+    tune = {f"cap{c}_chunk{k}": ms for (c, k), ms in table.items()}
     acc, _ = sim.forces()
     rng = np.random.default_rng(256)
     errs = []
@@ -323,9 +321,10 @@ def workload_ensemble(nb, args):
             "value": n * K / wall, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": 2,
             "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "fp32", "data": "synthetic",
-            "config": {"workload": f"ensemble_256_plummer_seeds100_355_N512_65536_eps1e-2_theta0.5_p4_cap{cap}",
-                       "n_total": n},
-            "autotune_ms_per_step": tune, "leaf_capacity": cap,
+            "config": {"workload": "ensemble_256_plummer_seeds100_355_N512_65536_eps1e-2_theta0.5_p4",
+                       "n_total": n, "leaf_capacity": cap, "p2p_chunk": chunk},
+            "autotune": {"ms_per_eval": tune, "chosen": {"leaf_capacity": cap, "p2p_chunk": chunk},
+                         "seconds": tune_s},
             "force_err": {"per_system_rms_median": float(np.median(errs)), "per_system_rms_max": float(np.max(errs))},
             "phases_ms": {k[3:]: agg[k] / K for k in agg}}
 
@@ -334,29 +333,28 @@ def workload_adjoint(nb, args):
     """Plummer N=16384 (seed 4), eps=1e-2, theta=0.5, p=4, 200 KDK steps at
     dt=1e-3, checkpoints every 20: forward trajectory vs adjoint (reverse)
     time for the loss mean |x_N - x*|^2, x* evolved from velocities perturbed
-    by sigma=1e-3 noise (seed 5)."""
-    from nbody_amd.adjoint import HipForceProvider, TrajectoryAdjoint
+    by sigma=1e-3 noise (seed 5); the whole reverse pass on the device
+    (nbh_adjoint_forward / nbh_adjoint_backward)."""
+    from nbody_amd.adjoint import DeviceTrajectory
     n, steps, dt = 16384, 200, 1e-3
     x0, v0, m = nb.ics.plummer(n, 4)
     cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=dt, leaf_capacity=16, order=4)
     rng = np.random.default_rng(5)
-    prov = HipForceProvider(x0, v0, m, cfg)
-    adj = TrajectoryAdjoint(prov, dt, steps, ckpt_every=20)
-    xt, _ = adj.forward(x0, v0 + 1e-3 * rng.standard_normal((n, 3)))
-    t0 = time.perf_counter()
-    xN, vN = adj.forward(x0, v0)
-    t_fwd = time.perf_counter() - t0
+    adj = DeviceTrajectory(x0, v0 + 1e-3 * rng.standard_normal((n, 3)), m, steps, ckpt_every=20, config=cfg)
+    xt, _ = adj.forward()           # the target snapshot (also warms the handle)
+    adj.sim.set_state(x0, v0)
+    xN, vN = adj.forward()
+    t_fwd = adj.seconds["forward"]
     loss = float(np.sum((xN - xt) ** 2) / (3 * n))
-    t0 = time.perf_counter()
     gx, gv, gm = adj.backward(2.0 * (xN - xt) / (3 * n), np.zeros_like(vN))
-    t_bwd = time.perf_counter() - t0
-    prov.close()
+    t_bwd = adj.seconds["backward"]
+    adj.close()
     return {"metric": "adjoint cost: reverse (gradient) time / forward trajectory time, N=16384, 200 steps",
-            "value": t_bwd / t_fwd, "unit": "ratio", "n_gpus": 1, "steps": steps, "warmup": 0,
+            "value": t_bwd / t_fwd, "unit": "ratio", "n_gpus": 1, "steps": steps, "warmup": 1,
             "ms_per_step": t_fwd / steps * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
             "dtype": "fp32", "data": "synthetic",
             "config": {"workload": "adjoint_plummer_N16384_seed4_eps1e-2_theta0.5_p4_cap16_200steps_ckpt20"},
-            "forward_s": t_fwd, "backward_s": t_bwd, "backward_parts_s": adj.seconds, "loss": loss,
+            "forward_s": t_fwd, "backward_s": t_bwd, "loss": loss,
             "grad_norms": {"x0": float(np.linalg.norm(gx)), "v0": float(np.linalg.norm(gv)), "m": float(np.linalg.norm(gm))}}
 
 
@@ -412,6 +410,9 @@ def main():
     ap.add_argument("--m2l-terms", default="all", choices=["all", "low"],
                     help="all: every term of degree <= q; low: beyond degree order only those with "
                          "min(|k|, |n|) <= q - order")
+    ap.add_argument("--p2p-chunk", default="auto", choices=["auto", "64", "128"],
+                    help="P2P tile: source particles staged per wavefront round (auto: nbh_autotune picks 64 or "
+                         "128 on the workload before the warm-up)")
     ap.add_argument("--serial", action="store_true",
                     help="run every kernel of an evaluation on one stream in the timed steps too")
     ap.add_argument("--roofline-steps", type=int, default=3,
@@ -454,8 +455,14 @@ def main():
     x, v, m = nb.ics.plummer(n, seed=1, a=1.0, M=1.0, G=1.0, rmax=20.0)
     cfg = nb.default_config(G=1.0, eps=1e-3, theta=0.5, dt=1e-3, leaf_capacity=32, order=p, m2l=m2l, p2p=p2p,
                             m2l_order=args.m2l_order,
-                            m2l_terms={"all": nb.M2L_TERMS_ALL, "low": nb.M2L_TERMS_LOW}[args.m2l_terms])
+                            m2l_terms={"all": nb.M2L_TERMS_ALL, "low": nb.M2L_TERMS_LOW}[args.m2l_terms],
+                            p2p_chunk=0 if args.p2p_chunk == "auto" else int(args.p2p_chunk))
     sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
+    tuned = None
+    if args.p2p_chunk == "auto" and args.p2p == "lane":
+        # the P2P tile shape (the leaf capacity is part of the workload)
+        _, chunk, _, table = sim.autotune(None, [64, 128], evals=2)
+        tuned = {"p2p_chunk": chunk, "ms_per_eval": {f"chunk{k}": ms for (_, k), ms in table.items()}}
 
     # one force evaluation, checked against the FP64 direct sum
     err = {}
@@ -489,7 +496,8 @@ def main():
     m2l_ms = agg["ms_m2l"] / K
     m2l_tf = m2l_flop(p, q, terms) * m2l_pairs / K / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     workload = f"plummer_N{n}_seed1_eps1e-3_theta0.5_cap32_p{p}_kdk_dt1e-3"
-    p2p_kernel = ("p2p_staged_kernel" if args.p2p == "lane" else "p2p_kernel") + f"<{p}>"
+    chunk = int(sim.config.p2p_chunk) or 64
+    p2p_kernel = f"p2p_staged_kernel<{p}, {chunk // 64}>" if args.p2p == "lane" else f"p2p_kernel<{p}>"
     qc = (q | 16) if (terms == "all" and p < q < 2 * p) else q
     mfma = args.m2l in ("bf16x3", "mfma_f32")
     same_kernel = (f"m2l_offset_kernel<{p}, {'true' if args.m2l == 'bf16x3' else 'false'}>" if mfma
@@ -571,9 +579,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload,
                    "n": n, "order": p, "m2l_order": q, "m2l_terms": terms, "theta": 0.5, "leaf_capacity": 32,
-                   "eps": 1e-3, "m2l": args.m2l, "p2p": args.p2p, "state_dtype": "fp64",
+                   "eps": 1e-3, "m2l": args.m2l, "p2p": args.p2p, "p2p_chunk": chunk, "state_dtype": "fp64",
                    "streams": "serialized" if args.serial else "overlapped"},
         "roofline": roof,
+        "autotune": tuned,
         "phases_ms": {k[3:]: agg[k] / K for k in agg},
         "overlapped": {} if args.serial else {"moments": "walk", "m2l": "walk+p2p", "p2p": "m2l+l2l"},
         "phase_roofline": phase_roofline,

@@ -93,6 +93,8 @@ typedef struct nbh_config {
                                |k|, |n| <= p and |k| + |n| <= q, as m2l_terms selects */
     uint32_t m2l_terms;     /* NBH_M2L_TERMS_*: which terms of degree p + 1 < |k| + |n| <= q
                                the M2L keeps (no effect at q = p) */
+    uint32_t p2p_chunk;     /* NBH_P2P_LANE tile shape: source particles staged in LDS per
+                               wavefront round, 64 or 128 (0 = 64); nbh_autotune picks it */
 } nbh_config;
 
 typedef struct nbh_handle nbh_handle;
@@ -157,6 +159,24 @@ int nbh_set_state(nbh_handle* h, const double* pos, const double* vel, double t)
  * acc[3n] = (q_i/m_i) * field (force.cl:4-19 divided by mass), pot[n] =
  * potential per unit charge.  Either may be NULL. */
 int nbh_compute_forces(nbh_handle* h, float* acc, float* pot);
+
+/* Autotuner (no counterpart in the reference, which hard-codes a leaf
+ * capacity of 8 with a FIXME, open_cl_simulation.cpp:41-47): for every leaf
+ * capacity in caps[0..ncaps) and P2P tile chunk (64 or 128) in
+ * chunks[0..nchunks) (NULL or 0: the handle's current value), runs one
+ * untimed and `evals` timed force evaluations at the current state, keeps
+ * the fastest pair on the handle and reports it in *out (may be NULL);
+ * ms_table (may be NULL) receives ncaps x nchunks mean ms per evaluation,
+ * chunk index fastest.  Positions and velocities are unchanged; the
+ * accelerations are those of the chosen configuration.  FMM solver only. */
+typedef struct nbh_tune_result {
+    uint32_t leaf_capacity;
+    uint32_t p2p_chunk;
+    double ms_per_eval;
+    uint32_t tried;
+} nbh_tune_result;
+int nbh_autotune(nbh_handle* h, const uint32_t* caps, size_t ncaps, const uint32_t* chunks, size_t nchunks,
+                 int evals, nbh_tune_result* out, double* ms_table);
 
 /* Reference check: FP64 direct sum on the GPU for the given target particle
  * indices (input order) over all sources of their system; acc[3*nt], pot[nt]

@@ -49,6 +49,7 @@ class Config(ctypes.Structure):
         ("p2p", ctypes.c_uint32),
         ("m2l_order", ctypes.c_uint32),
         ("m2l_terms", ctypes.c_uint32),
+        ("p2p_chunk", ctypes.c_uint32),
     ]
 
 
@@ -78,6 +79,15 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("leaf_capacity", ctypes.c_uint32),
+        ("p2p_chunk", ctypes.c_uint32),
+        ("ms_per_eval", ctypes.c_double),
+        ("tried", ctypes.c_uint32),
+    ]
+
+
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -102,6 +112,7 @@ SIGNATURES = {
     "nbh_get_state": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "nbh_set_state": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_double]),
     "nbh_compute_forces": (ctypes.c_int, [_vp, _fp, _fp]),
+    "nbh_autotune": (ctypes.c_int, [_vp, _u32p, _sz, _u32p, _sz, ctypes.c_int, ctypes.POINTER(TuneResult), _dp]),
     "nbh_direct_fp64": (ctypes.c_int, [_vp, _sz, _i64p, _dp, _dp]),
     "nbh_direct_fp64_noalloc": (ctypes.c_int, [_vp, _sz, _i64p, _dp, _dp]),
     "nbh_diagnostics": (ctypes.c_int, [_vp, _dp]),

@@ -291,19 +291,26 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
 // size.  Tc = ceil(nchunk / splits) is the same for every lane: slots past the
 // chunk's nchunk sources hold zero charges, every lane runs Tc iterations, and
 // a partial chunk costs only the iterations that it fills.
+// Tile shape: a chunk is 64 SPL sources (SPL = 1 or 2 per lane): with SPL = 2
+// each staging round (entry lookup, LDS writes, three wave syncs) serves
+// twice the pair iterations.
 constexpr int P2P_WAVES = 4;
-// stage slots: splits * ceil(nchunk / splits) <= 84 for nt, nchunk in [1, 64]
-constexpr int P2P_STAGE = 96;
+// stage slots: splits * ceil(nchunk / splits) <= 84 for nt in [1, 64] and
+// nchunk in [1, 64], <= 147 for nchunk in [1, 128]
+template <int SPL>
+constexpr int p2p_stage_slots() { return SPL == 1 ? 96 : 160; }
 
 // SOFT: eps > 0 (each target meets itself at r = 0 with a finite softened
 // potential); else the self pair and the empty slots are skipped by index.
-template <bool SOFT>
+template <bool SOFT, int SPL>
 __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State& S, uint32_t num_leaves) {
+    constexpr int P2P_STAGE = p2p_stage_slots<SPL>();
+    constexpr int CHUNK = WAVE * SPL;
     __shared__ float4 stage_s[P2P_WAVES][P2P_STAGE];
     __shared__ int stage_i[P2P_WAVES][P2P_STAGE];
     __shared__ int2 ent_s[P2P_WAVES][WAVE];      // list entry: first particle, flattened start
     __shared__ float4 entd_s[P2P_WAVES][WAVE];   // list entry: source centre - target centre
-    __shared__ unsigned long long start_s[P2P_WAVES];  // entries starting in the chunk
+    __shared__ unsigned long long start_s[P2P_WAVES][SPL];  // entries starting in each 64 of the chunk
     // (left as a vector value: made uniform with readfirstlane, the leaf
     // records come through scalar loads, whose waits share the counter of
     // the LDS reads, and the kernel is slower)
@@ -313,10 +320,11 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
     int* sidx = stage_i[wslot];
     int2* ent = ent_s[wslot];
     float4* entd = entd_s[wslot];
-    // slots past 64 are never staged: zero charges for good
-    if (lane < P2P_STAGE - WAVE) {
-        stage[WAVE + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-        sidx[WAVE + lane] = -1;
+    // slots past the chunk are never staged: zero charges for good
+    static_assert(P2P_STAGE - CHUNK <= WAVE, "one slot per lane");
+    if (lane < P2P_STAGE - CHUNK) {
+        stage[CHUNK + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        sidx[CHUNK + lane] = -1;
     }
     const uint32_t nw = gridDim.x * P2P_WAVES;
     unsigned long long inter = 0;  // particle pairs, summed over the lanes at the end
@@ -363,20 +371,26 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
                     ent[lane] = make_int2(beg, excl);
                     entd[lane] = make_float4(dx, dy, dz, 0.f);
                 }
-                for (int f0 = 0; f0 < total; f0 += WAVE) {
-                    const int fidx = f0 + lane;
+                for (int f0 = 0; f0 < total; f0 += CHUNK) {
                     // entry of flattened source fidx: the last entry with excl <=
                     // fidx.  Every entry has at least one particle, so the starts
-                    // are distinct: the entries before the chunk, plus one bit
-                    // per start inside it (set with one LDS atomic)
-                    const int kb0 = __popcll(__ballot(lane < ne && excl < f0));
-                    if (lane == 0) start_s[wslot] = 0ull;
+                    // are distinct: the entries before each 64 of the chunk, plus
+                    // one bit per start inside it (set with one LDS atomic)
+                    if (lane < SPL) start_s[wslot][lane] = 0ull;
                     wave_lds_sync();
-                    if (lane < ne && excl >= f0 && excl < f0 + WAVE)
-                        atomicOr(&start_s[wslot], 1ull << (excl - f0));
+#pragma unroll
+                    for (int hh = 0; hh < SPL; ++hh) {
+                        const int g0 = f0 + hh * WAVE;
+                        if (lane < ne && excl >= g0 && excl < g0 + WAVE)
+                            atomicOr(&start_s[wslot][hh], 1ull << (excl - g0));
+                    }
                     wave_lds_sync();
-                    const unsigned long long starts = start_s[wslot];
-                    {
+#pragma unroll
+                    for (int hh = 0; hh < SPL; ++hh) {
+                        const int g0 = f0 + hh * WAVE;
+                        const int fidx = g0 + lane;
+                        const int kb0 = __popcll(__ballot(lane < ne && excl < g0));
+                        const unsigned long long starts = start_s[wslot][hh];
                         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);  // past the sources: no charge
                         int src = -1;
                         if (fidx < total) {
@@ -387,10 +401,10 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
                             const float4 s = F.pf[src];
                             v = make_float4(d.x + s.x, d.y + s.y, d.z + s.z, s.w);
                         }
-                        stage[lane] = v;
-                        sidx[lane] = src;
+                        stage[hh * WAVE + lane] = v;
+                        sidx[hh * WAVE + lane] = src;
                     }
-                    const int nchunk = min(WAVE, total - f0);
+                    const int nchunk = min(CHUNK, total - f0);
                     // uniform: counted on the scalar unit
                     const int Tc = __builtin_amdgcn_readfirstlane(
                         splits == 1 ? nchunk : (int)__umulhi((uint32_t)(nchunk + splits - 1), magic));
@@ -462,15 +476,15 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
     if (lane == 0 && inter) atomicAdd(F.p2p_inter, inter);
 }
 
-template <int P>
+template <int P, int SPL>
 __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves) {
-    p2p_staged_body<true>(F, S, num_leaves);
+    p2p_staged_body<true, SPL>(F, S, num_leaves);
 }
 
-template <int P>
+template <int P, int SPL>
 __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_unsoftened_kernel(FmmParams F, State S,
                                                                                  uint32_t num_leaves) {
-    p2p_staged_body<false>(F, S, num_leaves);
+    p2p_staged_body<false, SPL>(F, S, num_leaves);
 }
 
 // ---------------------------------------------------------------------------
@@ -906,14 +920,15 @@ static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves
     hipLaunchKernelGGL((p2p_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves);
 }
 
-template <int P>
+template <int P, int SPL>
 static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     const unsigned grid = std::min<unsigned>(ceil_div(num_leaves, P2P_WAVES), 65536u);
     if (F.eps2 > 0.f)
-        hipLaunchKernelGGL((p2p_staged_kernel<P>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S, num_leaves);
-    else
-        hipLaunchKernelGGL((p2p_staged_unsoftened_kernel<P>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S,
+        hipLaunchKernelGGL((p2p_staged_kernel<P, SPL>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S,
                            num_leaves);
+    else
+        hipLaunchKernelGGL((p2p_staged_unsoftened_kernel<P, SPL>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F,
+                           S, num_leaves);
 }
 #define NBH_DISPATCH_P(p, call)          \
     switch (p) {                         \
@@ -944,9 +959,10 @@ void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint
     if (!count) return;
     NBH_DISPATCH_P(p, launch_l2l_t<PP>(F, count, off, st));
 }
-void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
+void launch_p2p_staged(int p, int chunk, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
     if (!num_leaves) return;
-    NBH_DISPATCH_P(p, launch_p2p_staged_t<PP>(F, S, num_leaves, st));
+    if (chunk == 128) NBH_DISPATCH_P(p, (launch_p2p_staged_t<PP, 2>(F, S, num_leaves, st)))
+    else NBH_DISPATCH_P(p, (launch_p2p_staged_t<PP, 1>(F, S, num_leaves, st)))
 }
 void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
                    hipStream_t st) {

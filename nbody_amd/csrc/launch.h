@@ -143,7 +143,8 @@ void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint
 // az, pot); launch_l2p adds the far field of the leaf's local expansion,
 // applies G q/m and the sign of the potential, and the kick v += a kick_dt
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
-void launch_p2p_staged(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
+// chunk: sources staged per wavefront round, 64 or 128 (the tile shape)
+void launch_p2p_staged(int p, int chunk, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
 void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st);
 // M2L of order (p, q): q = p keeps |k| + |n| <= p, q = p + 1 also the terms
 // of degree p + 1 with |n| = 1 or |k| = 1 (the expansions stay of order p).

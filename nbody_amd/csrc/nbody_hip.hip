@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -226,6 +227,10 @@ int validate_config(const nbh_config* c, std::string& why) {
             return NBH_E_INVALID;
         }
         if (c->m2l_terms > NBH_M2L_TERMS_LOW) { why = "unknown m2l_terms"; return NBH_E_INVALID; }
+        if (c->p2p_chunk != 0 && c->p2p_chunk != 64 && c->p2p_chunk != 128) {
+            why = "p2p_chunk must be 0 (= 64), 64 or 128";
+            return NBH_E_INVALID;
+        }
         if (c->max_depth > 21) { why = "max_depth must be <= 21"; return NBH_E_INVALID; }
         if (c->m2l > NBH_M2L_MFMA_F32) { why = "unknown m2l kernel"; return NBH_E_INVALID; }
         if ((c->m2l == NBH_M2L_BF16X3 || c->m2l == NBH_M2L_MFMA_F32) && c->order < 2) {
@@ -528,6 +533,9 @@ static int m2l_q(const nbh_handle* h) {
     return (h->cfg.m2l_terms == NBH_M2L_TERMS_ALL && p < q && q < 2 * p) ? (q | M2L_ALL) : q;
 }
 
+// P2P tile: sources staged per wavefront round
+static int p2p_chunk(const nbh_handle* h) { return h->cfg.p2p_chunk == 128 ? 128 : 64; }
+
 // the same-level pairs go to the matrix cores (one system; the window and
 // the offset encoding bound the opening ratio)
 static bool use_offset_m2l(const nbh_handle* h) {
@@ -644,7 +652,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         HIPCHK(h, hipEventRecord(h->ev_walk, h->stream));
         HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_walk, 0));
         record_side(h, 2);
-        if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_staged(p, F, S, h->num_leaves, h->stream2);
+        if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_staged(p, p2p_chunk(h), F, S, h->num_leaves, h->stream2);
         else launch_p2p(p, F, S, h->num_leaves, h->stream2);
         record_side(h, 3);
         HIPCHK(h, hipEventRecord(h->ev_near, h->stream2));
@@ -1143,6 +1151,61 @@ int nbh_compute_forces(nbh_handle* h, float* acc, float* pot) {
         HIPCHK(h, hipMemcpyAsync(a.data(), S.pot, n * 4, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         for (size_t i = 0; i < n; ++i) pot[ids[i]] = a[i];
+    }
+    return NBH_OK;
+}
+
+int nbh_autotune(nbh_handle* h, const uint32_t* caps, size_t ncaps, const uint32_t* chunks, size_t nchunks,
+                 int evals, nbh_tune_result* out, double* ms_table) {
+    if (!h) return NBH_E_INVALID;
+    if (h->cfg.solver != NBH_SOLVER_FMM) return h->fail(NBH_E_INVALID, "nbh_autotune: the FMM solver only");
+    if (evals < 1 || evals > 1000) return h->fail(NBH_E_INVALID, "nbh_autotune: evals must be in [1, 1000]");
+    HIPCHK(h, hipSetDevice(h->device));
+    const uint32_t cap0 = h->cfg.leaf_capacity, chunk0 = (uint32_t)p2p_chunk(h);
+    const std::vector<uint32_t> cs = (caps && ncaps) ? std::vector<uint32_t>(caps, caps + ncaps)
+                                                     : std::vector<uint32_t>{cap0};
+    const std::vector<uint32_t> ks = (chunks && nchunks) ? std::vector<uint32_t>(chunks, chunks + nchunks)
+                                                         : std::vector<uint32_t>{chunk0};
+    for (uint32_t c : cs)
+        if (c < 1 || c > 2048) return h->fail(NBH_E_INVALID, "nbh_autotune: leaf capacity %u not in [1, 2048]", c);
+    for (uint32_t k : ks)
+        if (k != 64 && k != 128) return h->fail(NBH_E_INVALID, "nbh_autotune: p2p chunk %u is not 64 or 128", k);
+    double best = 1e300;
+    uint32_t best_c = cap0, best_k = chunk0;
+    for (size_t i = 0; i < cs.size(); ++i)
+        for (size_t j = 0; j < ks.size(); ++j) {
+            h->cfg.leaf_capacity = cs[i];
+            h->cfg.p2p_chunk = ks[j];
+            // one untimed evaluation (buffers grow to this tree), then the
+            // timed ones; each evaluation ends with a host synchronisation
+            int rc = force_eval(h, 0.0);
+            const auto t0 = std::chrono::steady_clock::now();
+            for (int e = 0; e < evals && rc == NBH_OK; ++e) rc = force_eval(h, 0.0);
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / evals;
+            if (rc != NBH_OK) {
+                h->cfg.leaf_capacity = cap0;
+                h->cfg.p2p_chunk = chunk0;
+                h->acc_valid = false;
+                return rc;
+            }
+            if (ms_table) ms_table[i * ks.size() + j] = ms;
+            if (ms < best) {
+                best = ms;
+                best_c = cs[i];
+                best_k = ks[j];
+            }
+        }
+    h->cfg.leaf_capacity = best_c;
+    h->cfg.p2p_chunk = best_k;
+    // the accelerations of the chosen configuration at this state
+    CHK(force_eval(h, 0.0));
+    h->acc_valid = true;
+    if (out) {
+        out->leaf_capacity = best_c;
+        out->p2p_chunk = best_k;
+        out->ms_per_eval = best;
+        out->tried = (uint32_t)(cs.size() * ks.size());
     }
     return NBH_OK;
 }

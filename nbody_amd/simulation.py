@@ -177,6 +177,26 @@ class HipSimulation(Simulation):
         _check(self._h, lib().nbh_compute_forces(self._h, ptr(a, _fp), ptr(p, _fp)), "nbh_compute_forces")
         return a, p
 
+    def autotune(self, caps=None, chunks=None, evals=2):
+        """nbh_autotune: times `evals` force evaluations for every leaf
+        capacity in caps and P2P chunk (64 or 128) in chunks (None: keep the
+        current value), keeps the fastest pair.  Returns (leaf_capacity,
+        p2p_chunk, ms_per_eval, {(cap, chunk): ms})."""
+        cs = np.ascontiguousarray(caps if caps is not None else [], dtype=np.uint32)
+        ks = np.ascontiguousarray(chunks if chunks is not None else [], dtype=np.uint32)
+        tab = np.zeros(max(len(cs), 1) * max(len(ks), 1))
+        r = _lib.TuneResult()
+        _check(self._h, lib().nbh_autotune(self._h, ptr(cs, _u32p) if len(cs) else None, len(cs),
+                                           ptr(ks, _u32p) if len(ks) else None, len(ks), int(evals),
+                                           ctypes.byref(r), ptr(tab, _dp)), "nbh_autotune")
+        self.config.leaf_capacity = r.leaf_capacity
+        self.config.p2p_chunk = r.p2p_chunk
+        cl = list(cs) if len(cs) else [None]
+        kl = list(ks) if len(ks) else [None]
+        table = {(None if c is None else int(c), None if k is None else int(k)): float(tab[i * len(kl) + j])
+                 for i, c in enumerate(cl) for j, k in enumerate(kl)}
+        return int(r.leaf_capacity), int(r.p2p_chunk), float(r.ms_per_eval), table
+
     def direct_fp64(self, targets):
         t = np.ascontiguousarray(targets, dtype=np.int64)
         a = np.zeros((len(t), 3))
