@@ -110,7 +110,7 @@ def cpu_baseline(sample_n, budget_s):
     while True:
         run(P, 1.0, 1e-3, 1)
         steps += 1
-        if time.perf_counter() - t0 > budget_s * 0.5 or steps >= 3:
+        if time.perf_counter() - t0 > budget_s * 0.25 or steps >= 3:
             break
     dt = (time.perf_counter() - t0) / steps
     pairs_per_s = n * (n - 1) / 2 / dt
@@ -122,6 +122,22 @@ def cpu_baseline(sample_n, budget_s):
                      f"({pairs_per_s:.3g} pairs/s, -O2 -ffp-contract=off), extrapolated to N=1,048,576 "
                      f"as 2*pairs_per_s/(N-1)",
            "host_cpu": model, "nproc": nproc}
+    # the build's own OpenMP FP64 direct sum (include/nbody/cpu_simulation.h) on every host core, one
+    # force evaluation on the same sample: labelled "build oracle", not "reference"
+    from nbody_amd.cpu import CpuSimulation
+    xs = P[:, 0:3].astype(np.float64)
+    ms_ = np.full(n, 1.0 / n)
+    cpu = CpuSimulation.from_arrays(xs, np.zeros_like(xs), ms_, G=1.0, eps=1e-3)
+    t0 = time.perf_counter()
+    cpu.forces()
+    d1 = time.perf_counter() - t0
+    cpu.close()
+    p1 = n * (n - 1) / d1  # directed pairs per second
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+    out["build_oracle"] = {"value": p1 / (target_n - 1), "unit": "particle-steps/s", "cores": threads,
+                           "sample": f"the build's OpenMP FP64 direct sum, one force evaluation at N={n} in "
+                                     f"{d1:.2f} s ({p1:.3g} directed pairs/s on {threads} threads), extrapolated "
+                                     f"to N=1,048,576 as pairs_per_s/(N-1)"}
     if kind == "reference" and po.ref_lib(as_shipped=True) is not None:
         n0 = 4096
         t0 = time.perf_counter()
@@ -288,16 +304,16 @@ def workload_clustered(nb, args):
 def workload_ensemble(nb, args):
     """256 independent Plummer systems (seeds 100-355), N log-uniform in
     [512, 65536], eps=1e-2, theta=0.5, p=4, one handle; the library's
-    autotuner (nbh_autotune) picks the leaf capacity from 16, 24, 32, 48, 64
-    and the P2P tile from 64 and 128 staged sources (2 force evaluations
-    each); then 20 timed steps and the per-system force error on 256 targets
-    each."""
+    autotuner (nbh_autotune) picks the leaf capacity from 16, 24, 32, 48, 64,
+    96, 128 and the P2P tile from 64 and 128 staged sources (2 force
+    evaluations each); then 20 timed steps and the per-system force error on
+    256 targets each."""
     x, v, m, off = nb.ics.ensemble(256, 100, 512, 65536)
     n = len(m)
     cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=32, order=4)
     sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg, sys_off=off)
     t0 = time.perf_counter()
-    cap, chunk, tuned_ms, table = sim.autotune([16, 24, 32, 48, 64], [64, 128], evals=2)
+    cap, chunk, tuned_ms, table = sim.autotune([16, 24, 32, 48, 64, 96, 128], [64, 128], evals=2)
     tune_s = time.perf_counter() - t0
     tune = {f"cap{c}_chunk{k}": ms for (c, k), ms in table.items()}
     acc, _ = sim.forces()
@@ -418,7 +434,7 @@ def main():
     ap.add_argument("--roofline-steps", type=int, default=3,
                     help="steps of the serialized roofline pass after the timed steps (0: take the roofline "
                          "from the timed steps)")
-    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
