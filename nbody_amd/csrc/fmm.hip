@@ -501,14 +501,14 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_unsoftened_kernel
 // rows made this contraction L2-bandwidth-bound.)
 template <int Q>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) void dtable_kernel(
-    FmmParams F, float* out, int level0, int nlev, int E, int a, int b, int shift) {
+    FmmParams F, float* out, int level0, int nlev, int E, int a, int b, int shift, int nsys) {
     constexpr int NCQ = ncoef(Q);
     const int per = E * E * E;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= per * nlev) return;
-    const int lv = t / per, o = t % per;
+    if (t >= per * nlev * nsys) return;
+    const int s = t / (per * nlev), lv = (t / per) % nlev, o = t % per;
     const int ex = o / (E * E), ey = (o / E) % E, ez = o % E;
-    const double h = ldexp(F.box[0].w, -(level0 + lv + shift));
+    const double h = ldexp(F.box[s].w, -(level0 + lv + shift));
     double D[NCQ];
     derivs<Q, double>(-(a + b * ex) * h, -(a + b * ey) * h, -(a + b * ez) * h, (double)F.eps2_64, D);
     float* row = out + (size_t)t * F.dcp;
@@ -546,8 +546,10 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
         }
         __syncthreads();
     }
-    const float4* tab = IN_LDS ? dtab_lds : src;
     constexpr int RS = IN_LDS ? NQS : NQD;
+    // one system per launch when the table is in LDS; else each target reads
+    // its own system's table from global memory
+    const size_t sys_stride = (size_t)F.dnlev * F.dsz * NQD;
     const int lane = lane_id();
     const uint32_t wpb = blockDim.x / WAVE;
     int rs0, rsn;  // coefficients [rs0, rs0 + rsn) of this lane after the reduction
@@ -558,6 +560,7 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
     for (uint32_t t = blockIdx.x * wpb + threadIdx.x / WAVE; t < count; t += gridDim.x * wpb) {
         const uint32_t A = off + t;
         const int4 ga = F.nodes.geo[A];
+        const float4* tab = IN_LDS ? dtab_lds : src + (size_t)F.nodes.sys[A] * sys_stride;
         float L[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = 0.f;
@@ -651,7 +654,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 // the levels differ by more than one (or the offset lies beyond the table).
 // Centres in units of half the finer cell: 2 i + 1 at the finer level,
 // 4 i + 2 at the coarser; the difference is odd.
-__device__ __forceinline__ int xtab_index(const FmmParams& F, int4 ga, int4 gb, uint32_t& neg) {
+__device__ __forceinline__ int xtab_index(const FmmParams& F, int4 ga, int4 gb, int sys, uint32_t& neg) {
     const int la = ga.w & 0xff, lb = gb.w & 0xff;
     if (abs(la - lb) != 1) return -1;
     const int lf = la > lb ? la : lb;
@@ -663,7 +666,7 @@ __device__ __forceinline__ int xtab_index(const FmmParams& F, int4 ga, int4 gb, 
     const int E = F.xE;
     if (ex >= E || ey >= E || ez >= E || lf < F.xlevel0 || lf >= F.xlevel0 + F.xnlev) return -1;
     neg = (uint32_t)(dx < 0) | ((uint32_t)(dy < 0) << 1) | ((uint32_t)(dz < 0) << 2);
-    return ((lf - F.xlevel0) * E + ex) * E * E + ey * E + ez;
+    return (((sys * F.xnlev + lf - F.xlevel0) * E + ex) * E + ey) * E + ez;
 }
 
 // The other-level tails of the rows (after the same-level kernel): pairs one
@@ -696,6 +699,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 
         const uint32_t A = A0 + half;
         const bool act = A < num_nodes;
         const int4 ga = act ? F.nodes.geo[A] : make_int4(0, 0, 0, 0);
+        const int sysA = act ? F.nodes.sys[A] : 0;
         float L[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) L[k] = 0.f;
@@ -708,7 +712,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 
                 for (uint32_t e = e1 - nx + sub; e < e1; e += 32) {
                     const int B = F.lists.m_pool[e];
                     uint32_t neg = 0;
-                    const int t = xtab_index(F, ga, F.nodes.geo[B], neg);
+                    const int t = xtab_index(F, ga, F.nodes.geo[B], sysA, neg);
                     if (t < 0) {
                         ++nfar;
                         continue;
@@ -757,7 +761,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 
                 if (e < e1) {
                     B = F.lists.m_pool[e];
                     uint32_t neg = 0;
-                    far = xtab_index(F, ga, F.nodes.geo[B], neg) < 0;
+                    far = xtab_index(F, ga, F.nodes.geo[B], sysA, neg) < 0;
                 }
                 const uint64_t bal = __ballot(far) & hmask;
                 if (far) {
@@ -857,7 +861,7 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
         }
     }
     const size_t lds = (size_t)F.dsz * ((F.dcp / 4) | 1) * 16;  // rows padded to an odd float4 count
-    const bool in_lds = lds <= plan.max_lds && m2l_table_threads(P, Q) == 1024;
+    const bool in_lds = plan.nsys == 1 && lds <= plan.max_lds && m2l_table_threads(P, Q) == 1024;
     if (in_lds && (parts & M2L_LEVELS))
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -893,10 +897,10 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
 
 template <int Q>
 static void launch_dtable_t(const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
-                            hipStream_t st) {
-    const unsigned total = (unsigned)(E * E * E * nlev);
+                            int nsys, hipStream_t st) {
+    const unsigned total = (unsigned)(E * E * E * nlev * nsys);
     hipLaunchKernelGGL((dtable_kernel<Q>), dim3(ceil_div(total, 128)), dim3(128), 0, st, F, out, level0, nlev, E, a,
-                       b, shift);
+                       b, shift, nsys);
 }
 
 template <int P>
@@ -965,8 +969,8 @@ void launch_p2p_staged(int p, int chunk, const FmmParams& F, const State& S, uin
     else NBH_DISPATCH_P(p, (launch_p2p_staged_t<PP, 1>(F, S, num_leaves, st)))
 }
 void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
-                   hipStream_t st) {
-#define NBH_DT(QQ) launch_dtable_t<QQ>(F, out, level0, nlev, E, a, b, shift, st)
+                   int nsys, hipStream_t st) {
+#define NBH_DT(QQ) launch_dtable_t<QQ>(F, out, level0, nlev, E, a, b, shift, nsys, st)
     switch (q) {
         case 1: NBH_DT(1); break;
         case 2: NBH_DT(2); break;

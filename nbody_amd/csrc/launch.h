@@ -118,12 +118,14 @@ struct FmmParams {
     ListView lists;
     unsigned long long* p2p_inter;  // statistics: particle pairs of the P2P kernels
     uint32_t* pleaf;       // per particle: its leaf node
-    float* dtab;           // per level, derivative tensors of the offsets [0, K]^3 (or null)
+    float* dtab;           // per system and level, derivative tensors of the offsets [0, K]^3 (or null);
+                           // system s, level l at row ((s dnlev + l - dlevel0) dsz + offset)
     int dK, dlevel0, dnlev;
     int dcp;               // tensor row stride (floats): ncoef(Q) padded to 4
     int dsz;               // tensors per level: (K+1)^3
-    // pairs one level apart: per finer level lf, tensors of the centre offsets
-    // d (units of half the finer cell, odd) with |d_i| = 1, 3, .., 2 xE - 1
+    // pairs one level apart: per system and finer level lf, tensors of the
+    // centre offsets d (units of half the finer cell, odd) with |d_i| = 1, 3,
+    // .., 2 xE - 1
     float* xtab;
     int xE, xlevel0, xnlev;
     // pairs further apart in level: deferred as (target, source) pairs, the
@@ -146,8 +148,9 @@ void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, 
 // chunk: sources staged per wavefront round, 64 or 128 (the tile shape)
 void launch_p2p_staged(int p, int chunk, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
 void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st);
-// M2L of order (p, q): q = p keeps |k| + |n| <= p, q = p + 1 also the terms
-// of degree p + 1 with |n| = 1 or |k| = 1 (the expansions stay of order p).
+// M2L of order p with the truncation code q (expansion.h: m2l_deg(q) = p or
+// p + 1; q | M2L_ALL keeps every term of degree <= p + 1, q = p + 1 only
+// those with |n| = 1 or |k| = 1 beyond degree p).
 // ---- same-level M2L by translation offset on the matrix cores (m2l_offset.hip)
 constexpr int OFF_J = 4;      // translation offsets per batch (along the contraction)
 constexpr int OFF_WAVES = 4;  // wavefronts per workgroup (one 4x4x4 block of targets)
@@ -184,11 +187,13 @@ struct M2LPlan {
     uint32_t level_off[MAX_LEVELS + 2];
     uint32_t level_count[MAX_LEVELS + 1];
     size_t max_lds;  // LDS bytes per workgroup: the same-level table is held in LDS if it fits
+    int nsys;        // systems: with more than one, each target reads its own system's table
 };
 // tensors at R = -(a + b e) h_l / 2^shift for e in [0, E)^3, levels
-// level0 .. level0+nlev-1, into out (rows of F.dcp floats)
+// level0 .. level0+nlev-1 of systems 0 .. nsys-1 (h_l from each system's
+// root size), into out (rows of F.dcp floats, system-major, then level)
 void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
-                   hipStream_t st);
+                   int nsys, hipStream_t st);
 void launch_m2l(int p, int q, const FmmParams& F, const M2LPlan& plan, hipStream_t st);
 // The grouped M2L in parts (when F carries the tables): the same-level kernel
 // of one level (level < 0: every level), and the rest (initialisation, the

@@ -561,6 +561,7 @@ static M2LPlan m2l_plan(const nbh_handle* h) {
     }
     pl.num_nodes = h->num_nodes;
     pl.max_lds = (size_t)h->max_lds;
+    pl.nsys = h->nsys;
     std::copy(h->hc->level_off, h->hc->level_off + MAX_LEVELS + 2, pl.level_off);
     std::copy(h->hc->level_count, h->hc->level_count + MAX_LEVELS + 1, pl.level_count);
     return pl;
@@ -585,17 +586,17 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         launch_m2m_level(p, F, l, h->hc->level_count[l], h->hc->level_off[l], h->stream2);
     record_side(h, 1);
     const int q = m2l_q(h);
-    const bool tables = h->cfg.m2l != NBH_M2L_PLAIN && h->nsys == 1 && h->depth >= 2;
+    const bool tables = h->cfg.m2l != NBH_M2L_PLAIN && h->depth >= 2;
     FmmParams T{};  // the table part of the parameters
     if (tables) {
         // derivative tensors of the same-level offsets [0, K]^3 of levels
-        // 2..depth (cells of levels 0 and 1 are never well separated); one
-        // system only, since the tensors depend on the root size
+        // 2..depth (cells of levels 0 and 1 are never well separated), per
+        // system, since the tensors depend on each system's root size
         const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
         const int K = 2 * Kp + 1;
         const int dsz = (K + 1) * (K + 1) * (K + 1), dcp = ncp_of(m2l_deg(q));
         const int nlev = (int)h->depth - 1;
-        HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4));
+        HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4 * h->nsys));
         T = F;
         T.dtab = h->dtab.as<float>();
         T.dK = K;
@@ -603,18 +604,18 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         T.dnlev = nlev;
         T.dcp = dcp;
         T.dsz = dsz;
-        launch_dtable(m2l_deg(q), T, T.dtab, 2, nlev, K + 1, 0, 1, 0, h->stream2);
+        launch_dtable(m2l_deg(q), T, T.dtab, 2, nlev, K + 1, 0, 1, 0, (int)h->nsys, h->stream2);
         // pairs one level apart, by finer level 1..depth: offsets |d_i| <=
         // 4 Kp + 1 half finer cells (the coarser side's parent pair was
         // split), tabled at d_i = 1, 3, .., 4 Kp + 1
         const int E = 2 * Kp + 1;
         const int xn = (int)h->depth;
-        HIPCHK(h, h->xtab.ensure((size_t)E * E * E * xn * dcp * 4));
+        HIPCHK(h, h->xtab.ensure((size_t)E * E * E * xn * dcp * 4 * h->nsys));
         T.xtab = h->xtab.as<float>();
         T.xE = E;
         T.xlevel0 = 1;
         T.xnlev = xn;
-        launch_dtable(m2l_deg(q), T, T.xtab, 1, xn, E, 1, 2, 1, h->stream2);
+        launch_dtable(m2l_deg(q), T, T.xtab, 1, xn, E, 1, 2, 1, (int)h->nsys, h->stream2);
         if (use_offset_m2l(h)) {
             // the matrix-core same-level M2L: signed split moments and the
             // dense tensors (orders <= 2p) of the offsets [0, K]^3, levels 2..depth

@@ -42,7 +42,21 @@ def test_ensemble_matches_oracle_and_direct():
     assert np.array_equal(tsys, tree.sys[lists.m2l_src])
     acc_o, _ = po.fmm(tree, lists, 4, x[ids], m[ids], box_o, eps=1e-2, m2l_order=sim.m2l_order, m2l_terms=sim.m2l_terms,
                       same_level_order=sim.same_level_order)
-    assert rms_rel(acc[ids], acc_o) < 2e-5
+    # every system reads its own per-level tensor tables (root sizes differ)
+    e_rms, e_max = rms_rel(acc[ids], acc_o), max_rel_floor(acc[ids], acc_o)
+    print(f"  fp32 vs fp64-fmm: rms {e_rms:.2e} max {e_max:.2e}", flush=True)
+    assert e_rms < 2e-5 and e_max < 1e-3
+    # the force VJP through the same per-system tables, against the oracle's transpose
+    w = np.random.default_rng(3).standard_normal((n, 3))
+    gx, gm = sim.force_vjp(w)
+    gxo, gmo = po.fmm_vjp(tree, lists, 4, x[ids], m[ids], w[ids], box_o, eps=1e-2, m2l_order=sim.m2l_order,
+                          m2l_terms=sim.m2l_terms, same_level_order=sim.same_level_order)
+    inv = np.empty(n, dtype=np.int64)
+    inv[ids] = np.arange(n)
+    e_gx, e_gm = rms_rel(gx, gxo[inv]), rms_rel(gm, gmo[inv])
+    print(f"  force vjp vs fp64 transpose: gx {e_gx:.2e} gm {e_gm:.2e}", flush=True)
+    assert e_gx < 1e-4 and e_gm < 1e-4
+    acc, pot = sim.forces()
     # per-system error vs the FP64 direct sum on up to 256 targets each
     worst = 0.0
     for s in range(16):
