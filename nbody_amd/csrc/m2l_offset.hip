@@ -362,8 +362,10 @@ void m2l_offset_kernel(FmmParams F, OffsetParams O, uint32_t goff, int lvl) {
     constexpr int NQ = (DCP + WAVE - 1) / WAVE;  // tensor words per lane per row
     const int W = O.W, K = O.K;
     int* cnt = (int*)off_lds;         // [64] pairs per target, summed over the wavefronts
-    int* ls0 = cnt + 64;              // [64] start of each target's same-level entries
-    int* lpre = ls0 + 64;             // [65] exclusive prefix sum of their counts
+    // [64] start of each target's same-level entries: a pool position, which
+    // exceeds 2^31 on large trees (unsigned throughout)
+    uint32_t* ls0 = (uint32_t*)(cnt + 64);
+    int* lpre = (int*)(ls0 + 64);     // [65] exclusive prefix sum of their counts
     int* win = (int*)(off_lds + Lay::WIN);
     const int tid = threadIdx.x, lane = lane_id(), wave = tid / WAVE;
     uint32_t* db = (uint32_t*)(off_lds + Lay::DB) + wave * OFF_J * DCP;
@@ -423,7 +425,7 @@ void m2l_offset_kernel(FmmParams F, OffsetParams O, uint32_t goff, int lvl) {
         int c = 0;
         if (tv) {
             const uint32_t r = F.lists.first[gc0 + lane];
-            ls0[lane] = (int)F.lists.m_start[r];
+            ls0[lane] = F.lists.m_start[r];
             c = (int)(F.lists.m_cnt[r] - F.lists.mx_cnt[r]);
         }
         // exclusive prefix sum over the wavefront
@@ -453,7 +455,7 @@ void m2l_offset_kernel(FmmParams F, OffsetParams O, uint32_t goff, int lvl) {
                 Bv[u] = -1;
                 if (e < total) {
                     while (lpre[tcur + 1] <= e) ++tcur;
-                    Bv[u] = F.lists.m_pool[ls0[tcur] + (e - lpre[tcur])];
+                    Bv[u] = F.lists.m_pool[ls0[tcur] + (uint32_t)(e - lpre[tcur])];
                 }
             }
             int4 gv[U];
