@@ -174,15 +174,16 @@ def copy_calibration(nbytes):
 SUMMED = ["ms_m2l_same", "m2l_same_pairs", "m2l_same_launches", "ms_p2p", "p2p_interactions"]
 
 
-def time_steps(sim, warmup, steps, serial=False, totals=None):
+def time_steps(sim, warmup, steps, serial=False, totals=None, timing=True):
     """warmup untimed steps, then `steps` timed ones (each step ends with a
     stream synchronisation inside the library), with the library's per-phase
-    HIP events on; serial: every kernel of an evaluation on one stream, so
-    that each event pair brackets one kernel running alone.  totals (a dict)
-    receives the sums over the timed steps of the SUMMED statistics."""
+    HIP events on (timing=False: off, which also lets captured steps replay);
+    serial: every kernel of an evaluation on one stream, so that each event
+    pair brackets one kernel running alone.  totals (a dict) receives the
+    sums over the timed steps of the SUMMED statistics."""
     for _ in range(warmup):
         sim.step(1)
-    sim.set_timing(True, serial=serial)
+    sim.set_timing(timing, serial=serial)
     agg = {k: 0.0 for k in ["ms_tree", "ms_moments", "ms_walk", "ms_m2l", "ms_l2l", "ms_p2p", "ms_total"]}
     inter = m2l_pairs = split_pairs = p2p_pairs = 0
     if totals is not None:
@@ -224,13 +225,19 @@ def workload_uniform4096(nb, args):
     n, K = 4096, 10
     x, v, m = nb.ics.uniform(n, 0)
     res = {}
-    for label, m2l, p2p in [("tuned", nb.M2L_GROUPED, nb.P2P_LANE), ("plain_port", nb.M2L_PLAIN, nb.P2P_LEAF)]:
-        cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=16, order=4, m2l=m2l, p2p=p2p)
+    for label, m2l, p2p, graph in [("tuned", nb.M2L_GROUPED, nb.P2P_LANE, 0),
+                                   ("tuned_graph", nb.M2L_GROUPED, nb.P2P_LANE, 1),
+                                   ("plain_port", nb.M2L_PLAIN, nb.P2P_LEAF, 0)]:
+        cfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=16, order=4, m2l=m2l, p2p=p2p,
+                                graph=graph)
         sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
         err = force_error(sim, n, n, 0)
-        agg, wall, *_ = time_steps(sim, 2, K)
+        agg, wall, *_ = time_steps(sim, 2, K, timing=not graph)
+        st = sim.stats()
         sim.close()
         res[label] = {"particle_steps_per_s": n * K / wall, "ms_per_step": wall / K * 1e3, "force_err": err}
+        if graph:
+            res[label]["graph"] = {k: st[k] for k in ("graph_captures", "graph_replays", "graph_fallbacks")}
     out = {"metric": "particle-steps/s, N=4096 uniform, 10 KDK steps; force error vs FP64 direct",
            "value": res["tuned"]["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
            "warmup": 2, "ms_per_step": res["tuned"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -333,6 +340,13 @@ def workload_ensemble(nb, args):
     K = 20
     agg, wall, *_ = time_steps(sim, 2, K)
     sim.close()
+    # the same tuned configuration with captured steps (timing off)
+    gcfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=cap, p2p_chunk=chunk, order=4,
+                             graph=1)
+    gsim = nb.HipSimulation.from_arrays(x, v, m, m, config=gcfg, sys_off=off)
+    _, gwall, *_ = time_steps(gsim, 2, K, timing=False)
+    gst = gsim.stats()
+    gsim.close()
     return {"metric": "aggregate particle-steps/s, 256-system Plummer ensemble; per-system force error",
             "value": n * K / wall, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": 2,
             "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -341,6 +355,8 @@ def workload_ensemble(nb, args):
                        "n_total": n, "leaf_capacity": cap, "p2p_chunk": chunk},
             "autotune": {"ms_per_eval": tune, "chosen": {"leaf_capacity": cap, "p2p_chunk": chunk},
                          "seconds": tune_s},
+            "graph": {"ms_per_step": gwall / K * 1e3, "particle_steps_per_s": n * K / gwall,
+                      **{k: gst[k] for k in ("graph_captures", "graph_replays", "graph_fallbacks")}},
             "force_err": {"per_system_rms_median": float(np.median(errs)), "per_system_rms_max": float(np.max(errs))},
             "phases_ms": {k[3:]: agg[k] / K for k in agg}}
 

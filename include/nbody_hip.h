@@ -95,6 +95,9 @@ typedef struct nbh_config {
                                the M2L keeps (no effect at q = p) */
     uint32_t p2p_chunk;     /* NBH_P2P_LANE tile shape: source particles staged in LDS per
                                wavefront round, 64 or 128 (0 = 64); nbh_autotune picks it */
+    uint32_t graph;         /* 1: capture each KDK step in a HIP graph and replay it (no host
+                               round trip inside the step; FMM with NBH_M2L_GROUPED and
+                               NBH_P2P_LANE, timing off); 0: off */
 } nbh_config;
 
 typedef struct nbh_handle nbh_handle;
@@ -112,6 +115,9 @@ typedef struct nbh_stats {
     uint64_t m2l_same_pairs;  /* directed M2L entries between cells of one level */
     uint64_t m2l_same_launches; /* timing: same-level M2L launches bracketed by events */
     double ms_m2l_same;       /* timing: sum of their durations, each on its own stream */
+    uint64_t graph_captures;  /* cfg.graph: graphs captured, steps replayed from one, and replayed */
+    uint64_t graph_replays;   /* steps that outgrew the captured shapes (rolled back, run */
+    uint64_t graph_fallbacks; /* uncaptured; the next step captures anew); cumulative */
 } nbh_stats;
 
 int nbh_abi_version(void);

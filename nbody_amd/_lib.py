@@ -50,6 +50,7 @@ class Config(ctypes.Structure):
         ("m2l_order", ctypes.c_uint32),
         ("m2l_terms", ctypes.c_uint32),
         ("p2p_chunk", ctypes.c_uint32),
+        ("graph", ctypes.c_uint32),
     ]
 
 
@@ -73,6 +74,9 @@ class Stats(ctypes.Structure):
         ("m2l_same_pairs", ctypes.c_uint64),
         ("m2l_same_launches", ctypes.c_uint64),
         ("ms_m2l_same", ctypes.c_double),
+        ("graph_captures", ctypes.c_uint64),
+        ("graph_replays", ctypes.c_uint64),
+        ("graph_fallbacks", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -44,7 +44,16 @@ struct Counters {
 
 // OVF_GEOM: a same-level M2L block found a pair count or a source position
 // that differs from its list (an internal consistency check)
-enum : uint32_t { OVF_NODES = 1, OVF_SL = 2, OVF_ROWS = 4, OVF_M2L = 8, OVF_P2P = 16, OVF_GEOM = 32 };
+// OVF_FAR: the deferred-pair buffers were too small (captured steps size
+// them from the previous step)
+enum : uint32_t { OVF_NODES = 1, OVF_SL = 2, OVF_ROWS = 4, OVF_M2L = 8, OVF_P2P = 16, OVF_GEOM = 32, OVF_FAR = 64 };
+constexpr uint32_t OVF_WALK = OVF_NODES | OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P;
+
+// A count argument equal to DEV_COUNT means "read the count from the device
+// counters": the launch shapes of a captured step do not depend on values
+// read back by the host, and the kernels loop over what the tree and the
+// walk actually produced.
+constexpr uint32_t DEV_COUNT = 0xffffffffu;
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 

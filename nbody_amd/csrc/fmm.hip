@@ -44,8 +44,11 @@ void launch_gather_state(const State& src, const State& dst, const uint32_t* per
 template <int P>
 __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t num_leaves) {
     constexpr int NC = ncoef(P);
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= num_leaves) return;
+    if (num_leaves == DEV_COUNT) {
+        if (F.C->overflow & OVF_NODES) return;
+        num_leaves = F.C->num_leaves;
+    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < num_leaves; t += gridDim.x * blockDim.x) {
     const int A = (int)F.leaf_ids[t];
     const int4 g = F.nodes.geo[A];
     const int4 lk = F.nodes.link[A];
@@ -68,6 +71,7 @@ __global__ __launch_bounds__(256) void p2m_kernel(FmmParams F, State S, uint32_t
     float* out = F.M + (size_t)A * F.ncp;
 #pragma unroll
     for (int k = 0; k < NC; ++k) out[k] = M[k];
+    }
 }
 
 // L2P of the far field, one lane per particle: phi and grad phi of its
@@ -107,12 +111,16 @@ template <int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2m_kernel(FmmParams F, uint32_t count,
                                                                                            uint32_t off) {
     constexpr int NC = ncoef(P);
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
+    if (count == DEV_COUNT) {  // off is the level
+        if (F.C->overflow & OVF_NODES) return;
+        count = F.C->level_count[off];
+        off = F.C->level_off[off];
+    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
     const int A = (int)(off + t);
     const int4 g = F.nodes.geo[A];
     const int nch = g.w >> 8;
-    if (nch == 0) return;
+    if (nch == 0) continue;
     const int4 lk = F.nodes.link[A];
     const float unit = F.unit[F.nodes.sys[A]];
     float Mp[NC];
@@ -127,6 +135,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     float* out = F.M + (size_t)A * F.ncp;
 #pragma unroll
     for (int k = 0; k < NC; ++k) out[k] = Mp[k];
+    }
 }
 
 // M2L, plain: one thread per target cell, the derivative tensor rebuilt per
@@ -166,12 +175,16 @@ template <int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void l2l_kernel(FmmParams F, uint32_t count,
                                                                                            uint32_t off) {
     constexpr int NC = ncoef(P);
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
+    if (count == DEV_COUNT) {  // off is the level
+        if (F.C->overflow & OVF_WALK) return;
+        count = F.C->level_count[off];
+        off = F.C->level_off[off];
+    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
     const int A = (int)(off + t);
     const int4 g = F.nodes.geo[A];
     const int par = F.nodes.link[A].w;
-    if (par < 0) return;
+    if (par < 0) continue;
     const int4 gp = F.nodes.geo[par];
     const float3 d = centre_diff(gp, g, F.bits, F.unit[F.nodes.sys[A]]);  // c_child - c_parent
     float Lp[NC], Lc[NC];
@@ -187,6 +200,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     l2l_apply<P>(d.x, d.y, d.z, Lp, Lc);
 #pragma unroll
     for (int k = 0; k < NC; ++k) sc[k] = Lc[k];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -304,6 +318,7 @@ constexpr int p2p_stage_slots() { return SPL == 1 ? 96 : 160; }
 // potential); else the self pair and the empty slots are skipped by index.
 template <bool SOFT, int SPL>
 __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State& S, uint32_t num_leaves) {
+    // (num_leaves == DEV_COUNT: the leaf count of the device counters)
     constexpr int P2P_STAGE = p2p_stage_slots<SPL>();
     constexpr int CHUNK = WAVE * SPL;
     __shared__ float4 stage_s[P2P_WAVES][P2P_STAGE];
@@ -320,6 +335,10 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
     int* sidx = stage_i[wslot];
     int2* ent = ent_s[wslot];
     float4* entd = entd_s[wslot];
+    if (num_leaves == DEV_COUNT) {
+        if (F.C->overflow & OVF_WALK) return;
+        num_leaves = F.C->num_leaves;
+    }
     // slots past the chunk are never staged: zero charges for good
     static_assert(P2P_STAGE - CHUNK <= WAVE, "one slot per lane");
     if (lane < P2P_STAGE - CHUNK) {
@@ -536,6 +555,14 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
     // 14 at q = 5, maps every row to one of four groups of the 32 banks)
     constexpr int NQS = NQD | 1;
     extern __shared__ float4 dtab_lds[];
+    // launched beside the walk: after an overflow at this depth or above the
+    // level's rows are incomplete, and the host repeats the walk and M2L
+    if (F.C->overflow & OVF_WALK) return;
+    if (count == DEV_COUNT) {
+        count = F.C->level_count[lvl + F.dlevel0];
+        off = F.C->level_off[lvl + F.dlevel0];
+    }
+    if (blockIdx.x * (blockDim.x / WAVE) >= count) return;  // uniform: no target for this block
     const int K = F.dK, K1 = K + 1;
     const float4* src = reinterpret_cast<const float4*>(F.dtab + (size_t)lvl * F.dsz * F.dcp);
     if (IN_LDS) {
@@ -554,9 +581,6 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
     const uint32_t wpb = blockDim.x / WAVE;
     int rs0, rsn;  // coefficients [rs0, rs0 + rsn) of this lane after the reduction
     rs_slot(NC, 6, lane, rs0, rsn);
-    // launched beside the walk: after an overflow at this depth or above the
-    // level's rows are incomplete, and the host repeats the walk and M2L
-    if (F.C->overflow & (OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P)) return;
     for (uint32_t t = blockIdx.x * wpb + threadIdx.x / WAVE; t < count; t += gridDim.x * wpb) {
         const uint32_t A = off + t;
         const int4 ga = F.nodes.geo[A];
@@ -684,6 +708,10 @@ template <int P, int Q>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 : 1))) void m2l_cross_kernel(
     FmmParams F, uint32_t num_nodes) {
     constexpr int NC = ncoef(P), NCQ = ncoef(m2l_deg(Q)), NQD = (NCQ + 3) / 4;
+    if (num_nodes == DEV_COUNT) {
+        if (F.C->overflow & OVF_WALK) return;
+        num_nodes = F.C->num_nodes;
+    }
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const uint32_t nw = gridDim.x * blockDim.x / WAVE;
     const int lane = lane_id();
@@ -748,8 +776,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 
         }
         s0 = __shfl(s0, lane & 32, WAVE);
         // the capacity is the walk's count of other-level entries, which
-        // bounds the deferred pairs
-        if (!defer || (unsigned long long)s0 + nfar > F.far_cap) continue;
+        // bounds the deferred pairs (a captured step sizes it from the
+        // previous step and flags a shortfall)
+        if (defer && (unsigned long long)s0 + nfar > F.far_cap) {
+            if (sub == 0) atomicOr(const_cast<uint32_t*>(&F.C->overflow), (uint32_t)OVF_FAR);
+            continue;
+        }
+        if (!defer) continue;
         uint32_t pos = s0;
         for (uint32_t r = F.lists.first[A]; r != NO_ROW; r = F.lists.next[r]) {
             const uint32_t e1 = F.lists.m_start[r] + F.lists.m_cnt[r];
@@ -785,6 +818,10 @@ template <int P, int Q>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_far_kernel(FmmParams F,
                                                                                                 uint32_t num_nodes) {
     constexpr int NC = ncoef(P);
+    if (num_nodes == DEV_COUNT) {
+        if (F.C->overflow & OVF_WALK) return;
+        num_nodes = F.C->num_nodes;
+    }
     const uint32_t npairs = min(F.far_cnt[1], F.far_cap);
     const int lane = lane_id();
     const uint32_t w0 = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE * WAVE;
@@ -835,6 +872,15 @@ __global__ void zero_f32_kernel(float* p, size_t n) {
     if (i < n) p[i] = 0.f;
 }
 
+// rows [0, level_off[level]) of L (the cells above the tabled levels), the
+// count read from the device counters
+__global__ void zero_top_kernel(float* L, const Counters* C, int level, int ncp) {
+    if (C->overflow & OVF_WALK) return;
+    const size_t n = (size_t)C->level_off[level] * ncp;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        L[i] = 0.f;
+}
+
 template <int P, int Q>
 // parts: M2L_LEVELS runs the same-level kernel of level `level` (all tabled
 // levels when level < 0); M2L_REST runs everything else (initialisation, the
@@ -854,15 +900,22 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
     if (parts & M2L_REST) {
         hipLaunchKernelGGL(zero_u32x2_kernel, dim3(1), dim3(64), 0, st, F.far_cnt);
         // cells above the tabled levels have no same-level pairs: L starts at 0
-        const uint32_t top = plan.level_off[F.dlevel0];
-        if (top) {
-            const size_t nz = (size_t)top * F.ncp;
-            hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)ceil_div(nz, (size_t)256)), dim3(256), 0, st, F.L, nz);
+        if (plan.dev) {
+            hipLaunchKernelGGL(zero_top_kernel, dim3(64), dim3(256), 0, st, F.L, F.C, F.dlevel0, F.ncp);
+        } else {
+            const uint32_t top = plan.level_off[F.dlevel0];
+            if (top) {
+                const size_t nz = (size_t)top * F.ncp;
+                hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)ceil_div(nz, (size_t)256)), dim3(256), 0, st, F.L,
+                                   nz);
+            }
         }
     }
     const size_t lds = (size_t)F.dsz * ((F.dcp / 4) | 1) * 16;  // rows padded to an odd float4 count
     const bool in_lds = plan.nsys == 1 && lds <= plan.max_lds && m2l_table_threads(P, Q) == 1024;
-    if (in_lds && (parts & M2L_LEVELS))
+    // (set by the uncaptured evaluation that precedes every capture: not a
+    // stream operation, kept out of a capture)
+    if (in_lds && (parts & M2L_LEVELS) && !plan.dev)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     for (int l = F.dlevel0; l < F.dlevel0 + F.dnlev; ++l) {
@@ -878,21 +931,25 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
         }
         constexpr int T = m2l_table_threads(P, Q);
         // one target per wavefront: the workgroup scheduler balances the
-        // uneven list lengths better than a grid-stride loop (measured)
+        // uneven list lengths better than a grid-stride loop (measured);
+        // with device counts the grid is sized from plan.level_count and the
+        // kernel strides over the rest
         const unsigned g = std::min<unsigned>(ceil_div(cnt, T / WAVE), 65535u);
+        const uint32_t koff = plan.dev ? 0u : plan.level_off[l], kcnt = plan.dev ? DEV_COUNT : cnt;
         if (in_lds)
-            hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true>), dim3(g), dim3(T), lds, st, F, plan.level_off[l], cnt,
+            hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true>), dim3(g), dim3(T), lds, st, F, koff, kcnt,
                                l - F.dlevel0);
         else
-            hipLaunchKernelGGL((m2l_lds_kernel<P, Q, false>), dim3(g), dim3(T), 0, st, F, plan.level_off[l], cnt,
+            hipLaunchKernelGGL((m2l_lds_kernel<P, Q, false>), dim3(g), dim3(T), 0, st, F, koff, kcnt,
                                l - F.dlevel0);
     }
     if (!(parts & M2L_REST)) return;
+    const uint32_t knn = plan.dev ? DEV_COUNT : nn;
     hipLaunchKernelGGL((m2l_cross_kernel<P, Q>), dim3(std::min<unsigned>(ceil_div(nn, 8), 65536u)), dim3(256), 0, st,
-                       F, nn);
+                       F, knn);
     // the deferred pairs are a subset of the other-level entries (far_cap)
     const unsigned gfar = std::max<unsigned>(1u, std::min<unsigned>(ceil_div(F.far_cap, 256u), 1024u));
-    hipLaunchKernelGGL((m2l_far_kernel<P, Q>), dim3(gfar), dim3(256), 0, st, F, nn);
+    hipLaunchKernelGGL((m2l_far_kernel<P, Q>), dim3(gfar), dim3(256), 0, st, F, knn);
 }
 
 template <int Q>
@@ -903,21 +960,27 @@ static void launch_dtable_t(const FmmParams& F, float* out, int level0, int nlev
                        b, shift, nsys);
 }
 
+// grid of the launches whose count is read on the device (DEV_COUNT): sized
+// for grid_items items; the kernels loop over the actual count
+static uint32_t grid_items_of(uint32_t count, uint32_t grid_items) {
+    return count == DEV_COUNT ? std::max<uint32_t>(grid_items, 1u) : count;
+}
+
 template <int P>
-static void launch_p2m_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
-    hipLaunchKernelGGL((p2m_kernel<P>), dim3(ceil_div(num_leaves, 256)), dim3(256), 0, st, F, S, num_leaves);
+static void launch_p2m_t(const FmmParams& F, const State& S, uint32_t num_leaves, uint32_t items, hipStream_t st) {
+    hipLaunchKernelGGL((p2m_kernel<P>), dim3(ceil_div(items, 256)), dim3(256), 0, st, F, S, num_leaves);
 }
 template <int P>
 static void launch_l2p_t(const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st) {
     hipLaunchKernelGGL((l2p_kernel<P>), dim3(ceil_div(n, 256)), dim3(256), 0, st, F, S, n, kick_dt);
 }
 template <int P>
-static void launch_m2m_t(const FmmParams& F, uint32_t count, uint32_t off, hipStream_t st) {
-    hipLaunchKernelGGL((m2m_kernel<P>), dim3(ceil_div(count, 64)), dim3(64), 0, st, F, count, off);
+static void launch_m2m_t(const FmmParams& F, uint32_t count, uint32_t off, uint32_t items, hipStream_t st) {
+    hipLaunchKernelGGL((m2m_kernel<P>), dim3(ceil_div(items, 64)), dim3(64), 0, st, F, count, off);
 }
 template <int P>
-static void launch_l2l_t(const FmmParams& F, uint32_t count, uint32_t off, hipStream_t st) {
-    hipLaunchKernelGGL((l2l_kernel<P>), dim3(ceil_div(count, 64)), dim3(64), 0, st, F, count, off);
+static void launch_l2l_t(const FmmParams& F, uint32_t count, uint32_t off, uint32_t items, hipStream_t st) {
+    hipLaunchKernelGGL((l2l_kernel<P>), dim3(ceil_div(items, 64)), dim3(64), 0, st, F, count, off);
 }
 template <int P>
 static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
@@ -925,8 +988,9 @@ static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves
 }
 
 template <int P, int SPL>
-static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
-    const unsigned grid = std::min<unsigned>(ceil_div(num_leaves, P2P_WAVES), 65536u);
+static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num_leaves, uint32_t items,
+                                hipStream_t st) {
+    const unsigned grid = std::min<unsigned>(ceil_div(items, P2P_WAVES), 65536u);
     if (F.eps2 > 0.f)
         hipLaunchKernelGGL((p2p_staged_kernel<P, SPL>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S,
                            num_leaves);
@@ -945,28 +1009,35 @@ static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num
         default: break;                  \
     }
 
-void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
+void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st, uint32_t grid_items) {
     if (!num_leaves) return;
-    NBH_DISPATCH_P(p, launch_p2m_t<PP>(F, S, num_leaves, st));
+    const uint32_t items = grid_items_of(num_leaves, grid_items);
+    NBH_DISPATCH_P(p, launch_p2m_t<PP>(F, S, num_leaves, items, st));
 }
 void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st) {
     if (!n) return;
     NBH_DISPATCH_P(p, launch_l2p_t<PP>(F, S, n, kick_dt, st));
 }
-void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st) {
-    (void)level;
+void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st,
+                      uint32_t grid_items) {
     if (!count) return;
-    NBH_DISPATCH_P(p, launch_m2m_t<PP>(F, count, off, st));
+    const uint32_t items = grid_items_of(count, grid_items);
+    if (count == DEV_COUNT) off = (uint32_t)level;
+    NBH_DISPATCH_P(p, launch_m2m_t<PP>(F, count, off, items, st));
 }
-void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st) {
-    (void)level;
+void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st,
+                      uint32_t grid_items) {
     if (!count) return;
-    NBH_DISPATCH_P(p, launch_l2l_t<PP>(F, count, off, st));
+    const uint32_t items = grid_items_of(count, grid_items);
+    if (count == DEV_COUNT) off = (uint32_t)level;
+    NBH_DISPATCH_P(p, launch_l2l_t<PP>(F, count, off, items, st));
 }
-void launch_p2p_staged(int p, int chunk, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
+void launch_p2p_staged(int p, int chunk, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st,
+                       uint32_t grid_items) {
     if (!num_leaves) return;
-    if (chunk == 128) NBH_DISPATCH_P(p, (launch_p2p_staged_t<PP, 2>(F, S, num_leaves, st)))
-    else NBH_DISPATCH_P(p, (launch_p2p_staged_t<PP, 1>(F, S, num_leaves, st)))
+    const uint32_t items = grid_items_of(num_leaves, grid_items);
+    if (chunk == 128) NBH_DISPATCH_P(p, (launch_p2p_staged_t<PP, 2>(F, S, num_leaves, items, st)))
+    else NBH_DISPATCH_P(p, (launch_p2p_staged_t<PP, 1>(F, S, num_leaves, items, st)))
 }
 void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
                    int nsys, hipStream_t st) {

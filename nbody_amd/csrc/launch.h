@@ -138,15 +138,21 @@ struct FmmParams {
 };
 
 void launch_gather_state(const State& src, const State& dst, const uint32_t* perm, uint32_t n, hipStream_t st);
-void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
-void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
-void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st);
+// count (num_leaves) == DEV_COUNT: read on the device, the grid sized for
+// grid_items; the per-level launchers then read the level's count and offset
+void launch_p2m(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st,
+                uint32_t grid_items = 0);
+void launch_m2m_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st,
+                      uint32_t grid_items = 0);
+void launch_l2l_level(int p, const FmmParams& F, int level, uint32_t count, uint32_t off, hipStream_t st,
+                      uint32_t grid_items = 0);
 // the P2P kernels leave the raw near-field sums of every particle in (ax, ay,
 // az, pot); launch_l2p adds the far field of the leaf's local expansion,
 // applies G q/m and the sign of the potential, and the kick v += a kick_dt
 void launch_p2p(int p, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
 // chunk: sources staged per wavefront round, 64 or 128 (the tile shape)
-void launch_p2p_staged(int p, int chunk, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st);
+void launch_p2p_staged(int p, int chunk, const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st,
+                       uint32_t grid_items = 0);
 void launch_l2p(int p, const FmmParams& F, const State& S, uint32_t n, double kick_dt, hipStream_t st);
 // M2L of order p with the truncation code q (expansion.h: m2l_deg(q) = p or
 // p + 1; q | M2L_ALL keeps every term of degree <= p + 1, q = p + 1 only
@@ -188,6 +194,9 @@ struct M2LPlan {
     uint32_t level_count[MAX_LEVELS + 1];
     size_t max_lds;  // LDS bytes per workgroup: the same-level table is held in LDS if it fits
     int nsys;        // systems: with more than one, each target reads its own system's table
+    // counts read on the device (a captured step): num_nodes and level_count
+    // then only size the grids, and level_off is not used
+    bool dev;
 };
 // tensors at R = -(a + b e) h_l / 2^shift for e in [0, E)^3, levels
 // level0 .. level0+nlev-1 of systems 0 .. nsys-1 (h_l from each system's

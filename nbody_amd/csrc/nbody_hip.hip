@@ -74,6 +74,23 @@ struct StateBufs {
     }
 };
 
+// Launch shapes of a captured step, taken from the last uncaptured step with
+// headroom: the grids and the level range are fixed in the graph, the
+// buffers sized so that their pointers stay valid across replays, and the
+// kernels read the actual counts on the device.  A step that outgrows them
+// flags an overflow, is rolled back and run uncaptured.
+struct GraphShape {
+    uint32_t node_cap = 0;       // node arrays
+    uint32_t node_items = 0;     // grid sizing: nodes, leaves
+    uint32_t leaf_items = 0;
+    uint32_t depth_cap = 0;      // levels 0..depth_cap are processed
+    uint32_t level_items[MAX_LEVELS + 1] = {};
+    uint32_t walk_grid[MAX_LEVELS + 1] = {};
+    uint32_t far_cap = 0;
+    unsigned long long m2l_cap = 0, p2p_cap = 0;
+    uint32_t sl_cap = 0, row_cap = 0;
+};
+
 }  // namespace
 
 struct nbh_handle {
@@ -141,6 +158,13 @@ struct nbh_handle {
     // timing: each level's same-level M2L launch, bracketed on its own stream
     hipEvent_t ev_lvl[2 * (MAX_LEVELS + 1)] = {};
     bool lvl_timed[MAX_LEVELS + 1] = {};
+    // captured steps (cfg.graph): one executable graph per state buffer
+    // parity (the tree build alternates the two state buffers), the shapes
+    // they were captured with, and the state before the running step
+    GraphShape gshape;
+    bool gshape_valid = false;
+    hipGraphExec_t gexec[2] = {};
+    DevBuf gsnap;
 
     int fail(int code, const char* fmt, ...) {
         char buf[512];
@@ -187,6 +211,13 @@ bool theta_rational(double theta, int* num, int* den) {
 __global__ void zero_u32_kernel(uint32_t* p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+// A captured step processes levels 0..depth_cap: a deeper tree is flagged
+// like one that outgrew its node arrays, so that every later kernel of the
+// step stands down (those levels have no rows) and the host reruns it.
+__global__ void graph_guard_kernel(Counters* C, uint32_t depth_cap) {
+    if (threadIdx.x == 0 && C->depth > depth_cap) atomicOr(&C->overflow, (uint32_t)OVF_NODES);
 }
 
 void zero_u32(void* p, size_t n, hipStream_t st) {
@@ -304,7 +335,9 @@ static int sort_particles(nbh_handle* h) {
     return NBH_OK;
 }
 
-static int build_tree(nbh_handle* h) {
+// gs: a captured step (no read-back; the node arrays were sized beforehand
+// and the emission flags a tree that outgrows them)
+static int build_tree(nbh_handle* h, const GraphShape* gs = nullptr) {
     const uint32_t n = (uint32_t)h->n;
     Counters* dC = h->counters.as<Counters>();
     zero_u32(dC, sizeof(Counters) / 4, h->stream);
@@ -313,6 +346,16 @@ static int build_tree(nbh_handle* h) {
     launch_tree_build(h->keys.as<uint64_t>(), n, h->sys_off.as<uint32_t>(), h->nsys, h->bits,
                       (int)h->cfg.leaf_capacity, D, node_view(h), h->node_begin.as<uint32_t>(),
                       h->leaf_ids.as<uint32_t>(), h->node_cap, dC, h->tree_tmp.p, h->stream);
+    if (gs) {
+        launch_tree_emit(h->keys.as<uint64_t>(), n, h->sys_off.as<uint32_t>(), h->nsys, h->bits,
+                         (int)h->cfg.leaf_capacity, D, node_view(h), h->node_begin.as<uint32_t>(),
+                         h->leaf_ids.as<uint32_t>(), h->node_cap, dC, h->tree_tmp.p, h->stream);
+        launch_tree_link(h->keys.as<uint64_t>(), n, h->sys_off.as<uint32_t>(), h->nsys, h->bits,
+                         (int)h->cfg.leaf_capacity, D, dC, h->node_cap, h->node_begin.as<uint32_t>(), node_view(h),
+                         h->stream);
+        hipLaunchKernelGGL(graph_guard_kernel, dim3(1), dim3(64), 0, h->stream, dC, gs->depth_cap);
+        return NBH_OK;
+    }
     CHK(read_counters(h));
     if (h->hc->nonfinite) return h->fail(NBH_E_NONFINITE, "non-finite particle position");
     if (h->hc->outside)
@@ -396,34 +439,39 @@ static void copy_tables(FmmParams& F, const FmmParams& T) {
     F.xnlev = T.xnlev;
 }
 
-static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
+static M2LPlan graph_plan(const nbh_handle* h, const GraphShape& gs);
+
+// gs: a captured step: one attempt with the grids and buffers of gs, no
+// read-back (an overflow is flagged on the device and found after the step)
+static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr, const GraphShape* gs = nullptr) {
     Counters* dC = h->counters.as<Counters>();
     const uint32_t nn = h->num_nodes;
-    if (h->row_cap < 2 * nn + 1024) CHK(grow_rows(h, 2 * nn + 1024));
+    const uint32_t depth = gs ? gs->depth_cap : h->depth;
+    if (!gs && h->row_cap < 2 * nn + 1024) CHK(grow_rows(h, 2 * nn + 1024));
     // group-kernel grid per depth (one workgroup per group at a time): at
     // most what stays resident, so that the per-workgroup pool chunks waste
     // little; every workgroup may leave one unused chunk tail per depth
     uint32_t grid[MAX_LEVELS + 1] = {};
     unsigned long long blocks = 0;
-    for (uint32_t s = 1; s <= h->depth; ++s) {
-        const uint32_t groups = h->hc->level_count[s - 1];
-        grid[s] = std::max<uint32_t>(std::min<uint32_t>(groups + 16, h->walk_grid), 64);
+    for (uint32_t s = 1; s <= depth; ++s) {
+        const uint32_t groups = gs ? 0 : h->hc->level_count[s - 1];
+        grid[s] = gs ? gs->walk_grid[s] : std::max<uint32_t>(std::min<uint32_t>(groups + 16, h->walk_grid), 64);
         blocks += std::min<unsigned long long>(grid[s], nn);
     }
     const unsigned long long waste_m = blocks * POOL_CHUNK_M, waste_p = blocks * POOL_CHUNK_P;
     const unsigned long long waste_s = blocks * POOL_CHUNK_S;
     // pools: sized from the last evaluation (or a per-node guess) and grown
-    // on overflow
-    if (h->sl_cap < 64ull * nn + waste_s) {
+    // on overflow (a captured step's were sized beforehand)
+    if (!gs && h->sl_cap < 64ull * nn + waste_s) {
         h->sl_cap = (uint32_t)std::min<unsigned long long>(64ull * nn + waste_s, 0xffff0000ull);
         HIPCHK(h, h->sl.ensure((size_t)h->sl_cap * 4));
     }
     const unsigned long long mguess = 256ull * nn + waste_m, pguess = 64ull * nn + waste_p;
-    if (h->m2l_pool_cap < mguess) {
+    if (!gs && h->m2l_pool_cap < mguess) {
         h->m2l_pool_cap = std::min<unsigned long long>(mguess, 0xffff0000ull);
         HIPCHK(h, h->m2l_pool.ensure((size_t)h->m2l_pool_cap * 4));
     }
-    if (h->p2p_pool_cap < pguess) {
+    if (!gs && h->p2p_pool_cap < pguess) {
         h->p2p_pool_cap = std::min<unsigned long long>(pguess, 0xffff0000ull);
         HIPCHK(h, h->p2p_pool.ensure((size_t)h->p2p_pool_cap * 4));
     }
@@ -458,16 +506,16 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
         if (early) {
             Fe = fmm_params(h);  // this attempt's rows and pools
             copy_tables(Fe, em->T);
-            plan = m2l_plan(h);
+            plan = gs ? graph_plan(h, *gs) : m2l_plan(h);
             HIPCHK(h, hipStreamWaitEvent(h->stream3, h->ev_side, 0));  // moments and tables
             HIPCHK(h, hipStreamWaitEvent(h->stream4, h->ev_side, 0));
         }
         // reset the walk part of the counters
         zero_u32(&dC->carried[0], (offsetof(Counters, nonfinite) - offsetof(Counters, carried)) / 4, h->stream);
         launch_walk_root(W, h->stream);
-        for (uint32_t s = 1; s <= h->depth; ++s) {
+        for (uint32_t s = 1; s <= depth; ++s) {
             launch_walk_group(W, (int)s, grid[s], h->stream);
-            if (early && (int)s >= Fe.dlevel0 && (int)s < Fe.dlevel0 + Fe.dnlev && h->hc->level_count[s]) {
+            if (early && (int)s >= Fe.dlevel0 && (int)s < Fe.dlevel0 + Fe.dnlev && (gs || h->hc->level_count[s])) {
                 hipStream_t sl = (s & 1) ? h->stream4 : h->stream3;
                 HIPCHK(h, hipEventRecord(h->ev_depth[s], h->stream));
                 HIPCHK(h, hipStreamWaitEvent(sl, h->ev_depth[s], 0));
@@ -481,6 +529,7 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr) {
                 }
             }
         }
+        if (gs) return NBH_OK;
         CHK(read_counters(h));
         const uint32_t ovf = h->hc->overflow;
         const uint32_t walk_bits = OVF_SL | OVF_ROWS | OVF_M2L | OVF_P2P;
@@ -570,23 +619,38 @@ static M2LPlan m2l_plan(const nbh_handle* h) {
 // One force evaluation; kick_dt != 0 applies v += a * kick_dt afterwards.
 // for_vjp: only the tree, lists and mass expansions are needed (the adjoint
 // kernels follow on the same stream): no near field, no closing read-back.
-static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
+static void graph_invalidate(nbh_handle* h);
+static void graph_shape_update(nbh_handle* h);
+
+// gs: a captured step (graph mode): the launches of the evaluation with the
+// counts read on the device, the grids, levels and buffers of gs, and no
+// read-back; the host reads the counters after the graph.
+static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false, const GraphShape* gs = nullptr) {
     const int p = (int)h->cfg.order;
+    // an evaluation outside a graph may grow (move) the buffers a graph holds
+    if (!gs) graph_invalidate(h);
     record(h, 0);
-    CHK(build_tree(h));
+    CHK(build_tree(h, gs));
     record(h, 1);
     FmmParams F = fmm_params(h);
     State S = h->st[h->cur].view();
+    const uint32_t depth = gs ? gs->depth_cap : h->depth;  // levels 0..depth
     // side stream, overlapping the walk: moments (P2M, M2M) and the tables
     HIPCHK(h, hipEventRecord(h->ev_fork, h->stream));
     HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
     record_side(h, 0);
-    launch_p2m(p, F, S, h->num_leaves, h->stream2);
-    for (int l = (int)h->depth - 1; l >= 0; --l)
-        launch_m2m_level(p, F, l, h->hc->level_count[l], h->hc->level_off[l], h->stream2);
+    if (gs) {
+        launch_p2m(p, F, S, DEV_COUNT, h->stream2, gs->leaf_items);
+        for (int l = (int)depth - 1; l >= 0; --l)
+            launch_m2m_level(p, F, l, DEV_COUNT, 0, h->stream2, gs->level_items[l]);
+    } else {
+        launch_p2m(p, F, S, h->num_leaves, h->stream2);
+        for (int l = (int)depth - 1; l >= 0; --l)
+            launch_m2m_level(p, F, l, h->hc->level_count[l], h->hc->level_off[l], h->stream2);
+    }
     record_side(h, 1);
     const int q = m2l_q(h);
-    const bool tables = h->cfg.m2l != NBH_M2L_PLAIN && h->depth >= 2;
+    const bool tables = h->cfg.m2l != NBH_M2L_PLAIN && depth >= 2;
     FmmParams T{};  // the table part of the parameters
     if (tables) {
         // derivative tensors of the same-level offsets [0, K]^3 of levels
@@ -595,7 +659,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
         const int K = 2 * Kp + 1;
         const int dsz = (K + 1) * (K + 1) * (K + 1), dcp = ncp_of(m2l_deg(q));
-        const int nlev = (int)h->depth - 1;
+        const int nlev = (int)depth - 1;
         HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4 * h->nsys));
         T = F;
         T.dtab = h->dtab.as<float>();
@@ -609,7 +673,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         // 4 Kp + 1 half finer cells (the coarser side's parent pair was
         // split), tabled at d_i = 1, 3, .., 4 Kp + 1
         const int E = 2 * Kp + 1;
-        const int xn = (int)h->depth;
+        const int xn = (int)depth;
         HIPCHK(h, h->xtab.ensure((size_t)E * E * E * xn * dcp * 4 * h->nsys));
         T.xtab = h->xtab.as<float>();
         T.xE = E;
@@ -633,14 +697,15 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     em.p = p;
     em.q = q;
     em.T = T;
-    CHK(run_walk(h, &em));
+    CHK(run_walk(h, &em, gs));
     record(h, 2);
     F = fmm_params(h);
     if (tables) {
         copy_tables(F, T);
         F.Lx = h->Lx.as<float>();
-        // deferred pairs: at most every other-level entry
-        const unsigned long long nx = std::max<unsigned long long>(h->hc->m2lx_total, 1);
+        // deferred pairs: at most every other-level entry (a captured step:
+        // sized from the previous step, a shortfall flagged)
+        const unsigned long long nx = gs ? gs->far_cap : std::max<unsigned long long>(h->hc->m2lx_total, 1);
         HIPCHK(h, h->far_src.ensure((size_t)nx * 4));
         HIPCHK(h, h->far_tgt.ensure((size_t)nx * 4));
         F.far_src = h->far_src.as<int>();
@@ -653,7 +718,8 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
         HIPCHK(h, hipEventRecord(h->ev_walk, h->stream));
         HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_walk, 0));
         record_side(h, 2);
-        if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_staged(p, p2p_chunk(h), F, S, h->num_leaves, h->stream2);
+        if (gs) launch_p2p_staged(p, p2p_chunk(h), F, S, DEV_COUNT, h->stream2, gs->leaf_items);
+        else if (h->cfg.p2p == NBH_P2P_LANE) launch_p2p_staged(p, p2p_chunk(h), F, S, h->num_leaves, h->stream2);
         else launch_p2p(p, F, S, h->num_leaves, h->stream2);
         record_side(h, 3);
         HIPCHK(h, hipEventRecord(h->ev_near, h->stream2));
@@ -662,17 +728,19 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     if (em.on) {
         // the same-level kernels run beside the walk; the other-level pairs
         // (into Lx) beside them; L2L waits for both
-        launch_m2l_parts(p, q, F, m2l_plan(h), M2L_REST, -1, h->stream);
+        launch_m2l_parts(p, q, F, gs ? graph_plan(h, *gs) : m2l_plan(h), M2L_REST, -1, h->stream);
         HIPCHK(h, hipEventRecord(h->ev_same, h->stream3));
         HIPCHK(h, hipEventRecord(h->ev_same4, h->stream4));
         HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_same, 0));
         HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_same4, 0));
     } else {
-        launch_m2l(p, q, F, m2l_plan(h), h->stream);
+        launch_m2l(p, q, F, gs ? graph_plan(h, *gs) : m2l_plan(h), h->stream);
     }
     record(h, 3);
-    for (uint32_t l = 1; l <= h->depth; ++l)
-        launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
+    for (uint32_t l = 1; l <= depth; ++l) {
+        if (gs) launch_l2l_level(p, F, (int)l, DEV_COUNT, 0, h->stream, gs->level_items[l]);
+        else launch_l2l_level(p, F, (int)l, h->hc->level_count[l], h->hc->level_off[l], h->stream);
+    }
     record(h, 4);
     h->lastF = F;
     if (for_vjp) {
@@ -686,6 +754,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     launch_l2p(p, F, S, (uint32_t)h->n, kick_dt, h->stream);
     record(h, 6);
     HIPCHK(h, hipGetLastError());
+    if (gs) return NBH_OK;
     // stats (read at the end of the evaluation; one more sync)
     CHK(read_counters(h));
     if (h->hc->overflow) return h->fail(NBH_E_INTERNAL, "overflow flags %u", h->hc->overflow);
@@ -697,6 +766,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false) {
     h->stats.p2p_interactions = h->hc->p2p_inter;
     h->stats.split_pairs = h->hc->split_total;
     h->stats.m2l_same_pairs = h->hc->m2l_total - h->hc->m2lx_total;
+    graph_shape_update(h);
     h->stats.ms_m2l_same = 0.0;
     h->stats.m2l_same_launches = 0;
     for (uint32_t l = 0; l <= (uint32_t)MAX_LEVELS; ++l) {
@@ -746,6 +816,177 @@ static int force_direct(nbh_handle* h, double kick_dt) {
     return NBH_OK;
 }
 
+// ---- captured steps ---------------------------------------------------------
+static size_t snapshot_bytes(size_t n);
+static int snapshot_save(nbh_handle* h, DevBuf& dst);
+static int snapshot_restore(nbh_handle* h, const DevBuf& src, double t);
+
+static M2LPlan graph_plan(const nbh_handle* h, const GraphShape& gs) {
+    M2LPlan pl{};
+    pl.mode = 1;
+    pl.mfma = 0;
+    pl.num_nodes = gs.node_items;
+    pl.max_lds = (size_t)h->max_lds;
+    pl.nsys = h->nsys;
+    pl.dev = true;
+    for (uint32_t l = 0; l <= gs.depth_cap; ++l) pl.level_count[l] = gs.level_items[l];
+    return pl;
+}
+
+static void graph_invalidate(nbh_handle* h) {
+    for (auto& g : h->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// the launch shapes of a step like the one just evaluated, with headroom
+static void graph_shape_update(nbh_handle* h) {
+    const Counters& c = *h->hc;
+    GraphShape g;
+    const uint32_t nn = c.num_nodes, nl = c.num_leaves;
+    g.node_cap = nn + nn / 4 + 1024;
+    g.node_items = nn + nn / 8 + 64;
+    g.leaf_items = nl + nl / 8 + 64;
+    g.depth_cap = std::min<uint32_t>(h->cfg.max_depth, c.depth + 2);
+    unsigned long long blocks = 0;
+    for (uint32_t l = 0; l <= g.depth_cap; ++l) {
+        const uint32_t lc = l <= c.depth ? c.level_count[l] : 0;
+        g.level_items[l] = lc + lc / 4 + 64;
+        if (l >= 1) {
+            const uint32_t groups = g.level_items[l - 1] + (l - 1 <= c.depth ? c.carried[l - 1] : 0);
+            g.walk_grid[l] = std::max<uint32_t>(std::min<uint32_t>(groups + 16, h->walk_grid), 64);
+            blocks += g.walk_grid[l];
+        }
+    }
+    // every workgroup may leave one unused chunk tail per depth
+    const unsigned long long cap32 = 0xffff0000ull;
+    g.m2l_cap = std::min(std::max<unsigned long long>(h->m2l_pool_cap,
+                                                      c.m2l_pool_used + c.m2l_pool_used / 4 + blocks * POOL_CHUNK_M),
+                         cap32);
+    g.p2p_cap = std::min(std::max<unsigned long long>(h->p2p_pool_cap,
+                                                      c.p2p_pool_used + c.p2p_pool_used / 4 + blocks * POOL_CHUNK_P),
+                         cap32);
+    g.sl_cap = (uint32_t)std::min(std::max<unsigned long long>(h->sl_cap, c.sl_used + c.sl_used / 4 +
+                                                                              blocks * POOL_CHUNK_S),
+                                  cap32);
+    g.row_cap = std::max<uint32_t>(h->row_cap, 2 * g.node_cap + 1024);
+    g.far_cap = (uint32_t)std::min<unsigned long long>(c.m2lx_total + c.m2lx_total / 4 + 65536, 0xffffffffull);
+    h->gshape = g;
+    h->gshape_valid = true;
+}
+
+// captured steps apply to the KDK FMM step with the tabled VALU M2L and the
+// per-lane P2P, timing off
+static bool graph_ok(const nbh_handle* h) {
+    return h->cfg.graph && h->cfg.solver == NBH_SOLVER_FMM && h->cfg.integrator == NBH_INT_KDK && h->acc_valid &&
+           !h->timing && h->cfg.m2l == NBH_M2L_GROUPED && h->cfg.p2p == NBH_P2P_LANE && h->gshape_valid &&
+           h->gshape.depth_cap >= 2 && !use_offset_m2l(h);
+}
+
+// the buffers of the captured shape, allocated before the capture so that no
+// allocation happens inside it and the graph's pointers stay valid
+static int graph_prepare(nbh_handle* h) {
+    const GraphShape& g = h->gshape;
+    CHK(alloc_nodes(h, g.node_cap));
+    if (h->row_cap < g.row_cap) CHK(grow_rows(h, g.row_cap));
+    if (h->sl_cap < g.sl_cap) {
+        h->sl_cap = g.sl_cap;
+        HIPCHK(h, h->sl.ensure((size_t)h->sl_cap * 4));
+    }
+    if (h->m2l_pool_cap < g.m2l_cap) {
+        h->m2l_pool_cap = g.m2l_cap;
+        HIPCHK(h, h->m2l_pool.ensure((size_t)h->m2l_pool_cap * 4));
+    }
+    if (h->p2p_pool_cap < g.p2p_cap) {
+        h->p2p_pool_cap = g.p2p_cap;
+        HIPCHK(h, h->p2p_pool.ensure((size_t)h->p2p_pool_cap * 4));
+    }
+    // the tables (as force_fmm sizes them) and the deferred pairs
+    const int q = m2l_q(h);
+    const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
+    const int K = 2 * Kp + 1, E = 2 * Kp + 1;
+    const size_t dsz = (size_t)(K + 1) * (K + 1) * (K + 1), dcp = (size_t)ncp_of(m2l_deg(q));
+    HIPCHK(h, h->dtab.ensure(dsz * (g.depth_cap - 1) * dcp * 4 * h->nsys));
+    HIPCHK(h, h->xtab.ensure((size_t)E * E * E * g.depth_cap * dcp * 4 * h->nsys));
+    HIPCHK(h, h->far_src.ensure((size_t)g.far_cap * 4));
+    HIPCHK(h, h->far_tgt.ensure((size_t)g.far_cap * 4));
+    HIPCHK(h, h->gsnap.ensure(snapshot_bytes(h->n)));
+    return NBH_OK;
+}
+
+// Records one KDK step (state snapshot, opening kick and drift, the force
+// evaluation with its closing kick) for the current state buffer into
+// gexec[h->cur].  Nothing runs; the host state is left as it was.
+static int graph_capture(nbh_handle* h) {
+    CHK(graph_prepare(h));
+    const int cur0 = h->cur;
+    const double dt = h->cfg.dt;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = snapshot_save(h, h->gsnap);
+    if (rc == NBH_OK) {
+        launch_integrate(h->st[h->cur].view(), (uint32_t)h->n, 0, 0.5 * dt, dt, h->stream);
+        rc = force_fmm(h, 0.5 * dt, false, &h->gshape);
+    }
+    h->cur = cur0;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (rc != NBH_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != NBH_OK ? rc : h->fail(NBH_E_HIP, "step capture failed: %s", hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&h->gexec[cur0], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        h->gexec[cur0] = nullptr;
+        return h->fail(NBH_E_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    ++h->stats.graph_captures;
+    return NBH_OK;
+}
+
+// One KDK step from the captured graph of this state buffer (captured first
+// if needed): 1 when done, 0 when the step must run uncaptured (not
+// applicable, or it outgrew the captured shapes: then it was rolled back),
+// < 0 on error.
+static int graph_step(nbh_handle* h) {
+    if (!graph_ok(h)) return 0;
+    if (!h->gexec[h->cur]) CHK(graph_capture(h));
+    const int cur0 = h->cur;
+    HIPCHK(h, hipGraphLaunch(h->gexec[cur0], h->stream));
+    CHK(read_counters(h));
+    const Counters& c = *h->hc;
+    if (c.nonfinite) return h->fail(NBH_E_NONFINITE, "non-finite particle position");
+    if (c.outside) return h->fail(NBH_E_OUTSIDE, "a particle lies outside the fixed root box");
+    if ((c.overflow & (OVF_WALK | OVF_FAR)) || c.depth > h->gshape.depth_cap) {
+        // outgrew the captured shapes: back to the state before the step,
+        // which the caller runs uncaptured (and the next step captures anew)
+        h->cur = cur0;
+        CHK(snapshot_restore(h, h->gsnap, h->time));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        graph_invalidate(h);
+        h->gshape_valid = false;
+        ++h->stats.graph_fallbacks;
+        return 0;
+    }
+    if (c.overflow) return h->fail(NBH_E_INTERNAL, "overflow flags %u", c.overflow);
+    h->cur = 1 - cur0;
+    h->num_nodes = c.num_nodes;
+    h->num_leaves = c.num_leaves;
+    h->depth = c.depth;
+    h->stats.num_nodes = c.num_nodes;
+    h->stats.num_leaves = c.num_leaves;
+    h->stats.depth = c.depth;
+    h->stats.m2l_pairs = c.m2l_total;
+    h->stats.p2p_pairs = c.p2p_total;
+    h->stats.p2p_interactions = c.p2p_inter;
+    h->stats.split_pairs = c.split_total;
+    h->stats.m2l_same_pairs = c.m2l_total - c.m2lx_total;
+    ++h->stats.graph_replays;
+    return 1;
+}
+
 static int force_eval(nbh_handle* h, double kick_dt) {
     return h->cfg.solver == NBH_SOLVER_FMM ? force_fmm(h, kick_dt) : force_direct(h, kick_dt);
 }
@@ -754,11 +995,17 @@ static int do_step(nbh_handle* h) {
     const double dt = h->cfg.dt;
     const uint32_t n = (uint32_t)h->n;
     switch (h->cfg.integrator) {
-        case NBH_INT_KDK:
+        case NBH_INT_KDK: {
             if (!h->acc_valid) { CHK(force_eval(h, 0.0)); h->acc_valid = true; }
+            if (h->cfg.graph) {
+                const int r = graph_step(h);
+                if (r < 0) return r;
+                if (r == 1) break;
+            }
             launch_integrate(h->st[h->cur].view(), n, 0, 0.5 * dt, dt, h->stream);
             CHK(force_eval(h, 0.5 * dt));
             break;
+        }
         case NBH_INT_EULER:
             CHK(force_eval(h, 0.0));
             launch_integrate(h->st[h->cur].view(), n, 1, dt, dt, h->stream);
@@ -1028,6 +1275,8 @@ void nbh_destroy(nbh_handle* h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    graph_invalidate(h);
+    h->gsnap.release();
     h->st[0].release();
     h->st[1].release();
     DevBuf* all[] = {&h->sys_off, &h->keys, &h->keys_alt, &h->perm, &h->perm_alt, &h->sort_tmp, &h->tree_tmp,

@@ -308,7 +308,8 @@ __global__ __launch_bounds__(TB_THREADS) void tree_emit_kernel(TreeParams P, con
 // begin lies in [begin, end)).
 __global__ void tree_link_kernel(TreeParams P, const Counters* C, const uint32_t* node_begin, NodeView nodes) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= C->num_nodes) return;
+    // past the node arrays (a captured step sized them from the previous step)
+    if (id >= C->num_nodes || (C->overflow & OVF_NODES)) return;
     int4 g = nodes.geo[id];
     const int l = g.w & 0xff;
     const bool leaf = (g.w >> 8) == 0;

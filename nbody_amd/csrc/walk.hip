@@ -204,6 +204,7 @@ __device__ __forceinline__ uint32_t pool_take(uint2& ch, uint32_t n, uint32_t ch
 // split.  One thread per root.
 __global__ __launch_bounds__(64) void walk_root_kernel(WalkParams W) {
     Counters* C = W.C;
+    if (C->overflow & OVF_NODES) return;  // a captured step's tree outgrew its node arrays
     const uint32_t n_roots = C->level_count[0];
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_roots; r += gridDim.x * blockDim.x) {
         if (r >= W.row_cap) {
@@ -273,6 +274,7 @@ __global__ __launch_bounds__(WAVE * WALK_WAVES) void walk_group_kernel(WalkParam
     // pass 1's categories of a cached group, 3 bits per target, per call and lane
     __shared__ uint32_t cat_s[WALK_WAVES][CAT_CALLS][WAVE];
     Counters* C = W.C;
+    if (C->overflow & OVF_NODES) return;  // a captured step's tree outgrew its node arrays
     const uint32_t n_prev_level = C->level_count[s - 1];
     const uint32_t n_groups = n_prev_level + C->carried[s - 1];
     const uint32_t base_prev = depth_base(C, s - 1);
