@@ -817,10 +817,6 @@ static int force_direct(nbh_handle* h, double kick_dt) {
 }
 
 // ---- captured steps ---------------------------------------------------------
-static size_t snapshot_bytes(size_t n);
-static int snapshot_save(nbh_handle* h, DevBuf& dst);
-static int snapshot_restore(nbh_handle* h, const DevBuf& src, double t);
-
 static M2LPlan graph_plan(const nbh_handle* h, const GraphShape& gs) {
     M2LPlan pl{};
     pl.mode = 1;
@@ -912,7 +908,24 @@ static int graph_prepare(nbh_handle* h) {
     HIPCHK(h, h->xtab.ensure((size_t)E * E * E * g.depth_cap * dcp * 4 * h->nsys));
     HIPCHK(h, h->far_src.ensure((size_t)g.far_cap * 4));
     HIPCHK(h, h->far_tgt.ensure((size_t)g.far_cap * 4));
-    HIPCHK(h, h->gsnap.ensure(snapshot_bytes(h->n)));
+    HIPCHK(h, h->gsnap.ensure(h->n * 48));
+    return NBH_OK;
+}
+
+// Positions and velocities of the current state buffer to (save) or from the
+// graph's snapshot: a step changes only those in that buffer (the kick and
+// drift; the tree build gathers into the other buffer, where the forces and
+// the closing kick go), so they are all a rollback needs.
+static int graph_snapshot(nbh_handle* h, bool save) {
+    const size_t n = h->n;
+    StateBufs& sb = h->st[h->cur];
+    DevBuf* d[] = {&sb.x, &sb.y, &sb.z, &sb.vx, &sb.vy, &sb.vz};
+    char* o = (char*)h->gsnap.p;
+    for (DevBuf* b : d) {
+        HIPCHK(h, save ? hipMemcpyAsync(o, b->p, n * 8, hipMemcpyDeviceToDevice, h->stream)
+                       : hipMemcpyAsync(b->p, o, n * 8, hipMemcpyDeviceToDevice, h->stream));
+        o += n * 8;
+    }
     return NBH_OK;
 }
 
@@ -924,7 +937,7 @@ static int graph_capture(nbh_handle* h) {
     const int cur0 = h->cur;
     const double dt = h->cfg.dt;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int rc = snapshot_save(h, h->gsnap);
+    int rc = graph_snapshot(h, true);
     if (rc == NBH_OK) {
         launch_integrate(h->st[h->cur].view(), (uint32_t)h->n, 0, 0.5 * dt, dt, h->stream);
         rc = force_fmm(h, 0.5 * dt, false, &h->gshape);
@@ -963,7 +976,7 @@ static int graph_step(nbh_handle* h) {
         // outgrew the captured shapes: back to the state before the step,
         // which the caller runs uncaptured (and the next step captures anew)
         h->cur = cur0;
-        CHK(snapshot_restore(h, h->gsnap, h->time));
+        CHK(graph_snapshot(h, false));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         graph_invalidate(h);
         h->gshape_valid = false;
