@@ -46,9 +46,9 @@ namespace nbody {
 struct FmmOptions {
 	int order = 4;           // Taylor order p, 1..8
 	int m2l_order = 0;       // M2L truncation q, p <= q <= 2p (0: p + 1): terms of degree <= q
-	bool m2l_low_terms = false;  // beyond degree p only those with min(|k|, |n|) <= q - p
 	double theta = 0.5;      // opening ratio; must equal num/den with num, den <= 64
 	int leaf_capacity = 32;  // particles per leaf cell
+	bool m2l_low_terms = false;  // beyond degree p only the M2L terms with min(|k|, |n|) <= q - p
 };
 
 namespace fmm_detail {
