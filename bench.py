@@ -449,6 +449,8 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="captured steps (nbh_config.graph = 1) in the timed steps, with the library's timing "
                          "off (phases_ms then comes from the roofline pass alone)")
+    ap.add_argument("--no-phase-timing", action="store_true",
+                    help="the library's per-phase HIP events off in the timed steps (as with --graph)")
     ap.add_argument("--serial", action="store_true",
                     help="run every kernel of an evaluation on one stream in the timed steps too")
     ap.add_argument("--roofline-steps", type=int, default=3,
@@ -515,7 +517,8 @@ def main():
 
     K = args.steps
     tot = {}
-    timed = time_steps(sim, args.warmup, K, serial=args.serial, totals=tot, timing=not args.graph)
+    timed = time_steps(sim, args.warmup, K, serial=args.serial, totals=tot,
+                       timing=not (args.graph or args.no_phase_timing))
     gstats = {k: sim.stats()[k] for k in ("graph_captures", "graph_replays", "graph_fallbacks")}
     # the roofline pass: R more steps with every kernel of an evaluation on
     # one stream, so that each event pair times one kernel running alone (in
