@@ -201,6 +201,58 @@ __global__ __launch_bounds__(RS_THREADS) void radix_scatter_kernel(const uint64_
     }
 }
 
+// Exclusive scan of the digit counts of one pass in a single workgroup (one
+// launch where the three-phase scan takes three, and a dependent launch costs
+// ~10 us of idle GPU between short kernels): thread t owns the consecutive
+// elements [t per, t per + per), per a multiple of 4 (16-byte loads and
+// stores; the buffers are 16-byte aligned), read once for its total and once
+// more to write.  For n <= SCAN1_MAX.
+constexpr uint32_t SCAN1_THREADS = 1024, SCAN1_MAX = 1024u * 256u;
+__global__ __launch_bounds__(SCAN1_THREADS) void scan_counts_kernel(const uint32_t* __restrict__ in,
+                                                                    uint32_t* __restrict__ out, uint32_t n) {
+    const uint32_t per = ((n + SCAN1_THREADS - 1) / SCAN1_THREADS + 3u) & ~3u;
+    const uint32_t b0 = min(threadIdx.x * per, n), b1 = min(b0 + per, n);
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; i += 4) {
+        if (i + 4 <= b1) {
+            const uint4 x = *reinterpret_cast<const uint4*>(in + i);
+            s += x.x + x.y + x.z + x.w;
+        } else {
+            for (uint32_t j = i; j < b1; ++j) s += in[j];
+        }
+    }
+    // exclusive scan of the thread totals: within the wave, then over the 16 waves
+    uint32_t x = s;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, WAVE);
+        if (lane_id() >= o) x += y;
+    }
+    __shared__ uint32_t ws[SCAN1_THREADS / WAVE];
+    const int wave = threadIdx.x / WAVE;
+    if (lane_id() == WAVE - 1) ws[wave] = x;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (int w = 0; w < wave; ++w) wpre += ws[w];
+    uint32_t run = wpre + x - s;
+    for (uint32_t i = b0; i < b1; i += 4) {
+        if (i + 4 <= b1) {
+            const uint4 v = *reinterpret_cast<const uint4*>(in + i);
+            uint4 o;
+            o.x = run; run += v.x;
+            o.y = run; run += v.y;
+            o.z = run; run += v.z;
+            o.w = run; run += v.w;
+            *reinterpret_cast<uint4*>(out + i) = o;
+        } else {
+            for (uint32_t j = i; j < b1; ++j) {
+                out[j] = run;
+                run += in[j];
+            }
+        }
+    }
+}
+
 size_t radix_tmp_bytes(uint32_t n) {
     const uint32_t nb = ceil_div(n ? n : 1, RS_TILE);
     return (size_t)256 * nb * 4 * 2 + scan_tmp_bytes(256 * nb) + 64;
@@ -221,7 +273,10 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(RS_THREADS), 0, st, ka, n, shift, counts, nb);
-        exclusive_scan<uint32_t, uint32_t>(counts, offs, nullptr, 256 * nb, (uint32_t*)nullptr, stmp, st);
+        if (256u * nb <= SCAN1_MAX)
+            hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(SCAN1_THREADS), 0, st, counts, offs, 256u * nb);
+        else
+            exclusive_scan<uint32_t, uint32_t>(counts, offs, nullptr, 256 * nb, (uint32_t*)nullptr, stmp, st);
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(RS_THREADS), 0, st, ka, va, kb, vb, offs, n,
                            shift, nb);
         uint64_t* t = ka; ka = kb; kb = t;
