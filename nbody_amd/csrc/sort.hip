@@ -140,8 +140,11 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ROUNDS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
 
+// Per block: the digit histogram of its tile (digit-major counts), and the
+// digit totals of the pass accumulated in `totals` (cleared beforehand).
 __global__ __launch_bounds__(RS_THREADS) void radix_hist_kernel(const uint64_t* keys, uint32_t n, int shift,
-                                                                uint32_t* counts, uint32_t nblocks) {
+                                                                uint32_t* counts, uint32_t nblocks,
+                                                                uint32_t* totals) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -151,7 +154,71 @@ __global__ __launch_bounds__(RS_THREADS) void radix_hist_kernel(const uint64_t* 
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    counts[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    const uint32_t c = h[threadIdx.x];
+    counts[threadIdx.x * nblocks + blockIdx.x] = c;
+    if (c) atomicAdd(&totals[threadIdx.x], c);
+}
+
+// Exclusive scan of the digit-major counts of one pass in one launch: block
+// j takes the G whole digits [j G, j G + G) (G nblocks counts), starts from
+// the totals of the digits before them and scans its counts in tiles of
+// 16 per thread.  (The three-phase device scan needs three dependent
+// launches, and each costs ~10 us of idle GPU between these short kernels.)
+__global__ __launch_bounds__(RS_THREADS) void radix_offsets_kernel(const uint32_t* counts, uint32_t* offs,
+                                                                   const uint32_t* totals, uint32_t nblocks,
+                                                                   uint32_t G) {
+    constexpr int PER = 16;
+    __shared__ uint32_t ws[RS_THREADS / WAVE];
+    __shared__ uint32_t carry;
+    const int tid = threadIdx.x, wave = tid / WAVE;
+    const uint32_t d0 = blockIdx.x * G;
+    // base: the counts of every digit below d0
+    uint32_t b = (uint32_t)tid < d0 ? totals[tid] : 0u;
+    b = wave_sum(b);
+    if (lane_id() == 0) ws[wave] = b;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < RS_THREADS / WAVE; ++w) t += ws[w];
+        carry = t;
+    }
+    __syncthreads();
+    const uint32_t e0 = d0 * nblocks, e1 = min(d0 + G, 256u) * nblocks;
+    for (uint32_t t0 = e0; t0 < e1; t0 += RS_THREADS * PER) {
+        const uint32_t my = t0 + tid * PER;
+        uint32_t v[PER], s = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            v[k] = my + k < e1 ? counts[my + k] : 0u;
+            s += v[k];
+        }
+        uint32_t x = s;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, WAVE);
+            if (lane_id() >= o) x += y;
+        }
+        __syncthreads();  // ws and carry of the previous tile are read
+        if (lane_id() == WAVE - 1) ws[wave] = x;
+        __syncthreads();
+        uint32_t wpre = 0, tile = 0;
+        for (int w = 0; w < RS_THREADS / WAVE; ++w) {
+            wpre += w < wave ? ws[w] : 0u;
+            tile += ws[w];
+        }
+        uint32_t run = carry + wpre + x - s;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (my + k < e1) offs[my + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (tid == 0) carry += tile;
+    }
+}
+
+__global__ void radix_zero_kernel(uint32_t* p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
 __global__ __launch_bounds__(RS_THREADS) void radix_scatter_kernel(const uint64_t* kin, const uint32_t* vin,
@@ -201,61 +268,9 @@ __global__ __launch_bounds__(RS_THREADS) void radix_scatter_kernel(const uint64_
     }
 }
 
-// Exclusive scan of the digit counts of one pass in a single workgroup (one
-// launch where the three-phase scan takes three, and a dependent launch costs
-// ~10 us of idle GPU between short kernels): thread t owns the consecutive
-// elements [t per, t per + per), per a multiple of 4 (16-byte loads and
-// stores; the buffers are 16-byte aligned), read once for its total and once
-// more to write.  For n <= SCAN1_MAX.
-constexpr uint32_t SCAN1_THREADS = 1024, SCAN1_MAX = 1024u * 256u;
-__global__ __launch_bounds__(SCAN1_THREADS) void scan_counts_kernel(const uint32_t* __restrict__ in,
-                                                                    uint32_t* __restrict__ out, uint32_t n) {
-    const uint32_t per = ((n + SCAN1_THREADS - 1) / SCAN1_THREADS + 3u) & ~3u;
-    const uint32_t b0 = min(threadIdx.x * per, n), b1 = min(b0 + per, n);
-    uint32_t s = 0;
-    for (uint32_t i = b0; i < b1; i += 4) {
-        if (i + 4 <= b1) {
-            const uint4 x = *reinterpret_cast<const uint4*>(in + i);
-            s += x.x + x.y + x.z + x.w;
-        } else {
-            for (uint32_t j = i; j < b1; ++j) s += in[j];
-        }
-    }
-    // exclusive scan of the thread totals: within the wave, then over the 16 waves
-    uint32_t x = s;
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, WAVE);
-        if (lane_id() >= o) x += y;
-    }
-    __shared__ uint32_t ws[SCAN1_THREADS / WAVE];
-    const int wave = threadIdx.x / WAVE;
-    if (lane_id() == WAVE - 1) ws[wave] = x;
-    __syncthreads();
-    uint32_t wpre = 0;
-    for (int w = 0; w < wave; ++w) wpre += ws[w];
-    uint32_t run = wpre + x - s;
-    for (uint32_t i = b0; i < b1; i += 4) {
-        if (i + 4 <= b1) {
-            const uint4 v = *reinterpret_cast<const uint4*>(in + i);
-            uint4 o;
-            o.x = run; run += v.x;
-            o.y = run; run += v.y;
-            o.z = run; run += v.z;
-            o.w = run; run += v.w;
-            *reinterpret_cast<uint4*>(out + i) = o;
-        } else {
-            for (uint32_t j = i; j < b1; ++j) {
-                out[j] = run;
-                run += in[j];
-            }
-        }
-    }
-}
-
 size_t radix_tmp_bytes(uint32_t n) {
     const uint32_t nb = ceil_div(n ? n : 1, RS_TILE);
-    return (size_t)256 * nb * 4 * 2 + scan_tmp_bytes(256 * nb) + 64;
+    return (size_t)256 * nb * 4 * 2 + (size_t)16 * 256 * 4 + 64;  // counts, offsets, digit totals per pass
 }
 
 // Sorts (keys, vals) in place over bits [0, nbits); alt buffers are scratch.
@@ -265,18 +280,20 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
     const uint32_t nb = ceil_div(n, RS_TILE);
     uint32_t* counts = (uint32_t*)tmp;
     uint32_t* offs = counts + (size_t)256 * nb;
-    void* stmp = (void*)(offs + (size_t)256 * nb);
+    uint32_t* totals = offs + (size_t)256 * nb;  // [pass][256]
     int passes = (nbits + 7) / 8;
     if (passes & 1) ++passes;  // even number of passes: result ends in (keys, vals)
+    // digits per offsets block: whole digits, about RS_TILE counts per block
+    uint32_t G = 1;
+    while (G < 256 && G * 2 * nb <= (uint32_t)RS_TILE) G *= 2;
+    hipLaunchKernelGGL(radix_zero_kernel, dim3(1), dim3(256), 0, st, totals, (uint32_t)(passes * 256));
     uint64_t *ka = keys, *kb = keys_alt;
     uint32_t *va = vals, *vb = vals_alt;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(RS_THREADS), 0, st, ka, n, shift, counts, nb);
-        if (256u * nb <= SCAN1_MAX)
-            hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(SCAN1_THREADS), 0, st, counts, offs, 256u * nb);
-        else
-            exclusive_scan<uint32_t, uint32_t>(counts, offs, nullptr, 256 * nb, (uint32_t*)nullptr, stmp, st);
+        uint32_t* tot = totals + 256 * p;
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(RS_THREADS), 0, st, ka, n, shift, counts, nb, tot);
+        hipLaunchKernelGGL(radix_offsets_kernel, dim3(256 / G), dim3(RS_THREADS), 0, st, counts, offs, tot, nb, G);
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(RS_THREADS), 0, st, ka, va, kb, vb, offs, n,
                            shift, nb);
         uint64_t* t = ka; ka = kb; kb = t;
