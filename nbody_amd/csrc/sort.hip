@@ -141,7 +141,9 @@ constexpr int RS_ROUNDS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
 
 // Per block: the digit histogram of its tile (digit-major counts), and the
-// digit totals of the pass accumulated in `totals` (cleared beforehand).
+// digit totals of the pass accumulated in `totals` (cleared beforehand), in
+// RS_TOT_COPIES copies taken by block index to spread the atomics.
+constexpr int RS_TOT_COPIES = 8;
 __global__ __launch_bounds__(RS_THREADS) void radix_hist_kernel(const uint64_t* keys, uint32_t n, int shift,
                                                                 uint32_t* counts, uint32_t nblocks,
                                                                 uint32_t* totals) {
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(RS_THREADS) void radix_hist_kernel(const uint64_t* 
     __syncthreads();
     const uint32_t c = h[threadIdx.x];
     counts[threadIdx.x * nblocks + blockIdx.x] = c;
-    if (c) atomicAdd(&totals[threadIdx.x], c);
+    if (c) atomicAdd(&totals[(blockIdx.x % RS_TOT_COPIES) * 256 + threadIdx.x], c);
 }
 
 // Exclusive scan of the digit-major counts of one pass in one launch: block
@@ -173,7 +175,9 @@ __global__ __launch_bounds__(RS_THREADS) void radix_offsets_kernel(const uint32_
     const int tid = threadIdx.x, wave = tid / WAVE;
     const uint32_t d0 = blockIdx.x * G;
     // base: the counts of every digit below d0
-    uint32_t b = (uint32_t)tid < d0 ? totals[tid] : 0u;
+    uint32_t b = 0;
+    if ((uint32_t)tid < d0)
+        for (int c = 0; c < RS_TOT_COPIES; ++c) b += totals[c * 256 + tid];
     b = wave_sum(b);
     if (lane_id() == 0) ws[wave] = b;
     __syncthreads();
@@ -270,7 +274,8 @@ __global__ __launch_bounds__(RS_THREADS) void radix_scatter_kernel(const uint64_
 
 size_t radix_tmp_bytes(uint32_t n) {
     const uint32_t nb = ceil_div(n ? n : 1, RS_TILE);
-    return (size_t)256 * nb * 4 * 2 + (size_t)16 * 256 * 4 + 64;  // counts, offsets, digit totals per pass
+    // counts, offsets, digit totals per pass (RS_TOT_COPIES copies, up to 16 passes)
+    return (size_t)256 * nb * 4 * 2 + (size_t)16 * RS_TOT_COPIES * 256 * 4 + 64;
 }
 
 // Sorts (keys, vals) in place over bits [0, nbits); alt buffers are scratch.
@@ -280,18 +285,19 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
     const uint32_t nb = ceil_div(n, RS_TILE);
     uint32_t* counts = (uint32_t*)tmp;
     uint32_t* offs = counts + (size_t)256 * nb;
-    uint32_t* totals = offs + (size_t)256 * nb;  // [pass][256]
+    uint32_t* totals = offs + (size_t)256 * nb;  // [pass][RS_TOT_COPIES][256]
     int passes = (nbits + 7) / 8;
     if (passes & 1) ++passes;  // even number of passes: result ends in (keys, vals)
     // digits per offsets block: whole digits, about RS_TILE counts per block
     uint32_t G = 1;
     while (G < 256 && G * 2 * nb <= (uint32_t)RS_TILE) G *= 2;
-    hipLaunchKernelGGL(radix_zero_kernel, dim3(1), dim3(256), 0, st, totals, (uint32_t)(passes * 256));
+    hipLaunchKernelGGL(radix_zero_kernel, dim3(1), dim3(1024), 0, st, totals,
+                       (uint32_t)(passes * RS_TOT_COPIES * 256));
     uint64_t *ka = keys, *kb = keys_alt;
     uint32_t *va = vals, *vb = vals_alt;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        uint32_t* tot = totals + 256 * p;
+        uint32_t* tot = totals + RS_TOT_COPIES * 256 * p;
         hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(RS_THREADS), 0, st, ka, n, shift, counts, nb, tot);
         hipLaunchKernelGGL(radix_offsets_kernel, dim3(256 / G), dim3(RS_THREADS), 0, st, counts, offs, tot, nb, G);
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(RS_THREADS), 0, st, ka, va, kb, vb, offs, n,
