@@ -21,7 +21,9 @@ alone; `in_timed_steps` gives their overlapped figures).  `traffic` comes
 from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
 """
 import argparse
+import itertools
 import json
+import math
 import os
 import sys
 import time
@@ -49,6 +51,21 @@ def m2l_flop(p, q, terms="all"):
 
 def ncoef(p):
     return (p + 1) * (p + 2) * (p + 3) // 6
+
+
+def same_level_threads(num, den, q, lds_per_cu=160 * 1024):
+    """Workgroup size of the library's same-level M2L kernel with the table in
+    LDS: 512 where two workgroups' tables (the same-level offsets that occur,
+    a zero row and the offset map) fit one CU's LDS, else 1024."""
+    K = 2 * int(math.floor(math.sqrt(3.0) * den / num)) + 1
+    acc = lambda x, y, z: num * num * (x * x + y * y + z * z) > 3 * den * den
+    rows = set()
+    for o in itertools.product(range(-K, K + 1), repeat=3):
+        if acc(*o) and any(not acc(*[(c + v) // 2 for c, v in zip(cc, o)])
+                           for cc in itertools.product((0, 1), repeat=3)):
+            rows.add(tuple(abs(v) for v in o))
+    lds = (len(rows) + 1) * (((ncoef(q) + 3) // 4) | 1) * 16 + ((K + 1) ** 3 * 2 + 15) // 16 * 16
+    return 512 if 2 * lds <= lds_per_cu else 1024
 
 
 def measured_traffic(kernel, workload):
@@ -542,7 +559,7 @@ def main():
     qc = (q | 16) if (terms == "all" and p < q < 2 * p) else q
     mfma = args.m2l in ("bf16x3", "mfma_f32")
     same_kernel = (f"m2l_offset_kernel<{p}, {'true' if args.m2l == 'bf16x3' else 'false'}>" if mfma
-                   else f"m2l_lds_kernel<{p}, {qc}, true>")
+                   else f"m2l_lds_kernel<{p}, {qc}, true, {same_level_threads(1, 2, qc & 15)}>")
     # the same-level pairs take the dense operator (every |k|, |n| <= p) on
     # the matrix cores, the truncation q with the chosen term set on the VALU
     same_flop = m2l_flop(p, q_same, "all" if mfma else terms)

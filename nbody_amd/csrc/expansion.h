@@ -177,6 +177,32 @@ template <int P, int Q>
 struct M2LTab {
     static constexpr M2LTables<P, Q> t{};
 };
+// The same pairs ordered by tensor coefficient k+n (a stable counting sort of
+// the table above): a tensor coefficient is dead after its group of products,
+// so a kernel that streams the tensor in holds the moments, the local
+// expansion and a few tensor coefficients at a time, not the whole tensor.
+template <int P, int Q>
+struct M2LTablesByD {
+    static constexpr int NP = m2l_npairs(P, Q);
+    int pk[NP], pn[NP], pkn[NP];
+    constexpr M2LTablesByD() : pk(), pn(), pkn() {
+        constexpr auto& T = M2LTab<P, Q>::t;
+        constexpr int NCQ = ncoef(m2l_deg(Q));
+        int start[NCQ + 1] = {};
+        for (int i = 0; i < NP; ++i) ++start[T.pkn[i] + 1];
+        for (int c = 0; c < NCQ; ++c) start[c + 1] += start[c];
+        for (int i = 0; i < NP; ++i) {
+            const int t = start[T.pkn[i]]++;
+            pk[t] = T.pk[i];
+            pn[t] = T.pn[i];
+            pkn[t] = T.pkn[i];
+        }
+    }
+};
+template <int P, int Q>
+struct M2LTabByD {
+    static constexpr M2LTablesByD<P, Q> t{};
+};
 
 // ---------------------------------------------------------------------------
 // Straight-line code generation: each table entry becomes a constexpr
@@ -246,6 +272,22 @@ __device__ __forceinline__ void m2l_all(const float* D, const float* M, float* L
 template <int P, int Q>
 __device__ __forceinline__ void m2l_apply(const float* D, const float* M, float* L) {
     m2l_all<P, Q>(D, M, L, std::make_index_sequence<m2l_npairs(P, Q)>{});
+}
+// The same products in tensor-coefficient order (M2LTabByD)
+template <int P, int Q, int I>
+__device__ __forceinline__ void m2l_step_d(const float* D, const float* M, float* L) {
+    constexpr auto& T = M2LTabByD<P, Q>::t;
+    constexpr int k = T.pk[I], n = T.pn[I], kn = T.pkn[I];
+    if constexpr (Tab<P>::t.deg[n] & 1) L[k] = fmaf(-D[kn], M[n], L[k]);
+    else L[k] = fmaf(D[kn], M[n], L[k]);
+}
+template <int P, int Q, size_t... I>
+__device__ __forceinline__ void m2l_all_d(const float* D, const float* M, float* L, std::index_sequence<I...>) {
+    (m2l_step_d<P, Q, (int)I>(D, M, L), ...);
+}
+template <int P, int Q>
+__device__ __forceinline__ void m2l_apply_by_d(const float* D, const float* M, float* L) {
+    m2l_all_d<P, Q>(D, M, L, std::make_index_sequence<m2l_npairs(P, Q)>{});
 }
 
 // Octant symmetry of the derivative tensor: with S = diag(s) a sign flip of

@@ -511,43 +511,65 @@ __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_unsoftened_kernel
 // their derivative tensor whenever they share the offset o = i_B - i_A (cells
 // at level l, R = -o h_l).  Same-level pairs always have |o_i| <= K =
 // 2 floor(sqrt(3)/theta) + 1 (their parents were split), and by the octant
-// symmetry D_m(S R) = s^m D_m(R) only offsets with o_i >= 0 are tabled:
-// (K+1)^3 tensors per level, built once per step in FP64 and stored FP32.
-// For theta = 1/2 and Q <= 5 a level's table fits in LDS (8^3 x 56 floats =
-// 112 KiB): the grouped kernel runs one launch per level, every block loads
-// that level's table once, and the only global traffic per pair is the source
-// moment row.  (Per-lane 144-byte gathers of both the tensor and the moment
-// rows made this contraction L2-bandwidth-bound.)
+// symmetry D_m(S R) = s^m D_m(R) only offsets with o_i >= 0 are tabled, and
+// of [0, K]^3 only those the criterion accepts while rejecting the parent
+// offset for some target position in its parent (the offsets that occur: 273
+// of 512 at theta = 1/2), then a zero row; the host maps [0, K]^3 to rows.
+// Built once per step in FP64 and stored FP32.  For theta = 1/2 and Q <= 5 a
+// level's table fits in LDS twice per CU (274 x 60 floats = 64 KiB): the
+// grouped kernel runs one launch per level, every block loads that level's
+// table once, and the only global traffic per pair is the source moment row.
+// (Per-lane 144-byte gathers of both the tensor and the moment rows made this
+// contraction L2-bandwidth-bound.)
 template <int Q>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) void dtable_kernel(
-    FmmParams F, float* out, int level0, int nlev, int E, int a, int b, int shift, int nsys) {
+    FmmParams F, float* out, int level0, int nlev, int E, int a, int b, int shift, int nsys, const uint32_t* offs,
+    int nrows) {
     constexpr int NCQ = ncoef(Q);
-    const int per = E * E * E;
+    const int per = offs ? nrows : E * E * E;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= per * nlev * nsys) return;
     const int s = t / (per * nlev), lv = (t / per) % nlev, o = t % per;
-    const int ex = o / (E * E), ey = (o / E) % E, ez = o % E;
+    float* row = out + (size_t)t * F.dcp;
+    int ex, ey, ez;
+    if (offs) {
+        const uint32_t w = offs[o];
+        if (w == ~0u) {
+            for (int k = 0; k < F.dcp; ++k) row[k] = 0.f;
+            return;
+        }
+        ex = (int)(w & 255u);
+        ey = (int)((w >> 8) & 255u);
+        ez = (int)((w >> 16) & 255u);
+    } else {
+        ex = o / (E * E);
+        ey = (o / E) % E;
+        ez = o % E;
+    }
     const double h = ldexp(F.box[s].w, -(level0 + lv + shift));
     double D[NCQ];
     derivs<Q, double>(-(a + b * ex) * h, -(a + b * ey) * h, -(a + b * ez) * h, (double)F.eps2_64, D);
-    float* row = out + (size_t)t * F.dcp;
 #pragma unroll
     for (int k = 0; k < NCQ; ++k) row[k] = (float)D[k];
     for (int k = NCQ; k < F.dcp; ++k) row[k] = 0.f;
 }
 
-// 1024 threads (16 waves at <= 128 VGPRs, one workgroup per CU beside a
-// 112 KiB table) while expansion and tensor fit that register budget (up to
-// p = 4, q = 5); 256 above, with the table read from global memory
+// 1024 threads (16 waves at <= 128 VGPRs) while expansion and tensor fit that
+// register budget (up to p = 4, q = 5); 256 above, with the table read from
+// global memory
 constexpr int m2l_table_threads(int p, int qc) { return ncoef(p) + ncoef(m2l_deg(qc)) <= 91 ? 1024 : 256; }
 
 // Same-level part of the rows of the level's targets [off, off+count): one
 // wavefront per target, lanes over its entries; writes L.  IN_LDS: the
-// level's table is first copied to LDS (when it fits one workgroup's LDS),
-// else it is read from global memory.
-template <int P, int Q, bool IN_LDS>
-__global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count,
-                                                                  int lvl) {
+// level's table and the offset -> row map are first copied to LDS (when they
+// fit), else read from global memory.  NT threads per workgroup: 512 where
+// two workgroups' tables fit one CU's LDS (the table holds only the offsets
+// that occur: 273 rows, 64 KiB at theta = 1/2, q = 5), so that one workgroup
+// computes while the other copies its table or waits for its longest list;
+// else 1024.  Either way 16 wavefronts per CU at <= 128 VGPRs.
+template <int P, int Q, bool IN_LDS, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : 1)))
+void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
     constexpr int NC = ncoef(P), NCQ = ncoef(m2l_deg(Q));
     constexpr int NQD = (NCQ + 3) / 4;  // float4 per tensor row (the host sets dcp = 4 NQD)
     // LDS row stride in float4: odd, so that the rows that the lanes of a
@@ -562,15 +584,19 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
         count = F.C->level_count[lvl + F.dlevel0];
         off = F.C->level_off[lvl + F.dlevel0];
     }
-    if (blockIdx.x * (blockDim.x / WAVE) >= count) return;  // uniform: no target for this block
+    const uint32_t wpb = NT / WAVE;
+    if (blockIdx.x * wpb >= count) return;  // uniform: no target for this block
     const int K = F.dK, K1 = K + 1;
     const float4* src = reinterpret_cast<const float4*>(F.dtab + (size_t)lvl * F.dsz * F.dcp);
+    uint16_t* map_lds = reinterpret_cast<uint16_t*>(dtab_lds + F.dsz * NQS);
+    const uint16_t* map = IN_LDS ? map_lds : F.dmap;
     if (IN_LDS) {
         const int tot = F.dsz * NQD;
-        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        for (int i = threadIdx.x; i < tot; i += NT) {
             const int r = i / NQD;
             dtab_lds[r * NQS + (i - r * NQD)] = src[i];
         }
+        for (int i = threadIdx.x; i < K1 * K1 * K1; i += NT) map_lds[i] = F.dmap[i];
         __syncthreads();
     }
     constexpr int RS = IN_LDS ? NQS : NQD;
@@ -578,7 +604,6 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
     // its own system's table from global memory
     const size_t sys_stride = (size_t)F.dnlev * F.dsz * NQD;
     const int lane = lane_id();
-    const uint32_t wpb = blockDim.x / WAVE;
     int rs0, rsn;  // coefficients [rs0, rs0 + rsn) of this lane after the reduction
     rs_slot(NC, 6, lane, rs0, rsn);
     for (uint32_t t = blockIdx.x * wpb + threadIdx.x / WAVE; t < count; t += gridDim.x * wpb) {
@@ -595,12 +620,23 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
         {
             const uint32_t s0 = F.lists.m_start[r];
             const uint32_t e1 = s0 + F.lists.m_cnt[r] - F.lists.mx_cnt[r];
-            for (uint32_t e = s0 + lane; e < e1; e += WAVE) {
-                const int B = F.lists.m_pool[e];
+            // the lane's next source id is loaded before this entry's
+            // products, so that of the dependent loads pool -> geo ->
+            // moments only two wait per entry.  The products run in tensor
+            // coefficient order (m2l_apply_by_d): the tensor streams in from
+            // LDS and the registers freed hold the loads in flight (together
+            // 6% off the step against source-coefficient order with no
+            // prefetch; either alone gains 0-2%)
+            uint32_t e = s0 + lane;
+            int Bn = e < e1 ? F.lists.m_pool[e] : 0;
+            for (; e < e1; e += WAVE) {
+                const int B = Bn;
+                Bn = e + WAVE < e1 ? F.lists.m_pool[e + WAVE] : 0;
                 const int4 gb = F.nodes.geo[B];
                 const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
                 const uint32_t neg = (uint32_t)(ox < 0) | ((uint32_t)(oy < 0) << 1) | ((uint32_t)(oz < 0) << 2);
-                const float4* row = tab + ((abs(ox) * K1 + abs(oy)) * K1 + abs(oz)) * RS;
+                // (an offset outside the table maps to its zero row)
+                const float4* row = tab + (uint32_t)map[(abs(ox) * K1 + abs(oy)) * K1 + abs(oz)] * RS;
                 float D[NCQ];
 #pragma unroll
                 for (int q = 0; q < NQD; ++q) {
@@ -611,7 +647,7 @@ __global__ __launch_bounds__(m2l_table_threads(P, Q)) void m2l_lds_kernel(FmmPar
                     if (4 * q + 3 < NCQ) D[4 * q + 3] = v.w;
                 }
                 octant_signs<m2l_deg(Q)>(neg, D);
-                m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
+                m2l_apply_by_d<P, Q>(D, F.M + (size_t)B * F.ncp, L);
             }
         }
         group_reduce_scatter<NC, 6>(L);
@@ -911,13 +947,23 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
             }
         }
     }
-    const size_t lds = (size_t)F.dsz * ((F.dcp / 4) | 1) * 16;  // rows padded to an odd float4 count
-    const bool in_lds = plan.nsys == 1 && lds <= plan.max_lds && m2l_table_threads(P, Q) == 1024;
+    constexpr int TT = m2l_table_threads(P, Q);
+    const int K1 = F.dK + 1;
+    // rows padded to an odd float4 count, then the offset -> row map
+    const size_t lds = (size_t)F.dsz * ((F.dcp / 4) | 1) * 16 + (((size_t)K1 * K1 * K1 * 2 + 15) & ~(size_t)15);
+    const bool in_lds = plan.nsys == 1 && lds <= plan.max_lds && TT == 1024;
+    // two workgroups of 512 per CU where two tables fit (the largest
+    // workgroup LDS, max_lds, is all of a CU's LDS on MI355X)
+    const bool two = in_lds && 2 * lds <= plan.max_lds;
     // (set by the uncaptured evaluation that precedes every capture: not a
     // stream operation, kept out of a capture)
-    if (in_lds && (parts & M2L_LEVELS) && !plan.dev)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if constexpr (TT == 1024) {
+        if (in_lds && (parts & M2L_LEVELS) && !plan.dev) {
+            const void* k = two ? reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true, 512>)
+                                : reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true, 1024>);
+            (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        }
+    }
     for (int l = F.dlevel0; l < F.dlevel0 + F.dnlev; ++l) {
         if (!(parts & M2L_LEVELS) || (level >= 0 && l != level)) continue;
         const uint32_t cnt = plan.level_count[l];
@@ -929,19 +975,26 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
                                     l - 2, st);
             continue;
         }
-        constexpr int T = m2l_table_threads(P, Q);
         // one target per wavefront: the workgroup scheduler balances the
-        // uneven list lengths better than a grid-stride loop (measured);
-        // with device counts the grid is sized from plan.level_count and the
-        // kernel strides over the rest
+        // uneven list lengths better than a grid-stride loop or a resident
+        // grid with a work queue (measured); with device counts the grid is
+        // sized from plan.level_count and the kernel strides over the rest
+        const int T = two ? 512 : TT;
         const unsigned g = std::min<unsigned>(ceil_div(cnt, T / WAVE), 65535u);
         const uint32_t koff = plan.dev ? 0u : plan.level_off[l], kcnt = plan.dev ? DEV_COUNT : cnt;
-        if (in_lds)
-            hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true>), dim3(g), dim3(T), lds, st, F, koff, kcnt,
-                               l - F.dlevel0);
-        else
-            hipLaunchKernelGGL((m2l_lds_kernel<P, Q, false>), dim3(g), dim3(T), 0, st, F, koff, kcnt,
-                               l - F.dlevel0);
+        const int lv = l - F.dlevel0;
+        if constexpr (TT == 1024) {
+            if (two) {
+                hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true, 512>), dim3(g), dim3(512), lds, st, F, koff, kcnt, lv);
+                continue;
+            }
+            if (in_lds) {
+                hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true, 1024>), dim3(g), dim3(1024), lds, st, F, koff, kcnt,
+                                   lv);
+                continue;
+            }
+        }
+        hipLaunchKernelGGL((m2l_lds_kernel<P, Q, false, TT>), dim3(g), dim3(TT), 0, st, F, koff, kcnt, lv);
     }
     if (!(parts & M2L_REST)) return;
     const uint32_t knn = plan.dev ? DEV_COUNT : nn;
@@ -954,10 +1007,10 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
 
 template <int Q>
 static void launch_dtable_t(const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
-                            int nsys, hipStream_t st) {
-    const unsigned total = (unsigned)(E * E * E * nlev * nsys);
+                            int nsys, const uint32_t* offs, int nrows, hipStream_t st) {
+    const unsigned total = (unsigned)((offs ? nrows : E * E * E) * nlev * nsys);
     hipLaunchKernelGGL((dtable_kernel<Q>), dim3(ceil_div(total, 128)), dim3(128), 0, st, F, out, level0, nlev, E, a,
-                       b, shift, nsys);
+                       b, shift, nsys, offs, nrows);
 }
 
 // grid of the launches whose count is read on the device (DEV_COUNT): sized
@@ -1040,8 +1093,8 @@ void launch_p2p_staged(int p, int chunk, const FmmParams& F, const State& S, uin
     else NBH_DISPATCH_P(p, (launch_p2p_staged_t<PP, 1>(F, S, num_leaves, items, st)))
 }
 void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
-                   int nsys, hipStream_t st) {
-#define NBH_DT(QQ) launch_dtable_t<QQ>(F, out, level0, nlev, E, a, b, shift, nsys, st)
+                   int nsys, const uint32_t* offs, int nrows, hipStream_t st) {
+#define NBH_DT(QQ) launch_dtable_t<QQ>(F, out, level0, nlev, E, a, b, shift, nsys, offs, nrows, st)
     switch (q) {
         case 1: NBH_DT(1); break;
         case 2: NBH_DT(2); break;

@@ -118,11 +118,13 @@ struct FmmParams {
     ListView lists;
     unsigned long long* p2p_inter;  // statistics: particle pairs of the P2P kernels
     uint32_t* pleaf;       // per particle: its leaf node
-    float* dtab;           // per system and level, derivative tensors of the offsets [0, K]^3 (or null);
-                           // system s, level l at row ((s dnlev + l - dlevel0) dsz + offset)
+    float* dtab;           // per system and level, derivative tensors of the same-level offsets (or null);
+                           // system s, level l at row ((s dnlev + l - dlevel0) dsz + row)
     int dK, dlevel0, dnlev;
     int dcp;               // tensor row stride (floats): ncoef(Q) padded to 4
-    int dsz;               // tensors per level: (K+1)^3
+    int dsz;               // tensors per level: the offsets that occur, and a zero row last
+    const uint32_t* doffs; // [dsz] offset (|o_x|, |o_y|, |o_z|) of each row, 8 bits each (the zero row: ~0)
+    const uint16_t* dmap;  // [(K+1)^3] row of the offset (|o_x| (K+1) + |o_y|) (K+1) + |o_z|
     // pairs one level apart: per system and finer level lf, tensors of the
     // centre offsets d (units of half the finer cell, odd) with |d_i| = 1, 3,
     // .., 2 xE - 1
@@ -198,11 +200,13 @@ struct M2LPlan {
     // then only size the grids, and level_off is not used
     bool dev;
 };
-// tensors at R = -(a + b e) h_l / 2^shift for e in [0, E)^3, levels
-// level0 .. level0+nlev-1 of systems 0 .. nsys-1 (h_l from each system's
-// root size), into out (rows of F.dcp floats, system-major, then level)
+// tensors at R = -(a + b e) h_l / 2^shift for e in [0, E)^3 (offs null) or
+// for the nrows offsets e of offs (8 bits per axis; ~0: a row of zeros),
+// levels level0 .. level0+nlev-1 of systems 0 .. nsys-1 (h_l from each
+// system's root size), into out (rows of F.dcp floats, system-major, then
+// level)
 void launch_dtable(int q, const FmmParams& F, float* out, int level0, int nlev, int E, int a, int b, int shift,
-                   int nsys, hipStream_t st);
+                   int nsys, const uint32_t* offs, int nrows, hipStream_t st);
 void launch_m2l(int p, int q, const FmmParams& F, const M2LPlan& plan, hipStream_t st);
 // The grouped M2L in parts (when F carries the tables): the same-level kernel
 // of one level (level < 0: every level), and the rest (initialisation, the

@@ -124,6 +124,8 @@ struct nbh_handle {
     // same-level M2L by translation offset (matrix cores): split moments
     // (forward and adjoint), dense tensor tables, candidate offsets
     DevBuf mb, mbw, d2tab, cand, hank, opar;
+    DevBuf sl_offs, sl_map;  // same-level table: offsets that occur (rows), offset -> row map
+    int sl_rows = 0;         // rows of the same-level table per level, its zero row included
     // trajectory adjoint: checkpoints of the full state (every adj_k steps),
     // the final state, the recomputed states of one segment, the cotangents
     std::vector<DevBuf> adj_ckpt, adj_seg;
@@ -433,6 +435,8 @@ static void copy_tables(FmmParams& F, const FmmParams& T) {
     F.dnlev = T.dnlev;
     F.dcp = T.dcp;
     F.dsz = T.dsz;
+    F.doffs = T.doffs;
+    F.dmap = T.dmap;
     F.xtab = T.xtab;
     F.xE = T.xE;
     F.xlevel0 = T.xlevel0;
@@ -658,7 +662,7 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false, const 
         // system, since the tensors depend on each system's root size
         const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
         const int K = 2 * Kp + 1;
-        const int dsz = (K + 1) * (K + 1) * (K + 1), dcp = ncp_of(m2l_deg(q));
+        const int dsz = h->sl_rows, dcp = ncp_of(m2l_deg(q));
         const int nlev = (int)depth - 1;
         HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4 * h->nsys));
         T = F;
@@ -668,7 +672,9 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false, const 
         T.dnlev = nlev;
         T.dcp = dcp;
         T.dsz = dsz;
-        launch_dtable(m2l_deg(q), T, T.dtab, 2, nlev, K + 1, 0, 1, 0, (int)h->nsys, h->stream2);
+        T.doffs = h->sl_offs.as<uint32_t>();
+        T.dmap = h->sl_map.as<uint16_t>();
+        launch_dtable(m2l_deg(q), T, T.dtab, 2, nlev, K + 1, 0, 1, 0, (int)h->nsys, T.doffs, dsz, h->stream2);
         // pairs one level apart, by finer level 1..depth: offsets |d_i| <=
         // 4 Kp + 1 half finer cells (the coarser side's parent pair was
         // split), tabled at d_i = 1, 3, .., 4 Kp + 1
@@ -679,13 +685,14 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false, const 
         T.xE = E;
         T.xlevel0 = 1;
         T.xnlev = xn;
-        launch_dtable(m2l_deg(q), T, T.xtab, 1, xn, E, 1, 2, 1, (int)h->nsys, h->stream2);
+        launch_dtable(m2l_deg(q), T, T.xtab, 1, xn, E, 1, 2, 1, (int)h->nsys, nullptr, 0, h->stream2);
         if (use_offset_m2l(h)) {
             // the matrix-core same-level M2L: signed split moments and the
             // dense tensors (orders <= 2p) of the offsets [0, K]^3, levels 2..depth
             const bool bf3 = h->cfg.m2l == NBH_M2L_BF16X3;
             HIPCHK(h, h->mb.ensure(off_moment_bytes(p, bf3, h->num_nodes)));
-            HIPCHK(h, h->d2tab.ensure((size_t)dsz * nlev * off_table_words(p) * 4));
+            const size_t cube = (size_t)(K + 1) * (K + 1) * (K + 1);  // every offset of [0, K]^3
+            HIPCHK(h, h->d2tab.ensure(cube * nlev * off_table_words(p) * 4));
             launch_off_moments(p, bf3, F.M, F.ncp, h->num_nodes, h->geo.as<int4>(), h->box.as<double4>(), h->mb.p, h->stream2);
             launch_off_dtable(p, bf3, h->box.as<double4>(), h->cfg.eps * h->cfg.eps, 2, nlev, K + 1,
                               h->d2tab.as<uint32_t>(), h->stream2);
@@ -902,8 +909,8 @@ static int graph_prepare(nbh_handle* h) {
     // the tables (as force_fmm sizes them) and the deferred pairs
     const int q = m2l_q(h);
     const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
-    const int K = 2 * Kp + 1, E = 2 * Kp + 1;
-    const size_t dsz = (size_t)(K + 1) * (K + 1) * (K + 1), dcp = (size_t)ncp_of(m2l_deg(q));
+    const int E = 2 * Kp + 1;
+    const size_t dsz = (size_t)h->sl_rows, dcp = (size_t)ncp_of(m2l_deg(q));
     HIPCHK(h, h->dtab.ensure(dsz * (g.depth_cap - 1) * dcp * 4 * h->nsys));
     HIPCHK(h, h->xtab.ensure((size_t)E * E * E * g.depth_cap * dcp * 4 * h->nsys));
     HIPCHK(h, h->far_src.ensure((size_t)g.far_cap * 4));
@@ -1194,6 +1201,46 @@ static int create_impl(const nbh_config* cfg, size_t nsys, const uint32_t* sys_o
         // criterion accepts o and rejects the parent offset floor((c + o) / 2)
         const int num = h->theta_num, den = h->theta_den;
         const int K = 2 * (int)std::floor(std::sqrt(3.0) * den / num) + 1;
+        {
+            // the same-level tables of the vector M2L hold the offsets (|o_x|,
+            // |o_y|, |o_z|) that occur (same criterion, every target class),
+            // in lexicographic order, then a zero row that the other entries
+            // of the (K+1)^3 map point to
+            auto acc = [&](int x, int y, int z) {
+                return (long long)num * num * (x * x + y * y + z * z) > 3ll * den * den;
+            };
+            auto half = [](int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); };
+            const int K1 = K + 1;
+            std::vector<uint8_t> occurs((size_t)K1 * K1 * K1, 0);
+            for (int ox = -K; ox <= K; ++ox)
+                for (int oy = -K; oy <= K; ++oy)
+                    for (int oz = -K; oz <= K; ++oz) {
+                        if (!acc(ox, oy, oz)) continue;
+                        for (int c = 0; c < 8; ++c)
+                            if (!acc(half(((c >> 2) & 1) + ox), half(((c >> 1) & 1) + oy), half((c & 1) + oz))) {
+                                occurs[((size_t)std::abs(ox) * K1 + std::abs(oy)) * K1 + std::abs(oz)] = 1;
+                                break;
+                            }
+                    }
+            std::vector<uint32_t> offs;
+            std::vector<uint16_t> map((size_t)K1 * K1 * K1);
+            for (int x = 0; x < K1; ++x)
+                for (int y = 0; y < K1; ++y)
+                    for (int z = 0; z < K1; ++z)
+                        if (occurs[((size_t)x * K1 + y) * K1 + z]) offs.push_back((uint32_t)x | (uint32_t)y << 8 | (uint32_t)z << 16);
+            if (K1 > 256 || offs.size() >= 0xffff) {
+                h->fail(NBH_E_INVALID, "theta too small for the same-level tables");
+                return bail(NBH_E_INVALID);
+            }
+            const uint16_t zero_row = (uint16_t)offs.size();
+            for (size_t i = 0, r = 0; i < map.size(); ++i) map[i] = occurs[i] ? (uint16_t)r++ : zero_row;
+            offs.push_back(~0u);
+            CCHK(h->sl_offs.ensure(offs.size() * 4));
+            CCHK(hipMemcpy(h->sl_offs.p, offs.data(), offs.size() * 4, hipMemcpyHostToDevice));
+            CCHK(h->sl_map.ensure(map.size() * 2));
+            CCHK(hipMemcpy(h->sl_map.p, map.data(), map.size() * 2, hipMemcpyHostToDevice));
+            h->sl_rows = (int)offs.size();
+        }
         int dev_lds = 0;
         CCHK(hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
         std::vector<uint32_t> cand;
@@ -1298,7 +1345,7 @@ void nbh_destroy(nbh_handle* h) {
                      &h->row_next, &h->row_sl_start, &h->row_sl_cnt, &h->row_m_start, &h->row_m_cnt,
                      &h->row_mx_cnt, &h->row_p_start, &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->counters, &h->diag,
                      &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf, &h->mb, &h->mbw, &h->d2tab,
-                     &h->cand, &h->hank, &h->opar};
+                     &h->cand, &h->hank, &h->opar, &h->sl_offs, &h->sl_map};
     for (DevBuf* b : all) b->release();
     for (DevBuf& b : h->adj_ckpt) b.release();
     for (DevBuf& b : h->adj_seg) b.release();

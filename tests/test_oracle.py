@@ -267,6 +267,58 @@ def test_walk_lattice_counts():
         assert L.m2l_off[target + 1] - L.m2l_off[target] == far
 
 
+def _same_level_offsets(num, den):
+    """(|o_x|, |o_y|, |o_z|) of the same-level M2L pairs that can occur: the
+    criterion accepts o and rejects the parent offset floor((c + o) / 2) for
+    some target position c in its parent (the set the library tables)."""
+    K = 2 * int(np.floor(np.sqrt(3.0) * den / num)) + 1
+
+    def acc(x, y, z):
+        return num * num * (x * x + y * y + z * z) > 3 * den * den
+
+    out = set()
+    for o in itertools.product(range(-K, K + 1), repeat=3):
+        if acc(*o) and any(not acc(*[(ci + oi) // 2 for ci, oi in zip(c, o)])
+                           for c in itertools.product((0, 1), repeat=3)):
+            out.add(tuple(abs(v) for v in o))
+    return out
+
+
+def test_same_level_offsets_that_occur_are_the_tabled_set():
+    """Uniform lattice with complete neighbourhoods around the 8 level-5
+    cells (16|17)^3 (one per position in the parent): the folded offsets of
+    their same-level M2L partners in the oracle's walk are exactly the set;
+    a clustered cloud only meets offsets of the set."""
+    g = 32
+    ijk = np.stack(np.meshgrid(*[np.arange(g)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    ijk = ijk[np.all(np.abs(ijk - 16) <= 10, axis=1)]
+    x = np.concatenate([(ijk + 0.5) / g, [[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]]])
+    _, _, t, _ = _tree(x, cap=1, max_depth=5)
+    rng = np.random.default_rng(11)
+    xc = np.concatenate([rng.random((1500, 3)), 0.3 + 0.02 * rng.standard_normal((1500, 3))]).clip(0, 1)
+    _, _, tc, _ = _tree(xc, cap=4)
+    for (num, den), rows in [((1, 2), 273), ((2, 5), 483)]:
+        want = _same_level_offsets(num, den)
+        assert len(want) == rows
+        L = po.walk(t, num, den)
+        seen = set()
+        for A in np.where(t.level == 5)[0]:
+            if not all(v in (16, 17) for v in (t.ix[A], t.iy[A], t.iz[A])):
+                continue
+            for B in L.m2l_src[L.m2l_off[A]:L.m2l_off[A + 1]]:
+                if t.level[B] == 5:
+                    seen.add((abs(int(t.ix[B]) - int(t.ix[A])), abs(int(t.iy[B]) - int(t.iy[A])),
+                              abs(int(t.iz[B]) - int(t.iz[A]))))
+        assert seen == want
+        Lc = po.walk(tc, num, den)
+        for A in range(len(tc.level)):
+            for B in Lc.m2l_src[Lc.m2l_off[A]:Lc.m2l_off[A + 1]]:
+                if tc.level[B] == tc.level[A]:
+                    o = (abs(int(tc.ix[B]) - int(tc.ix[A])), abs(int(tc.iy[B]) - int(tc.iy[A])),
+                         abs(int(tc.iz[B]) - int(tc.iz[A])))
+                    assert o in want
+
+
 # ---------------------------------------------------------------------------
 # expansions
 def test_derivative_tensor_vs_sympy():
