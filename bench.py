@@ -53,10 +53,12 @@ def ncoef(p):
     return (p + 1) * (p + 2) * (p + 3) // 6
 
 
-def same_level_threads(num, den, q, lds_per_cu=160 * 1024):
-    """Workgroup size of the library's same-level M2L kernel with the table in
-    LDS: 512 where two workgroups' tables (the same-level offsets that occur,
-    a zero row and the offset map) fit one CU's LDS, else 1024."""
+def same_level_kernel(p, qc, num, den, lds_per_cu=160 * 1024):
+    """Name of the library's same-level M2L kernel with the table in LDS: the
+    pipelined 256-thread form where two workgroups' tables (the same-level
+    offsets that occur, a zero row and the offset map) fit one CU's LDS, else
+    the 1024-thread form."""
+    q = qc & 15
     K = 2 * int(math.floor(math.sqrt(3.0) * den / num)) + 1
     acc = lambda x, y, z: num * num * (x * x + y * y + z * z) > 3 * den * den
     rows = set()
@@ -65,7 +67,8 @@ def same_level_threads(num, den, q, lds_per_cu=160 * 1024):
                            for cc in itertools.product((0, 1), repeat=3)):
             rows.add(tuple(abs(v) for v in o))
     lds = (len(rows) + 1) * (((ncoef(q) + 3) // 4) | 1) * 16 + ((K + 1) ** 3 * 2 + 15) // 16 * 16
-    return 512 if 2 * lds <= lds_per_cu else 1024
+    two = 2 * lds <= lds_per_cu
+    return f"m2l_lds_kernel<{p}, {qc}, true, {256 if two else 1024}, {'true' if two else 'false'}>"
 
 
 def measured_traffic(kernel, workload):
@@ -559,7 +562,7 @@ def main():
     qc = (q | 16) if (terms == "all" and p < q < 2 * p) else q
     mfma = args.m2l in ("bf16x3", "mfma_f32")
     same_kernel = (f"m2l_offset_kernel<{p}, {'true' if args.m2l == 'bf16x3' else 'false'}>" if mfma
-                   else f"m2l_lds_kernel<{p}, {qc}, true, {same_level_threads(1, 2, qc & 15)}>")
+                   else same_level_kernel(p, qc, 1, 2))
     # the same-level pairs take the dense operator (every |k|, |n| <= p) on
     # the matrix cores, the truncation q with the chosen term set on the VALU
     same_flop = m2l_flop(p, q_same, "all" if mfma else terms)

@@ -562,13 +562,18 @@ constexpr int m2l_table_threads(int p, int qc) { return ncoef(p) + ncoef(m2l_deg
 // Same-level part of the rows of the level's targets [off, off+count): one
 // wavefront per target, lanes over its entries; writes L.  IN_LDS: the
 // level's table and the offset -> row map are first copied to LDS (when they
-// fit), else read from global memory.  NT threads per workgroup: 512 where
-// two workgroups' tables fit one CU's LDS (the table holds only the offsets
-// that occur: 273 rows, 64 KiB at theta = 1/2, q = 5), so that one workgroup
-// computes while the other copies its table or waits for its longest list;
-// else 1024.  Either way 16 wavefronts per CU at <= 128 VGPRs.
-template <int P, int Q, bool IN_LDS, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : 1)))
+// fit), else read from global memory.  Two forms:
+//  - PIPE, 256 threads, where two workgroups' tables fit one CU's LDS (the
+//    table holds only the offsets that occur: 273 rows, 64 KiB at theta =
+//    1/2, q = 5): 2 wavefronts per SIMD with 256 VGPRs each, every wavefront
+//    holding two entries so that one entry's loads fly while the other's
+//    products run; one workgroup computes while the other copies its table
+//    or finishes its last list;
+//  - else 1024 threads, 4 wavefronts per SIMD at <= 128 VGPRs, one entry at
+//    a time with the next source id prefetched (256 with the table in
+//    global memory where the expansion needs more registers).
+template <int P, int Q, bool IN_LDS, int NT, bool PIPE = false>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : (PIPE ? 2 : 1), NT >= 512 ? 4 : (PIPE ? 2 : 8))))
 void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
     constexpr int NC = ncoef(P), NCQ = ncoef(m2l_deg(Q));
     constexpr int NQD = (NCQ + 3) / 4;  // float4 per tensor row (the host sets dcp = 4 NQD)
@@ -627,6 +632,84 @@ void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
             // LDS and the registers freed hold the loads in flight (together
             // 6% off the step against source-coefficient order with no
             // prefetch; either alone gains 0-2%)
+            if constexpr (PIPE) {
+                // two entries in registers: while one entry's products run,
+                // the next entry's tensor (LDS) and moments (global) are
+                // loaded, with the cell coordinates one further and the
+                // source id two further ahead.  Every load is issued on
+                // every lane (a lane past its list reads pool entry 0, that
+                // node and the zero row of the table, and computes nothing),
+                // so that the compiler counts the loads in flight exactly and
+                // waits only for the ones an instruction needs; the octant
+                // signs are applied when the entry's products start.
+                struct Ent {
+                    float D[NCQ];
+                    float M[NC];
+                    uint32_t neg;
+                };
+                // (the zero row where gb = ga)
+                auto fetch = [&](Ent& X, const int4 gb, const int B) {
+                    const int ox = gb.x - ga.x, oy = gb.y - ga.y, oz = gb.z - ga.z;
+                    X.neg = (uint32_t)(ox < 0) | ((uint32_t)(oy < 0) << 1) | ((uint32_t)(oz < 0) << 2);
+                    const float4* row = tab + (uint32_t)map[(abs(ox) * K1 + abs(oy)) * K1 + abs(oz)] * RS;
+#pragma unroll
+                    for (int q = 0; q < NQD; ++q) {
+                        const float4 v = row[q];
+                        if (4 * q + 0 < NCQ) X.D[4 * q + 0] = v.x;
+                        if (4 * q + 1 < NCQ) X.D[4 * q + 1] = v.y;
+                        if (4 * q + 2 < NCQ) X.D[4 * q + 2] = v.z;
+                        if (4 * q + 3 < NCQ) X.D[4 * q + 3] = v.w;
+                    }
+                    const float4* mr = reinterpret_cast<const float4*>(F.M + (size_t)B * F.ncp);
+#pragma unroll
+                    for (int q = 0; q < (NC + 3) / 4; ++q) {
+                        const float4 v = mr[q];
+                        if (4 * q + 0 < NC) X.M[4 * q + 0] = v.x;
+                        if (4 * q + 1 < NC) X.M[4 * q + 1] = v.y;
+                        if (4 * q + 2 < NC) X.M[4 * q + 2] = v.z;
+                        if (4 * q + 3 < NC) X.M[4 * q + 3] = v.w;
+                    }
+                };
+                auto apply = [&](Ent& X) {
+                    octant_signs<m2l_deg(Q)>(X.neg, X.D);
+                    m2l_apply_by_d<P, Q>(X.D, X.M, L);
+                };
+                // source id of entry i; past the list, that of pool entry 0
+                // (a node id, so its loads stay in bounds; the value itself
+                // is not selected away, which would wait for the load here)
+                auto source = [&](uint32_t i) { return F.lists.m_pool[i < e1 ? i : 0u]; };
+                uint32_t e = s0 + lane;
+                const int B0 = source(e);
+                int Bn = source(e + WAVE);
+                Ent X, Y;
+                {
+                    const int4 g0 = F.nodes.geo[B0];
+                    fetch(X, e < e1 ? g0 : ga, B0);
+                }
+                int4 gn = F.nodes.geo[Bn];
+                int Bnn = source(e + 2 * WAVE);
+                // at the top: X holds entry e; gn, Bn are entry e+W's cell
+                // and source, Bnn entry e+2W's source.  (The scheduling
+                // barriers keep the loads ahead of the products: the
+                // scheduler would otherwise sink them to the products that
+                // use them, to save registers.)
+                auto stage = [&](Ent& Z) {
+                    fetch(Z, e + WAVE < e1 ? gn : ga, Bn);
+                    gn = F.nodes.geo[Bnn];
+                    Bn = Bnn;
+                    Bnn = source(e + 3 * WAVE);
+                    __builtin_amdgcn_sched_barrier(0);
+                };
+                while (e < e1) {
+                    stage(Y);
+                    apply(X);
+                    e += WAVE;
+                    if (e >= e1) break;
+                    stage(X);
+                    apply(Y);
+                    e += WAVE;
+                }
+            } else {
             uint32_t e = s0 + lane;
             int Bn = e < e1 ? F.lists.m_pool[e] : 0;
             for (; e < e1; e += WAVE) {
@@ -648,6 +731,7 @@ void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
                 }
                 octant_signs<m2l_deg(Q)>(neg, D);
                 m2l_apply_by_d<P, Q>(D, F.M + (size_t)B * F.ncp, L);
+            }
             }
         }
         group_reduce_scatter<NC, 6>(L);
@@ -952,14 +1036,14 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
     // rows padded to an odd float4 count, then the offset -> row map
     const size_t lds = (size_t)F.dsz * ((F.dcp / 4) | 1) * 16 + (((size_t)K1 * K1 * K1 * 2 + 15) & ~(size_t)15);
     const bool in_lds = plan.nsys == 1 && lds <= plan.max_lds && TT == 1024;
-    // two workgroups of 512 per CU where two tables fit (the largest
+    // two workgroups of 256 per CU where two tables fit (the largest
     // workgroup LDS, max_lds, is all of a CU's LDS on MI355X)
     const bool two = in_lds && 2 * lds <= plan.max_lds;
     // (set by the uncaptured evaluation that precedes every capture: not a
     // stream operation, kept out of a capture)
     if constexpr (TT == 1024) {
         if (in_lds && (parts & M2L_LEVELS) && !plan.dev) {
-            const void* k = two ? reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true, 512>)
+            const void* k = two ? reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true, 256, true>)
                                 : reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true, 1024>);
             (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         }
@@ -979,13 +1063,14 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
         // uneven list lengths better than a grid-stride loop or a resident
         // grid with a work queue (measured); with device counts the grid is
         // sized from plan.level_count and the kernel strides over the rest
-        const int T = two ? 512 : TT;
+        const int T = two ? 256 : TT;
         const unsigned g = std::min<unsigned>(ceil_div(cnt, T / WAVE), 65535u);
         const uint32_t koff = plan.dev ? 0u : plan.level_off[l], kcnt = plan.dev ? DEV_COUNT : cnt;
         const int lv = l - F.dlevel0;
         if constexpr (TT == 1024) {
             if (two) {
-                hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true, 512>), dim3(g), dim3(512), lds, st, F, koff, kcnt, lv);
+                hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true, 256, true>), dim3(g), dim3(256), lds, st, F, koff, kcnt,
+                                   lv);
                 continue;
             }
             if (in_lds) {
