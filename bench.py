@@ -38,15 +38,17 @@ HBM_PEAK_GBS = 8000.0
 P2P_FLOP = 20              # per directed particle-particle interaction
 
 
-def m2l_flop(p, q, terms="all"):
+def m2l_flop(p, q, terms="all", pot=True):
     """2 x #{kept (k, n): |k|, |n| <= p and |k| + |n| <= q ("all"), or
     |k| + |n| <= p, or |k| + |n| <= q with min(|k|, |n|) <= q - p ("low")}
     per directed cell pair (420 for p = q = 4; at p = 4, q = 5: 840 with
-    every term, 600 with the low-degree set)."""
+    every term, 600 with the low-degree set).  pot=False leaves out the
+    output k = 0, which only the potential uses (a step's evaluations: 770
+    and 530 at p = 4, q = 5)."""
     tri = lambda d: (d + 1) * (d + 2) // 2
     r = q if terms == "all" else q - p
     keep = lambda a, b: a + b <= p or (a + b <= q and min(a, b) <= r)
-    return 2 * sum(tri(a) * tri(b) for a in range(p + 1) for b in range(p + 1) if keep(a, b))
+    return 2 * sum(tri(a) * tri(b) for a in range(0 if pot else 1, p + 1) for b in range(p + 1) if keep(a, b))
 
 
 def ncoef(p):
@@ -555,7 +557,8 @@ def main():
     agg, wall, inter, m2l_pairs, split_pairs, p2p_pairs, st = timed
     value = n * K / wall
     m2l_ms = agg["ms_m2l"] / K
-    m2l_tf = m2l_flop(p, q, terms) * m2l_pairs / K / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
+    # the timed steps evaluate accelerations only (no potential terms)
+    m2l_tf = m2l_flop(p, q, terms, pot=False) * m2l_pairs / K / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     workload = f"plummer_N{n}_seed1_eps1e-3_theta0.5_cap32_p{p}_kdk_dt1e-3"
     chunk = int(sim.config.p2p_chunk) or 64
     p2p_kernel = f"p2p_staged_kernel<{p}, {chunk // 64}>" if args.p2p == "lane" else f"p2p_kernel<{p}>"
@@ -565,7 +568,8 @@ def main():
                    else same_level_kernel(p, qc, 1, 2))
     # the same-level pairs take the dense operator (every |k|, |n| <= p) on
     # the matrix cores, the truncation q with the chosen term set on the VALU
-    same_flop = m2l_flop(p, q_same, "all" if mfma else terms)
+    # (the matrix-core kernels apply the dense operator, potential row included)
+    same_flop = m2l_flop(p, q_same, "all") if mfma else m2l_flop(p, q_same, terms, pot=False)
 
     def kernel_roofline(src, steps, kernel, flop_per_step, ms_per_step, launches_per_step, bound, tr):
         ach = flop_per_step / (ms_per_step * 1e-3) / 1e12 if ms_per_step > 0 else 0.0
@@ -621,7 +625,7 @@ def main():
         ms = agg["ms_" + ph] / K
         gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         phase_roofline[ph] = {"bound": "hbm", "bytes": b, "ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
-    phase_roofline["m2l"] = {"bound": "fp32-valu", "flop_per_pair": m2l_flop(p, q, terms), "ms": m2l_ms,
+    phase_roofline["m2l"] = {"bound": "fp32-valu", "flop_per_pair": m2l_flop(p, q, terms, pot=False), "ms": m2l_ms,
                              "achieved_TFLOPs": m2l_tf, "frac": m2l_tf / FP32_PEAK_TFLOPS,
                              "note": "all M2L pairs over the M2L phase's wall time (from the first same-level "
                                      "launch, beside the walk, to the last other-level kernel)"}
@@ -650,7 +654,7 @@ def main():
         "overlapped": {} if args.serial else {"moments": "walk", "m2l": "walk+p2p", "p2p": "m2l+l2l"},
         "phase_roofline": phase_roofline,
         "m2l": {"pairs_per_step": m2l_pairs / K, "same_level_pairs_per_step": tot["m2l_same_pairs"] / K,
-                "flop_per_pair": m2l_flop(p, q, terms), "tflops": m2l_tf},
+                "flop_per_pair": m2l_flop(p, q, terms, pot=False), "tflops": m2l_tf},
         "tree": {"nodes": nodes, "leaves": leaves, "depth": st["depth"]},
     }
     if not args.no_plain_compare and (args.m2l, args.p2p) != ("plain", "leaf"):

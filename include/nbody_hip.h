@@ -202,8 +202,12 @@ int nbh_direct_fp64(nbh_handle* h, size_t nt, const int64_t* targets, double* ac
 int nbh_direct_fp64_noalloc(nbh_handle* h, size_t nt, const int64_t* targets, double* acc, double* pot);
 
 /* Energy and momentum of the current state (FP64, on the GPU):
- * out[0] kinetic, out[1] potential (uses the last force evaluation),
- * out[2..4] linear momentum, out[5..7] angular momentum. */
+ * out[0] kinetic, out[1] potential, out[2..4] linear momentum,
+ * out[5..7] angular momentum.  Steps evaluate accelerations only (as the
+ * reference's step, open_cl_simulation.cpp:81-98): when the last evaluation
+ * was a step's, this call runs one evaluation with potentials at the current
+ * positions first, leaving the accelerations the integrator holds as they
+ * were. */
 int nbh_diagnostics(nbh_handle* h, double* out);
 
 int nbh_get_stats(nbh_handle* h, nbh_stats* s);

@@ -257,37 +257,45 @@ __device__ __forceinline__ void derivs(T_ Rx, T_ Ry, T_ Rz, T_ eps2, T_* D) {
     deriv_all<P, T_>(R, inv_s, D, std::make_index_sequence<ncoef(P) - 1>{});
 }
 
-template <int P, int Q, int I>
+// POT = false leaves out the products into L[0], the coefficient that only
+// the potential uses (the acceleration is the gradient: L2L never moves L[0]
+// into another coefficient, and L2P reads it for the potential alone), but
+// for the one with D[0]: with D[0] dead the compiler narrows the 16-byte
+// tensor loads to pairs of 4-byte LDS reads, twice the LDS instructions
+// (measured: 35% slower same-level kernel)
+template <int P, int Q, int I, bool POT>
 __device__ __forceinline__ void m2l_step(const float* D, const float* M, float* L) {
     constexpr auto& T = M2LTab<P, Q>::t;
     constexpr int k = T.pk[I], n = T.pn[I], kn = T.pkn[I];
-    if constexpr (Tab<P>::t.deg[n] & 1) L[k] = fmaf(-D[kn], M[n], L[k]);
+    if constexpr (!POT && k == 0 && kn != 0) return;
+    else if constexpr (Tab<P>::t.deg[n] & 1) L[k] = fmaf(-D[kn], M[n], L[k]);
     else L[k] = fmaf(D[kn], M[n], L[k]);
 }
-template <int P, int Q, size_t... I>
+template <int P, int Q, bool POT, size_t... I>
 __device__ __forceinline__ void m2l_all(const float* D, const float* M, float* L, std::index_sequence<I...>) {
-    (m2l_step<P, Q, (int)I>(D, M, L), ...);
+    (m2l_step<P, Q, (int)I, POT>(D, M, L), ...);
 }
 // D: derivatives to order Q; M, L: order P
-template <int P, int Q>
+template <int P, int Q, bool POT = true>
 __device__ __forceinline__ void m2l_apply(const float* D, const float* M, float* L) {
-    m2l_all<P, Q>(D, M, L, std::make_index_sequence<m2l_npairs(P, Q)>{});
+    m2l_all<P, Q, POT>(D, M, L, std::make_index_sequence<m2l_npairs(P, Q)>{});
 }
 // The same products in tensor-coefficient order (M2LTabByD)
-template <int P, int Q, int I>
+template <int P, int Q, int I, bool POT>
 __device__ __forceinline__ void m2l_step_d(const float* D, const float* M, float* L) {
     constexpr auto& T = M2LTabByD<P, Q>::t;
     constexpr int k = T.pk[I], n = T.pn[I], kn = T.pkn[I];
-    if constexpr (Tab<P>::t.deg[n] & 1) L[k] = fmaf(-D[kn], M[n], L[k]);
+    if constexpr (!POT && k == 0 && kn != 0) return;
+    else if constexpr (Tab<P>::t.deg[n] & 1) L[k] = fmaf(-D[kn], M[n], L[k]);
     else L[k] = fmaf(D[kn], M[n], L[k]);
 }
-template <int P, int Q, size_t... I>
+template <int P, int Q, bool POT, size_t... I>
 __device__ __forceinline__ void m2l_all_d(const float* D, const float* M, float* L, std::index_sequence<I...>) {
-    (m2l_step_d<P, Q, (int)I>(D, M, L), ...);
+    (m2l_step_d<P, Q, (int)I, POT>(D, M, L), ...);
 }
-template <int P, int Q>
+template <int P, int Q, bool POT = true>
 __device__ __forceinline__ void m2l_apply_by_d(const float* D, const float* M, float* L) {
-    m2l_all_d<P, Q>(D, M, L, std::make_index_sequence<m2l_npairs(P, Q)>{});
+    m2l_all_d<P, Q, POT>(D, M, L, std::make_index_sequence<m2l_npairs(P, Q)>{});
 }
 
 // Octant symmetry of the derivative tensor: with S = diag(s) a sign flip of

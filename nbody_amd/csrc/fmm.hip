@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void l2p_kernel(FmmParams F, State S, uint32_t
     S.ax[i] = axo;
     S.ay[i] = ayo;
     S.az[i] = azo;
-    S.pot[i] = -F.G * (nph + phi);
+    if (F.pot) S.pot[i] = -F.G * (nph + phi);
     if (kick_dt != 0.0) {
         S.vx[i] += (double)axo * kick_dt;
         S.vy[i] += (double)ayo * kick_dt;
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 
 // M2L, plain: one thread per target cell, the derivative tensor rebuilt per
 // pair (replaces field.cl:153-211's order-0 cell field).
-template <int P, int Q>
+template <int P, int Q, bool POT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_plain_kernel(FmmParams F,
                                                                                                   uint32_t num_nodes) {
     constexpr int NC = ncoef(P);
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
             derivs<m2l_deg(Q)>(d.x, d.y, d.z, F.eps2, D);
             // source moments read straight from memory; the pair table is ordered
             // by source coefficient so each M_n is loaded once and used at once
-            m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
+            m2l_apply<P, Q, POT>(D, F.M + (size_t)B * F.ncp, L);
         }
     }
     float* out = F.L + (size_t)A * F.ncp;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 // (target particle) x (source stride): with nt targets in a chunk, lane l
 // handles target l % nt and every (64/nt)-th particle of each source leaf, so
 // all 64 lanes work for any leaf occupancy; partial sums meet in LDS.
-template <int P>
+template <int P, bool POT>
 __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t num_leaves) {
     const int lane = lane_id();
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
                     ax = fmaf(rx, qi3, ax);
                     ay = fmaf(ry, qi3, ay);
                     az = fmaf(rz, qi3, az);
-                    ph += qi;
+                    if (POT) ph += qi;
                 }
             } else {
                 const int self = t0 + ti;
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
                     ax = fmaf(rx, qi3, ax);
                     ay = fmaf(ry, qi3, ay);
                     az = fmaf(rz, qi3, az);
-                    ph += qi;
+                    if (POT) ph += qi;
                 }
             }
         }
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(256) void p2p_kernel(FmmParams F, State S, uint32_t
             S.ax[i] = ax;
             S.ay[i] = ay;
             S.az[i] = az;
-            S.pot[i] = ph;
+            if (POT) S.pot[i] = ph;
         }
     }
     if (lane == 0 && inter) atomicAdd(F.p2p_inter, inter);
@@ -316,7 +316,8 @@ constexpr int p2p_stage_slots() { return SPL == 1 ? 96 : 160; }
 
 // SOFT: eps > 0 (each target meets itself at r = 0 with a finite softened
 // potential); else the self pair and the empty slots are skipped by index.
-template <bool SOFT, int SPL>
+// POT: the potential sums too (else the accelerations alone).
+template <bool SOFT, int SPL, bool POT>
 __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State& S, uint32_t num_leaves) {
     // (num_leaves == DEV_COUNT: the leaf count of the device counters)
     constexpr int P2P_STAGE = p2p_stage_slots<SPL>();
@@ -448,7 +449,7 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
                                 ax = fmaf(rx, qi3, ax);
                                 ay = fmaf(ry, qi3, ay);
                                 az = fmaf(rz, qi3, az);
-                                ph += qi;
+                                if (POT) ph += qi;
                             }
                         } else {
                             // zero-charge slots (index -1) are skipped like the self pair
@@ -463,7 +464,7 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
                                 ax = fmaf(rx, qi3, ax);
                                 ay = fmaf(ry, qi3, ay);
                                 az = fmaf(rz, qi3, az);
-                                ph += qi;
+                                if (POT) ph += qi;
                             }
                         }
                     }
@@ -471,7 +472,7 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
                 }
             }
             // the self term q_i/eps sits in exactly one stride's partial sum
-            if (SOFT && sp == 0) ph -= me.w * __builtin_amdgcn_rsqf(eps2);
+            if (POT && SOFT && sp == 0) ph -= me.w * __builtin_amdgcn_rsqf(eps2);
             // combine the source strides of each target
             {
                 const float ax0 = ax, ay0 = ay, az0 = az, ph0 = ph;
@@ -487,7 +488,7 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
                 S.ax[i] = ax;
                 S.ay[i] = ay;
                 S.az[i] = az;
-                S.pot[i] = ph;
+                if (POT) S.pot[i] = ph;
             }
         }
     }
@@ -495,15 +496,15 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
     if (lane == 0 && inter) atomicAdd(F.p2p_inter, inter);
 }
 
-template <int P, int SPL>
+template <int P, int SPL, bool POT>
 __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_kernel(FmmParams F, State S, uint32_t num_leaves) {
-    p2p_staged_body<true, SPL>(F, S, num_leaves);
+    p2p_staged_body<true, SPL, POT>(F, S, num_leaves);
 }
 
-template <int P, int SPL>
+template <int P, int SPL, bool POT>
 __global__ __launch_bounds__(WAVE * P2P_WAVES) void p2p_staged_unsoftened_kernel(FmmParams F, State S,
                                                                                  uint32_t num_leaves) {
-    p2p_staged_body<false, SPL>(F, S, num_leaves);
+    p2p_staged_body<false, SPL, POT>(F, S, num_leaves);
 }
 
 // ---------------------------------------------------------------------------
@@ -554,26 +555,27 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     for (int k = NCQ; k < F.dcp; ++k) row[k] = 0.f;
 }
 
-// 1024 threads (16 waves at <= 128 VGPRs) while expansion and tensor fit that
-// register budget (up to p = 4, q = 5); 256 above, with the table read from
+// The pipelined form (two entries of expansion and tensor in 256 VGPRs) up to
+// p = 4, q = 5; above, the one-entry form with 256 threads and the table in
 // global memory
-constexpr int m2l_table_threads(int p, int qc) { return ncoef(p) + ncoef(m2l_deg(qc)) <= 91 ? 1024 : 256; }
+constexpr bool m2l_pipelined(int p, int qc) { return ncoef(p) + ncoef(m2l_deg(qc)) <= 91; }
 
 // Same-level part of the rows of the level's targets [off, off+count): one
 // wavefront per target, lanes over its entries; writes L.  IN_LDS: the
 // level's table and the offset -> row map are first copied to LDS (when they
 // fit), else read from global memory.  Two forms:
-//  - PIPE, 256 threads, where two workgroups' tables fit one CU's LDS (the
-//    table holds only the offsets that occur: 273 rows, 64 KiB at theta =
-//    1/2, q = 5): 2 wavefronts per SIMD with 256 VGPRs each, every wavefront
-//    holding two entries so that one entry's loads fly while the other's
-//    products run; one workgroup computes while the other copies its table
-//    or finishes its last list;
-//  - else 1024 threads, 4 wavefronts per SIMD at <= 128 VGPRs, one entry at
-//    a time with the next source id prefetched (256 with the table in
-//    global memory where the expansion needs more registers).
-template <int P, int Q, bool IN_LDS, int NT, bool PIPE = false>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : (PIPE ? 2 : 1), NT >= 512 ? 4 : (PIPE ? 2 : 8))))
+//  - PIPE (up to p = 4, q = 5): 2 wavefronts per SIMD with 256 VGPRs each,
+//    every wavefront holding two entries so that one entry's loads fly while
+//    the other's products run.  256 threads where two workgroups' tables fit
+//    one CU's LDS (the table holds only the offsets that occur: 273 rows,
+//    64 KiB at theta = 1/2, q = 5; one workgroup computes while the other
+//    copies its table or finishes its last list) or where the table stays in
+//    global memory; 512 where one table fits;
+//  - else 256 threads, one entry at a time with the next source id
+//    prefetched, the table in global memory.
+//  POT: the products into L[0] too (the potential; else accelerations only).
+template <int P, int Q, bool IN_LDS, int NT, bool PIPE, bool POT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PIPE ? 2 : 1, PIPE ? 2 : 8)))
 void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
     constexpr int NC = ncoef(P), NCQ = ncoef(m2l_deg(Q));
     constexpr int NQD = (NCQ + 3) / 4;  // float4 per tensor row (the host sets dcp = 4 NQD)
@@ -672,7 +674,7 @@ void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
                 };
                 auto apply = [&](Ent& X) {
                     octant_signs<m2l_deg(Q)>(X.neg, X.D);
-                    m2l_apply_by_d<P, Q>(X.D, X.M, L);
+                    m2l_apply_by_d<P, Q, POT>(X.D, X.M, L);
                 };
                 // source id of entry i; past the list, that of pool entry 0
                 // (a node id, so its loads stay in bounds; the value itself
@@ -730,7 +732,7 @@ void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
                     if (4 * q + 3 < NCQ) D[4 * q + 3] = v.w;
                 }
                 octant_signs<m2l_deg(Q)>(neg, D);
-                m2l_apply_by_d<P, Q>(D, F.M + (size_t)B * F.ncp, L);
+                m2l_apply_by_d<P, Q, POT>(D, F.M + (size_t)B * F.ncp, L);
             }
             }
         }
@@ -824,7 +826,7 @@ __device__ __forceinline__ T half_sum(T v) {
     return v;
 }
 
-template <int P, int Q>
+template <int P, int Q, bool POT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 : 1))) void m2l_cross_kernel(
     FmmParams F, uint32_t num_nodes) {
     constexpr int NC = ncoef(P), NCQ = ncoef(m2l_deg(Q)), NQD = (NCQ + 3) / 4;
@@ -867,7 +869,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 
                     }
                     float D[NCQ];
                     load_tensor<m2l_deg(Q)>(X4 + (size_t)t * NQD, neg, D);
-                    m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
+                    m2l_apply<P, Q, POT>(D, F.M + (size_t)B * F.ncp, L);
                 }
             }
         }
@@ -934,7 +936,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P <= 4 ? 4 
 // over the lanes leaves each run's total in its first lane, which adds it to
 // the target's L with float atomics (a run may continue in the next
 // wavefront).
-template <int P, int Q>
+template <int P, int Q, bool POT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void m2l_far_kernel(FmmParams F,
                                                                                                 uint32_t num_nodes) {
     constexpr int NC = ncoef(P);
@@ -962,7 +964,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
             const float3 d = centre_diff(F.nodes.geo[B], ga, F.bits, unit);  // R = c_A - c_B
             float D[ncoef(m2l_deg(Q))];
             derivs<m2l_deg(Q)>(d.x, d.y, d.z, F.eps2, D);
-            m2l_apply<P, Q>(D, F.M + (size_t)B * F.ncp, L);
+            m2l_apply<P, Q, POT>(D, F.M + (size_t)B * F.ncp, L);
         }
 #pragma unroll
         for (int off = 1; off < WAVE; off <<= 1) {
@@ -1001,16 +1003,17 @@ __global__ void zero_top_kernel(float* L, const Counters* C, int level, int ncp)
         L[i] = 0.f;
 }
 
-template <int P, int Q>
 // parts: M2L_LEVELS runs the same-level kernel of level `level` (all tabled
 // levels when level < 0); M2L_REST runs everything else (initialisation, the
-// other-level and deferred pairs), after the same-level kernels.
-static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, int level, hipStream_t st) {
+// other-level and deferred pairs), after the same-level kernels.  POT: F.pot
+// (the per-pair and matrix-core kernels compute the potential always).
+template <int P, int Q, bool POT>
+static void launch_m2l_pq_pot(const FmmParams& F, const M2LPlan& plan, int parts, int level, hipStream_t st) {
     const uint32_t nn = plan.num_nodes;
     const unsigned grid = std::min<unsigned>(ceil_div(nn, 4), 65536u);
     if (plan.mode == 0) {
         if (parts & M2L_REST)
-            hipLaunchKernelGGL((m2l_plain_kernel<P, Q>), dim3(ceil_div(nn, 256)), dim3(256), 0, st, F, nn);
+            hipLaunchKernelGGL((m2l_plain_kernel<P, Q, POT>), dim3(ceil_div(nn, 256)), dim3(256), 0, st, F, nn);
         return;
     }
     if (!F.dtab) {
@@ -1031,20 +1034,21 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
             }
         }
     }
-    constexpr int TT = m2l_table_threads(P, Q);
+    constexpr bool PIPE = m2l_pipelined(P, Q);
     const int K1 = F.dK + 1;
     // rows padded to an odd float4 count, then the offset -> row map
     const size_t lds = (size_t)F.dsz * ((F.dcp / 4) | 1) * 16 + (((size_t)K1 * K1 * K1 * 2 + 15) & ~(size_t)15);
-    const bool in_lds = plan.nsys == 1 && lds <= plan.max_lds && TT == 1024;
+    const bool in_lds = PIPE && plan.nsys == 1 && lds <= plan.max_lds;
     // two workgroups of 256 per CU where two tables fit (the largest
     // workgroup LDS, max_lds, is all of a CU's LDS on MI355X)
     const bool two = in_lds && 2 * lds <= plan.max_lds;
+    const int T = in_lds && !two ? 512 : 256;
     // (set by the uncaptured evaluation that precedes every capture: not a
     // stream operation, kept out of a capture)
-    if constexpr (TT == 1024) {
+    if constexpr (PIPE) {
         if (in_lds && (parts & M2L_LEVELS) && !plan.dev) {
-            const void* k = two ? reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true, 256, true>)
-                                : reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true, 1024>);
+            const void* k = two ? reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true, 256, true, POT>)
+                                : reinterpret_cast<const void*>(&m2l_lds_kernel<P, Q, true, 512, true, POT>);
             (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         }
     }
@@ -1063,31 +1067,36 @@ static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, in
         // uneven list lengths better than a grid-stride loop or a resident
         // grid with a work queue (measured); with device counts the grid is
         // sized from plan.level_count and the kernel strides over the rest
-        const int T = two ? 256 : TT;
         const unsigned g = std::min<unsigned>(ceil_div(cnt, T / WAVE), 65535u);
         const uint32_t koff = plan.dev ? 0u : plan.level_off[l], kcnt = plan.dev ? DEV_COUNT : cnt;
         const int lv = l - F.dlevel0;
-        if constexpr (TT == 1024) {
-            if (two) {
-                hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true, 256, true>), dim3(g), dim3(256), lds, st, F, koff, kcnt,
-                                   lv);
-                continue;
-            }
-            if (in_lds) {
-                hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true, 1024>), dim3(g), dim3(1024), lds, st, F, koff, kcnt,
-                                   lv);
-                continue;
-            }
+        if constexpr (PIPE) {
+            if (two)
+                hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true, 256, true, POT>), dim3(g), dim3(256), lds, st, F, koff,
+                                   kcnt, lv);
+            else if (in_lds)
+                hipLaunchKernelGGL((m2l_lds_kernel<P, Q, true, 512, true, POT>), dim3(g), dim3(512), lds, st, F, koff,
+                                   kcnt, lv);
+            else
+                hipLaunchKernelGGL((m2l_lds_kernel<P, Q, false, 256, true, POT>), dim3(g), dim3(256), 0, st, F, koff,
+                                   kcnt, lv);
+        } else {
+            hipLaunchKernelGGL((m2l_lds_kernel<P, Q, false, 256, false, POT>), dim3(g), dim3(256), 0, st, F, koff,
+                               kcnt, lv);
         }
-        hipLaunchKernelGGL((m2l_lds_kernel<P, Q, false, TT>), dim3(g), dim3(TT), 0, st, F, koff, kcnt, lv);
     }
     if (!(parts & M2L_REST)) return;
     const uint32_t knn = plan.dev ? DEV_COUNT : nn;
-    hipLaunchKernelGGL((m2l_cross_kernel<P, Q>), dim3(std::min<unsigned>(ceil_div(nn, 8), 65536u)), dim3(256), 0, st,
-                       F, knn);
+    hipLaunchKernelGGL((m2l_cross_kernel<P, Q, POT>), dim3(std::min<unsigned>(ceil_div(nn, 8), 65536u)), dim3(256),
+                       0, st, F, knn);
     // the deferred pairs are a subset of the other-level entries (far_cap)
     const unsigned gfar = std::max<unsigned>(1u, std::min<unsigned>(ceil_div(F.far_cap, 256u), 1024u));
-    hipLaunchKernelGGL((m2l_far_kernel<P, Q>), dim3(gfar), dim3(256), 0, st, F, knn);
+    hipLaunchKernelGGL((m2l_far_kernel<P, Q, POT>), dim3(gfar), dim3(256), 0, st, F, knn);
+}
+template <int P, int Q>
+static void launch_m2l_pq(const FmmParams& F, const M2LPlan& plan, int parts, int level, hipStream_t st) {
+    if (F.pot) launch_m2l_pq_pot<P, Q, true>(F, plan, parts, level, st);
+    else launch_m2l_pq_pot<P, Q, false>(F, plan, parts, level, st);
 }
 
 template <int Q>
@@ -1122,19 +1131,26 @@ static void launch_l2l_t(const FmmParams& F, uint32_t count, uint32_t off, uint3
 }
 template <int P>
 static void launch_p2p_t(const FmmParams& F, const State& S, uint32_t num_leaves, hipStream_t st) {
-    hipLaunchKernelGGL((p2p_kernel<P>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves);
+    if (F.pot) hipLaunchKernelGGL((p2p_kernel<P, true>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves);
+    else hipLaunchKernelGGL((p2p_kernel<P, false>), dim3(ceil_div(num_leaves, 4)), dim3(256), 0, st, F, S, num_leaves);
 }
 
+template <int P, int SPL, bool POT>
+static void launch_p2p_staged_pot(const FmmParams& F, const State& S, uint32_t num_leaves, unsigned grid,
+                                  hipStream_t st) {
+    if (F.eps2 > 0.f)
+        hipLaunchKernelGGL((p2p_staged_kernel<P, SPL, POT>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S,
+                           num_leaves);
+    else
+        hipLaunchKernelGGL((p2p_staged_unsoftened_kernel<P, SPL, POT>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st,
+                           F, S, num_leaves);
+}
 template <int P, int SPL>
 static void launch_p2p_staged_t(const FmmParams& F, const State& S, uint32_t num_leaves, uint32_t items,
                                 hipStream_t st) {
     const unsigned grid = std::min<unsigned>(ceil_div(items, P2P_WAVES), 65536u);
-    if (F.eps2 > 0.f)
-        hipLaunchKernelGGL((p2p_staged_kernel<P, SPL>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F, S,
-                           num_leaves);
-    else
-        hipLaunchKernelGGL((p2p_staged_unsoftened_kernel<P, SPL>), dim3(grid), dim3(WAVE * P2P_WAVES), 0, st, F,
-                           S, num_leaves);
+    if (F.pot) launch_p2p_staged_pot<P, SPL, true>(F, S, num_leaves, grid, st);
+    else launch_p2p_staged_pot<P, SPL, false>(F, S, num_leaves, grid, st);
 }
 #define NBH_DISPATCH_P(p, call)          \
     switch (p) {                         \

@@ -137,6 +137,8 @@ struct FmmParams {
     uint32_t far_cap;
     uint32_t* far_cnt;     // [targets, pairs]
     double eps2_64;
+    int pot;               // 1: potentials too; 0: accelerations only (the tuned P2P and M2L kernels leave
+                           // out the potential's terms, l2p_kernel does not write it)
 };
 
 void launch_gather_state(const State& src, const State& dst, const uint32_t* perm, uint32_t n, hipStream_t st);

@@ -105,6 +105,11 @@ struct nbh_handle {
     int theta_num = 1, theta_den = 2;
     double time = 0.0;
     bool acc_valid = false;
+    // the evaluation being built computes potentials (FmmParams::pot); the
+    // potentials in the state are those of its positions (steps evaluate
+    // accelerations only, as the reference's step does)
+    bool eval_pot = true;
+    bool pot_valid = false;
     bool timing = false;
     int max_lds = 65536;  // LDS bytes per workgroup on this device
     uint32_t walk_grid = 1024;  // group-kernel workgroups resident at once
@@ -125,6 +130,7 @@ struct nbh_handle {
     // (forward and adjoint), dense tensor tables, candidate offsets
     DevBuf mb, mbw, d2tab, cand, hank, opar;
     DevBuf sl_offs, sl_map;  // same-level table: offsets that occur (rows), offset -> row map
+    DevBuf acc_save;         // nbh_diagnostics: the accelerations kept across its evaluation
     int sl_rows = 0;         // rows of the same-level table per level, its zero row included
     // trajectory adjoint: checkpoints of the full state (every adj_k steps),
     // the final state, the recomputed states of one segment, the cotangents
@@ -405,6 +411,7 @@ static FmmParams fmm_params(nbh_handle* h) {
     F.dtab = nullptr;
     F.Lx = nullptr;
     F.eps2_64 = h->cfg.eps * h->cfg.eps;
+    F.pot = h->eval_pot ? 1 : 0;
     return F;
 }
 
@@ -629,8 +636,11 @@ static void graph_shape_update(nbh_handle* h);
 // gs: a captured step (graph mode): the launches of the evaluation with the
 // counts read on the device, the grids, levels and buffers of gs, and no
 // read-back; the host reads the counters after the graph.
-static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false, const GraphShape* gs = nullptr) {
+static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false, const GraphShape* gs = nullptr,
+                     bool pot = true) {
     const int p = (int)h->cfg.order;
+    h->eval_pot = pot;
+    h->pot_valid = pot;
     // an evaluation outside a graph may grow (move) the buffers a graph holds
     if (!gs) graph_invalidate(h);
     record(h, 0);
@@ -947,7 +957,7 @@ static int graph_capture(nbh_handle* h) {
     int rc = graph_snapshot(h, true);
     if (rc == NBH_OK) {
         launch_integrate(h->st[h->cur].view(), (uint32_t)h->n, 0, 0.5 * dt, dt, h->stream);
-        rc = force_fmm(h, 0.5 * dt, false, &h->gshape);
+        rc = force_fmm(h, 0.5 * dt, false, &h->gshape, false);
     }
     h->cur = cur0;
     hipGraph_t graph = nullptr;
@@ -1004,11 +1014,16 @@ static int graph_step(nbh_handle* h) {
     h->stats.split_pairs = c.split_total;
     h->stats.m2l_same_pairs = c.m2l_total - c.m2lx_total;
     ++h->stats.graph_replays;
+    h->pot_valid = false;  // a captured step evaluates accelerations only
     return 1;
 }
 
-static int force_eval(nbh_handle* h, double kick_dt) {
-    return h->cfg.solver == NBH_SOLVER_FMM ? force_fmm(h, kick_dt) : force_direct(h, kick_dt);
+// pot: the potentials too (a step needs the accelerations only; the direct
+// sum computes both always)
+static int force_eval(nbh_handle* h, double kick_dt, bool pot = true) {
+    if (h->cfg.solver == NBH_SOLVER_FMM) return force_fmm(h, kick_dt, false, nullptr, pot);
+    h->pot_valid = true;
+    return force_direct(h, kick_dt);
 }
 
 static int do_step(nbh_handle* h) {
@@ -1016,25 +1031,27 @@ static int do_step(nbh_handle* h) {
     const uint32_t n = (uint32_t)h->n;
     switch (h->cfg.integrator) {
         case NBH_INT_KDK: {
-            if (!h->acc_valid) { CHK(force_eval(h, 0.0)); h->acc_valid = true; }
+            if (!h->acc_valid) { CHK(force_eval(h, 0.0, false)); h->acc_valid = true; }
             if (h->cfg.graph) {
                 const int r = graph_step(h);
                 if (r < 0) return r;
                 if (r == 1) break;
             }
             launch_integrate(h->st[h->cur].view(), n, 0, 0.5 * dt, dt, h->stream);
-            CHK(force_eval(h, 0.5 * dt));
+            CHK(force_eval(h, 0.5 * dt, false));
             break;
         }
         case NBH_INT_EULER:
-            CHK(force_eval(h, 0.0));
+            CHK(force_eval(h, 0.0, false));
             launch_integrate(h->st[h->cur].view(), n, 1, dt, dt, h->stream);
             h->acc_valid = false;
+            h->pot_valid = false;  // the positions moved after the evaluation
             break;
         default:
-            CHK(force_eval(h, 0.0));
+            CHK(force_eval(h, 0.0, false));
             launch_integrate(h->st[h->cur].view(), n, 2, dt, dt, h->stream);
             h->acc_valid = false;
+            h->pot_valid = false;
             break;
     }
     h->time += dt;
@@ -1345,7 +1362,7 @@ void nbh_destroy(nbh_handle* h) {
                      &h->row_next, &h->row_sl_start, &h->row_sl_cnt, &h->row_m_start, &h->row_m_cnt,
                      &h->row_mx_cnt, &h->row_p_start, &h->row_p_cnt, &h->m2l_pool, &h->p2p_pool, &h->sl, &h->counters, &h->diag,
                      &h->tmp_small, &h->Mw, &h->Lw, &h->pw, &h->wbuf, &h->gbuf, &h->mb, &h->mbw, &h->d2tab,
-                     &h->cand, &h->hank, &h->opar, &h->sl_offs, &h->sl_map};
+                     &h->cand, &h->hank, &h->opar, &h->sl_offs, &h->sl_map, &h->acc_save};
     for (DevBuf* b : all) b->release();
     for (DevBuf& b : h->adj_ckpt) b.release();
     for (DevBuf& b : h->adj_seg) b.release();
@@ -1435,6 +1452,7 @@ int nbh_set_state(nbh_handle* h, const double* pos, const double* vel, double t)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->time = t;
     h->acc_valid = false;
+    h->pot_valid = false;
     return NBH_OK;
 }
 
@@ -1488,15 +1506,16 @@ int nbh_autotune(nbh_handle* h, const uint32_t* caps, size_t ncaps, const uint32
             h->cfg.p2p_chunk = ks[j];
             // one untimed evaluation (buffers grow to this tree), then the
             // timed ones; each evaluation ends with a host synchronisation
-            int rc = force_eval(h, 0.0);
+            int rc = force_eval(h, 0.0, false);
             const auto t0 = std::chrono::steady_clock::now();
-            for (int e = 0; e < evals && rc == NBH_OK; ++e) rc = force_eval(h, 0.0);
+            for (int e = 0; e < evals && rc == NBH_OK; ++e) rc = force_eval(h, 0.0, false);
             const double ms =
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / evals;
             if (rc != NBH_OK) {
                 h->cfg.leaf_capacity = cap0;
                 h->cfg.p2p_chunk = chunk0;
                 h->acc_valid = false;
+                h->pot_valid = false;
                 return rc;
             }
             if (ms_table) ms_table[i * ks.size() + j] = ms;
@@ -1621,6 +1640,31 @@ int nbh_direct_fp64_noalloc(nbh_handle* h, size_t nt, const int64_t* targets, do
 int nbh_diagnostics(nbh_handle* h, double* out) {
     if (!h || !out) return h ? h->fail(NBH_E_INVALID, "out is NULL") : NBH_E_INVALID;
     HIPCHK(h, hipSetDevice(h->device));
+    if (!h->pot_valid) {
+        // steps evaluate accelerations only: one evaluation with potentials
+        // at the current positions, the accelerations the integrator holds
+        // kept as they were (the evaluation rewrites them, in another
+        // summation order)
+        // (the positions are already in the order the evaluation sorts
+        // them into, so the saved accelerations go back in the same order,
+        // into whichever state buffer is current afterwards)
+        const size_t n = h->n;
+        HIPCHK(h, h->acc_save.ensure(n * 12));
+        char* o = (char*)h->acc_save.p;
+        {
+            const StateBufs& sb = h->st[h->cur];
+            const DevBuf* a[3] = {&sb.ax, &sb.ay, &sb.az};
+            for (int k = 0; k < 3; ++k)
+                HIPCHK(h, hipMemcpyAsync(o + k * n * 4, a[k]->p, n * 4, hipMemcpyDeviceToDevice, h->stream));
+        }
+        CHK(force_eval(h, 0.0, true));
+        {
+            const StateBufs& sb = h->st[h->cur];
+            const DevBuf* a[3] = {&sb.ax, &sb.ay, &sb.az};
+            for (int k = 0; k < 3; ++k)
+                HIPCHK(h, hipMemcpyAsync(a[k]->p, o + k * n * 4, n * 4, hipMemcpyDeviceToDevice, h->stream));
+        }
+    }
     const uint32_t nb = diagnostics_blocks((uint32_t)h->n);
     launch_diagnostics(h->st[h->cur].view(), (uint32_t)h->n, h->diag.as<double>(), nb, h->stream);
     std::vector<double> part((size_t)nb * 8);
@@ -1663,6 +1707,7 @@ static int vjp_device(nbh_handle* h, const double* w_dev) {
     // tree, lists and mass expansions at the current positions
     CHK(force_fmm(h, 0.0, true));
     h->acc_valid = false;
+    h->pot_valid = false;
     const int p = (int)h->cfg.order;
     const size_t ncp = (size_t)ncp_of(p);
     HIPCHK(h, h->Mw.ensure((size_t)h->num_nodes * ncp * 4));
@@ -1728,6 +1773,7 @@ static int snapshot_restore(nbh_handle* h, const DevBuf& src, double t) {
     for (DevBuf* b : d4) { HIPCHK(h, hipMemcpyAsync(b->p, o, n * 4, hipMemcpyDeviceToDevice, h->stream)); o += n * 4; }
     h->time = t;
     h->acc_valid = true;  // the snapshot carries the accelerations of its positions
+    h->pot_valid = false;  // (and the potentials only if that evaluation computed them)
     return NBH_OK;
 }
 
@@ -1768,7 +1814,7 @@ int nbh_adjoint_forward(nbh_handle* h, int nsteps, int ckpt_every) {
     if ((int)h->adj_ckpt.size() < nck) h->adj_ckpt.resize(nck);
     h->adj_ckpt_t.assign(nck, 0.0);
     h->adj_ready = false;
-    if (!h->acc_valid) { CHK(force_eval(h, 0.0)); h->acc_valid = true; }
+    if (!h->acc_valid) { CHK(force_eval(h, 0.0, false)); h->acc_valid = true; }
     for (int s = 0; s < nsteps; ++s) {
         if (s % ckpt_every == 0) {
             CHK(snapshot_save(h, h->adj_ckpt[s / ckpt_every]));

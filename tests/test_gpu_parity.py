@@ -29,6 +29,9 @@ CASES = [
     ("uniform4096_cap16", "uniform", 4096, 16, 4, (1, 2), 1e-2),
     ("plummer20k_cap32", "plummer", 20000, 32, 4, (1, 2), 1e-3),
     ("plummer8k_cap8_p6_theta04", "plummer", 8192, 8, 6, (2, 5), 5e-4),
+    # p = 4 at theta = 0.4: the same-level table (483 offsets, 116 KiB)
+    # fits one workgroup per CU, the 512-thread form of the kernel
+    ("plummer8k_cap16_p4_theta04", "plummer", 8192, 16, 4, (2, 5), 5e-4),
     ("uniform3000_cap1_p2", "uniform", 3000, 1, 2, (1, 2), 1e-2),
     ("uniform2048_cap16_unsoftened", "uniform", 2048, 16, 4, (1, 2), 0.0),
     # P2P lane layouts at the ends: leaves of 33..48 particles (one source
