@@ -56,10 +56,10 @@ def ncoef(p):
 
 
 def same_level_kernel(p, qc, num, den, lds_per_cu=160 * 1024):
-    """Name of the library's same-level M2L kernel with the table in LDS: the
-    pipelined 256-thread form where two workgroups' tables (the same-level
-    offsets that occur, a zero row and the offset map) fit one CU's LDS, else
-    the 1024-thread form."""
+    """Name of the library's same-level M2L kernel of a step (accelerations
+    only) on one system at p <= 4: the pipelined form with 256 threads where
+    two workgroups' tables (the same-level offsets that occur, a zero row and
+    the offset map) fit one CU's LDS, 512 where one does."""
     q = qc & 15
     K = 2 * int(math.floor(math.sqrt(3.0) * den / num)) + 1
     acc = lambda x, y, z: num * num * (x * x + y * y + z * z) > 3 * den * den
@@ -69,8 +69,9 @@ def same_level_kernel(p, qc, num, den, lds_per_cu=160 * 1024):
                            for cc in itertools.product((0, 1), repeat=3)):
             rows.add(tuple(abs(v) for v in o))
     lds = (len(rows) + 1) * (((ncoef(q) + 3) // 4) | 1) * 16 + ((K + 1) ** 3 * 2 + 15) // 16 * 16
-    two = 2 * lds <= lds_per_cu
-    return f"m2l_lds_kernel<{p}, {qc}, true, {256 if two else 1024}, {'true' if two else 'false'}>"
+    if lds > lds_per_cu:
+        return f"m2l_lds_kernel<{p}, {qc}, false, 256, true, false>"
+    return f"m2l_lds_kernel<{p}, {qc}, true, {256 if 2 * lds <= lds_per_cu else 512}, true, false>"
 
 
 def measured_traffic(kernel, workload):
@@ -561,7 +562,8 @@ def main():
     m2l_tf = m2l_flop(p, q, terms, pot=False) * m2l_pairs / K / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     workload = f"plummer_N{n}_seed1_eps1e-3_theta0.5_cap32_p{p}_kdk_dt1e-3"
     chunk = int(sim.config.p2p_chunk) or 64
-    p2p_kernel = f"p2p_staged_kernel<{p}, {chunk // 64}>" if args.p2p == "lane" else f"p2p_kernel<{p}>"
+    # (the variants of a step: accelerations only)
+    p2p_kernel = f"p2p_staged_kernel<{p}, {chunk // 64}, false>" if args.p2p == "lane" else f"p2p_kernel<{p}, false>"
     qc = (q | 16) if (terms == "all" and p < q < 2 * p) else q
     mfma = args.m2l in ("bf16x3", "mfma_f32")
     same_kernel = (f"m2l_offset_kernel<{p}, {'true' if args.m2l == 'bf16x3' else 'false'}>" if mfma
