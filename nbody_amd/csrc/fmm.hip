@@ -627,13 +627,9 @@ void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
         {
             const uint32_t s0 = F.lists.m_start[r];
             const uint32_t e1 = s0 + F.lists.m_cnt[r] - F.lists.mx_cnt[r];
-            // the lane's next source id is loaded before this entry's
-            // products, so that of the dependent loads pool -> geo ->
-            // moments only two wait per entry.  The products run in tensor
-            // coefficient order (m2l_apply_by_d): the tensor streams in from
-            // LDS and the registers freed hold the loads in flight (together
-            // 6% off the step against source-coefficient order with no
-            // prefetch; either alone gains 0-2%)
+            // The products run in tensor coefficient order (m2l_apply_by_d):
+            // the tensor streams in and the registers it held take loads in
+            // flight.
             if constexpr (PIPE) {
                 // two entries in registers: while one entry's products run,
                 // the next entry's tensor (LDS) and moments (global) are
@@ -712,6 +708,11 @@ void m2l_lds_kernel(FmmParams F, uint32_t off, uint32_t count, int lvl) {
                     e += WAVE;
                 }
             } else {
+            // one entry at a time; the lane's next source id is loaded before
+            // this entry's products, so that of the dependent loads pool ->
+            // geo -> moments only two wait per entry (with the tensor order,
+            // 6% off the step of the time against source-coefficient order
+            // and no prefetch; either alone gained 0-2%)
             uint32_t e = s0 + lane;
             int Bn = e < e1 ? F.lists.m_pool[e] : 0;
             for (; e < e1; e += WAVE) {
