@@ -9,7 +9,9 @@ checked against the FP64 direct sum on 4096 random targets; then W warm-up
 steps and K timed steps.  The plain FP32 port (same operator, untuned kernels)
 is timed beside it for 3 steps.
 
-Prints one JSON line.  The library runs three streams: the moments and the
+Prints one JSON line.  The timed steps run without the library's per-phase
+HIP events; `phases_ms` comes from a phase pass of --phase-steps steps after
+them with the events on.  The library runs four streams: the moments and the
 same-level M2L run beside the walk, P2P beside M2L and L2L, so `phases_ms`
 overlap (`overlapped` names what each ran beside) and only `total` is wall
 time.  The roofline entry is for the dominant kernel, the same-level M2L
@@ -255,7 +257,7 @@ def workload_uniform4096(nb, args):
                                 graph=graph)
         sim = nb.HipSimulation.from_arrays(x, v, m, m, config=cfg)
         err = force_error(sim, n, n, 0)
-        agg, wall, *_ = time_steps(sim, 2, K, timing=not graph)
+        agg, wall, *_ = time_steps(sim, 2, K, timing=False)
         st = sim.stats()
         sim.close()
         res[label] = {"particle_steps_per_s": n * K / wall, "ms_per_step": wall / K * 1e3, "force_err": err}
@@ -362,7 +364,10 @@ def workload_ensemble(nb, args):
         pos += len(t)
         errs.append(float(np.sqrt(np.sum((a - r) ** 2) / np.sum(r ** 2))))
     K = 20
-    agg, wall, *_ = time_steps(sim, 2, K)
+    _, wall, *_ = time_steps(sim, 2, K, timing=False)
+    # phases from a pass with the library's per-phase events on
+    PK = 5
+    agg = time_steps(sim, 0, PK)[0]
     sim.close()
     # the same tuned configuration with captured steps (timing off)
     gcfg = nb.default_config(G=1.0, eps=1e-2, theta=0.5, dt=1e-3, leaf_capacity=cap, p2p_chunk=chunk, order=4,
@@ -382,7 +387,7 @@ def workload_ensemble(nb, args):
             "graph": {"ms_per_step": gwall / K * 1e3, "particle_steps_per_s": n * K / gwall,
                       **{k: gst[k] for k in ("graph_captures", "graph_replays", "graph_fallbacks")}},
             "force_err": {"per_system_rms_median": float(np.median(errs)), "per_system_rms_max": float(np.max(errs))},
-            "phases_ms": {k[3:]: agg[k] / K for k in agg}}
+            "phases_ms": {k[3:]: agg[k] / PK for k in agg}, "phases_source": f"phase pass ({PK} steps)"}
 
 
 def workload_adjoint(nb, args):
@@ -470,10 +475,15 @@ def main():
                     help="P2P tile: source particles staged per wavefront round (auto: nbh_autotune picks 64 or "
                          "128 on the workload before the warm-up; on this workload it picks 128, by ~1%%)")
     ap.add_argument("--graph", action="store_true",
-                    help="captured steps (nbh_config.graph = 1) in the timed steps, with the library's timing "
-                         "off (phases_ms then comes from the roofline pass alone)")
+                    help="captured steps (nbh_config.graph = 1) in the timed steps (the phase pass after them "
+                         "runs uncaptured steps, since per-phase events keep steps out of a graph)")
+    ap.add_argument("--phase-timing", action="store_true",
+                    help="the library's per-phase HIP events on in the timed steps (default: off, and phases_ms "
+                         "comes from a phase pass of --phase-steps steps after them)")
     ap.add_argument("--no-phase-timing", action="store_true",
-                    help="the library's per-phase HIP events off in the timed steps (as with --graph)")
+                    help="(the default; kept for old command lines)")
+    ap.add_argument("--phase-steps", type=int, default=5,
+                    help="steps of the phase pass (per-phase events on) after the timed steps")
     ap.add_argument("--serial", action="store_true",
                     help="run every kernel of an evaluation on one stream in the timed steps too")
     ap.add_argument("--roofline-steps", type=int, default=3,
@@ -540,9 +550,17 @@ def main():
 
     K = args.steps
     tot = {}
-    timed = time_steps(sim, args.warmup, K, serial=args.serial, totals=tot,
-                       timing=not (args.graph or args.no_phase_timing))
+    # the timed steps run without the library's per-phase HIP events (about
+    # 1% of a step: each event is a marker in its stream); a phase pass after
+    # them, streams as in the timed steps, gives phases_ms
+    timing = args.phase_timing and not args.graph
+    timed = time_steps(sim, args.warmup, K, serial=args.serial, totals=tot, timing=timing)
     gstats = {k: sim.stats()[k] for k in ("graph_captures", "graph_replays", "graph_fallbacks")}
+    PK, ptot = K, tot
+    pagg = timed[0]
+    if not timing and args.phase_steps > 0:
+        PK, ptot = args.phase_steps, {}
+        pagg = time_steps(sim, 0, PK, serial=args.serial, totals=ptot, timing=True)[0]
     # the roofline pass: R more steps with every kernel of an evaluation on
     # one stream, so that each event pair times one kernel running alone (in
     # the timed steps the same-level M2L launches of successive levels, the
@@ -555,9 +573,9 @@ def main():
     terms = sim.m2l_terms
     q_same = int(sim.stats()["m2l_same_order"]) or q
     sim.close()
-    agg, wall, inter, m2l_pairs, split_pairs, p2p_pairs, st = timed
+    _, wall, inter, m2l_pairs, split_pairs, p2p_pairs, st = timed
     value = n * K / wall
-    m2l_ms = agg["ms_m2l"] / K
+    m2l_ms = pagg["ms_m2l"] / PK
     # the timed steps evaluate accelerations only (no potential terms)
     m2l_tf = m2l_flop(p, q, terms, pot=False) * m2l_pairs / K / (m2l_ms * 1e-3) / 1e12 if m2l_ms > 0 else 0.0
     workload = f"plummer_N{n}_seed1_eps1e-3_theta0.5_cap32_p{p}_kdk_dt1e-3"
@@ -588,7 +606,8 @@ def main():
     # the dominant kernel is the same-level M2L (most of the step's FLOP);
     # P2P is the second entry
     src, S, tt = ("roofline pass (serialized, after the timed steps)", R, rl) if R > 0 else \
-                 ("timed steps" + (" (serialized)" if args.serial else " (streams overlapped)"), K, tot)
+                 (("timed steps" if PK == K and ptot is tot else "phase pass") +
+                  (" (serialized)" if args.serial else " (streams overlapped)"), PK, ptot)
     same_launches = tt["m2l_same_launches"] / S
     roof = kernel_roofline(src, S, same_kernel, same_flop * tt["m2l_same_pairs"] / S, tt["ms_m2l_same"] / S,
                            same_launches, "mfma" if mfma else "fp32-valu",
@@ -600,14 +619,16 @@ def main():
                                      tt["ms_p2p"] / S, 1, "fp32-valu", measured_traffic(p2p_kernel, workload))
     roof["second"].update({"phase": "P2P near field", "flop_per_pair": P2P_FLOP,
                            "pairs_per_step": tt["p2p_interactions"] / S})
-    # the same kernels inside the timed steps (streams overlapped unless --serial)
+    # the same kernels in steps like the timed ones (streams overlapped unless
+    # --serial), from the phase pass when the timed steps ran without events
     roof["in_timed_steps"] = {
-        "same_level_m2l_ms_per_step": tot["ms_m2l_same"] / K,
-        "same_level_m2l_achieved": same_flop * tot["m2l_same_pairs"] / K / (tot["ms_m2l_same"] / K * 1e-3) / 1e12
-        if tot["ms_m2l_same"] > 0 else None,
-        "p2p_ms_per_step": tot["ms_p2p"] / K,
-        "p2p_achieved": P2P_FLOP * tot["p2p_interactions"] / K / (tot["ms_p2p"] / K * 1e-3) / 1e12
-        if tot["ms_p2p"] > 0 else None}
+        "source": "timed steps" if ptot is tot else f"phase pass ({PK} steps after the timed steps)",
+        "same_level_m2l_ms_per_step": ptot["ms_m2l_same"] / PK,
+        "same_level_m2l_achieved": same_flop * ptot["m2l_same_pairs"] / PK / (ptot["ms_m2l_same"] / PK * 1e-3)
+        / 1e12 if ptot["ms_m2l_same"] > 0 else None,
+        "p2p_ms_per_step": ptot["ms_p2p"] / PK,
+        "p2p_achieved": P2P_FLOP * ptot["p2p_interactions"] / PK / (ptot["ms_p2p"] / PK * 1e-3) / 1e12
+        if ptot["ms_p2p"] > 0 else None}
     # algorithmic bytes of the HBM-bound phases (each input read once, each
     # output written once), per step
     nodes, leaves = st["num_nodes"], st["num_leaves"]
@@ -624,7 +645,7 @@ def main():
     }
     phase_roofline = {}
     for ph, b in phase_bytes.items():
-        ms = agg["ms_" + ph] / K
+        ms = pagg["ms_" + ph] / PK
         gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         phase_roofline[ph] = {"bound": "hbm", "bytes": b, "ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
     phase_roofline["m2l"] = {"bound": "fp32-valu", "flop_per_pair": m2l_flop(p, q, terms, pot=False), "ms": m2l_ms,
@@ -652,7 +673,8 @@ def main():
         "roofline": roof,
         "autotune": tuned,
         "graph": gstats if args.graph else None,
-        "phases_ms": {k[3:]: agg[k] / K for k in agg},
+        "phases_ms": {k[3:]: pagg[k] / PK for k in pagg},
+        "phases_source": "timed steps" if ptot is tot else f"phase pass ({PK} steps after the timed steps)",
         "overlapped": {} if args.serial else {"moments": "walk", "m2l": "walk+p2p", "p2p": "m2l+l2l"},
         "phase_roofline": phase_roofline,
         "m2l": {"pairs_per_step": m2l_pairs / K, "same_level_pairs_per_step": tot["m2l_same_pairs"] / K,

@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -454,7 +455,12 @@ static M2LPlan graph_plan(const nbh_handle* h, const GraphShape& gs);
 
 // gs: a captured step: one attempt with the grids and buffers of gs, no
 // read-back (an overflow is flagged on the device and found after the step)
-static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr, const GraphShape* gs = nullptr) {
+// side: enqueues the side stream's work of the evaluation (moments and
+// tables, ending with ev_side), called once, after the first walk depths are
+// queued: the walk needs only the tree, so the main stream starts it while the
+// host is still enqueueing the side stream's launches
+static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr, const GraphShape* gs = nullptr,
+                    const std::function<int()>* side = nullptr) {
     Counters* dC = h->counters.as<Counters>();
     const uint32_t nn = h->num_nodes;
     const uint32_t depth = gs ? gs->depth_cap : h->depth;
@@ -492,6 +498,14 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr, const GraphShap
     W.bits = h->bits;
     W.num4 = 4u * h->theta_num * h->theta_num;
     W.den3 = 3u * h->theta_den * h->theta_den;
+    bool side_done = false;
+    // the first walk depths are queued before the side stream's launches
+    const uint32_t side_after = std::min<uint32_t>(depth, 2);
+    auto run_side = [&]() -> int {
+        if (side_done) return NBH_OK;
+        side_done = true;
+        return side ? (*side)() : NBH_OK;
+    };
     for (int attempt = 0; attempt < 12; ++attempt) {
         W.row_cap = h->row_cap;
         W.row_node = h->row_node.as<int>();
@@ -518,14 +532,29 @@ static int run_walk(nbh_handle* h, const EarlyM2L* em = nullptr, const GraphShap
             Fe = fmm_params(h);  // this attempt's rows and pools
             copy_tables(Fe, em->T);
             plan = gs ? graph_plan(h, *gs) : m2l_plan(h);
-            HIPCHK(h, hipStreamWaitEvent(h->stream3, h->ev_side, 0));  // moments and tables
-            HIPCHK(h, hipStreamWaitEvent(h->stream4, h->ev_side, 0));
         }
+        // the same-level streams wait for the moments and tables (ev_side,
+        // recorded by the side stream's work)
+        auto fork_same = [&]() -> int {
+            if (!early) return NBH_OK;
+            HIPCHK(h, hipStreamWaitEvent(h->stream3, h->ev_side, 0));
+            HIPCHK(h, hipStreamWaitEvent(h->stream4, h->ev_side, 0));
+            return NBH_OK;
+        };
+        if (side_done) CHK(fork_same());
         // reset the walk part of the counters
         zero_u32(&dC->carried[0], (offsetof(Counters, nonfinite) - offsetof(Counters, carried)) / 4, h->stream);
         launch_walk_root(W, h->stream);
+        if (!side_done && side_after == 0) {
+            CHK(run_side());
+            CHK(fork_same());
+        }
         for (uint32_t s = 1; s <= depth; ++s) {
             launch_walk_group(W, (int)s, grid[s], h->stream);
+            if (!side_done && s == side_after) {
+                CHK(run_side());
+                CHK(fork_same());
+            }
             if (early && (int)s >= Fe.dlevel0 && (int)s < Fe.dlevel0 + Fe.dnlev && (gs || h->hc->level_count[s])) {
                 hipStream_t sl = (s & 1) ? h->stream4 : h->stream3;
                 HIPCHK(h, hipEventRecord(h->ev_depth[s], h->stream));
@@ -649,31 +678,24 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false, const 
     FmmParams F = fmm_params(h);
     State S = h->st[h->cur].view();
     const uint32_t depth = gs ? gs->depth_cap : h->depth;  // levels 0..depth
-    // side stream, overlapping the walk: moments (P2M, M2M) and the tables
+    // side stream, overlapping the walk: moments (P2M, M2M) and the tables.
+    // The host part (table parameters, buffers) comes first; the launches are
+    // in `side`, which run_walk enqueues once the walk's first depths are
+    // queued (the walk needs only the tree)
     HIPCHK(h, hipEventRecord(h->ev_fork, h->stream));
-    HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
-    record_side(h, 0);
-    if (gs) {
-        launch_p2m(p, F, S, DEV_COUNT, h->stream2, gs->leaf_items);
-        for (int l = (int)depth - 1; l >= 0; --l)
-            launch_m2m_level(p, F, l, DEV_COUNT, 0, h->stream2, gs->level_items[l]);
-    } else {
-        launch_p2m(p, F, S, h->num_leaves, h->stream2);
-        for (int l = (int)depth - 1; l >= 0; --l)
-            launch_m2m_level(p, F, l, h->hc->level_count[l], h->hc->level_off[l], h->stream2);
-    }
-    record_side(h, 1);
     const int q = m2l_q(h);
     const bool tables = h->cfg.m2l != NBH_M2L_PLAIN && depth >= 2;
     FmmParams T{};  // the table part of the parameters
+    int K = 0, nlev = 0, E = 0, xn = 0;
+    const bool off = tables && use_offset_m2l(h), bf3 = h->cfg.m2l == NBH_M2L_BF16X3;
     if (tables) {
         // derivative tensors of the same-level offsets [0, K]^3 of levels
         // 2..depth (cells of levels 0 and 1 are never well separated), per
         // system, since the tensors depend on each system's root size
         const int Kp = (int)std::floor(std::sqrt(3.0) * h->theta_den / h->theta_num);
-        const int K = 2 * Kp + 1;
+        K = 2 * Kp + 1;
         const int dsz = h->sl_rows, dcp = ncp_of(m2l_deg(q));
-        const int nlev = (int)depth - 1;
+        nlev = (int)depth - 1;
         HIPCHK(h, h->dtab.ensure((size_t)dsz * nlev * dcp * 4 * h->nsys));
         T = F;
         T.dtab = h->dtab.as<float>();
@@ -684,37 +706,56 @@ static int force_fmm(nbh_handle* h, double kick_dt, bool for_vjp = false, const 
         T.dsz = dsz;
         T.doffs = h->sl_offs.as<uint32_t>();
         T.dmap = h->sl_map.as<uint16_t>();
-        launch_dtable(m2l_deg(q), T, T.dtab, 2, nlev, K + 1, 0, 1, 0, (int)h->nsys, T.doffs, dsz, h->stream2);
         // pairs one level apart, by finer level 1..depth: offsets |d_i| <=
         // 4 Kp + 1 half finer cells (the coarser side's parent pair was
         // split), tabled at d_i = 1, 3, .., 4 Kp + 1
-        const int E = 2 * Kp + 1;
-        const int xn = (int)depth;
+        E = 2 * Kp + 1;
+        xn = (int)depth;
         HIPCHK(h, h->xtab.ensure((size_t)E * E * E * xn * dcp * 4 * h->nsys));
         T.xtab = h->xtab.as<float>();
         T.xE = E;
         T.xlevel0 = 1;
         T.xnlev = xn;
-        launch_dtable(m2l_deg(q), T, T.xtab, 1, xn, E, 1, 2, 1, (int)h->nsys, nullptr, 0, h->stream2);
-        if (use_offset_m2l(h)) {
+        if (off) {
             // the matrix-core same-level M2L: signed split moments and the
             // dense tensors (orders <= 2p) of the offsets [0, K]^3, levels 2..depth
-            const bool bf3 = h->cfg.m2l == NBH_M2L_BF16X3;
             HIPCHK(h, h->mb.ensure(off_moment_bytes(p, bf3, h->num_nodes)));
             const size_t cube = (size_t)(K + 1) * (K + 1) * (K + 1);  // every offset of [0, K]^3
             HIPCHK(h, h->d2tab.ensure(cube * nlev * off_table_words(p) * 4));
-            launch_off_moments(p, bf3, F.M, F.ncp, h->num_nodes, h->geo.as<int4>(), h->box.as<double4>(), h->mb.p, h->stream2);
-            launch_off_dtable(p, bf3, h->box.as<double4>(), h->cfg.eps * h->cfg.eps, 2, nlev, K + 1,
-                              h->d2tab.as<uint32_t>(), h->stream2);
         }
     }
-    HIPCHK(h, hipEventRecord(h->ev_side, h->stream2));
+    const std::function<int()> side = [&]() -> int {
+        HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+        record_side(h, 0);
+        if (gs) {
+            launch_p2m(p, F, S, DEV_COUNT, h->stream2, gs->leaf_items);
+            for (int l = (int)depth - 1; l >= 0; --l)
+                launch_m2m_level(p, F, l, DEV_COUNT, 0, h->stream2, gs->level_items[l]);
+        } else {
+            launch_p2m(p, F, S, h->num_leaves, h->stream2);
+            for (int l = (int)depth - 1; l >= 0; --l)
+                launch_m2m_level(p, F, l, h->hc->level_count[l], h->hc->level_off[l], h->stream2);
+        }
+        record_side(h, 1);
+        if (tables) {
+            launch_dtable(m2l_deg(q), T, T.dtab, 2, nlev, K + 1, 0, 1, 0, (int)h->nsys, T.doffs, T.dsz, h->stream2);
+            launch_dtable(m2l_deg(q), T, T.xtab, 1, xn, E, 1, 2, 1, (int)h->nsys, nullptr, 0, h->stream2);
+            if (off) {
+                launch_off_moments(p, bf3, F.M, F.ncp, h->num_nodes, h->geo.as<int4>(), h->box.as<double4>(),
+                                   h->mb.p, h->stream2);
+                launch_off_dtable(p, bf3, h->box.as<double4>(), h->cfg.eps * h->cfg.eps, 2, nlev, K + 1,
+                                  h->d2tab.as<uint32_t>(), h->stream2);
+            }
+        }
+        HIPCHK(h, hipEventRecord(h->ev_side, h->stream2));
+        return NBH_OK;
+    };
     EarlyM2L em;
     em.on = tables;
     em.p = p;
     em.q = q;
     em.T = T;
-    CHK(run_walk(h, &em, gs));
+    CHK(run_walk(h, &em, gs, &side));
     record(h, 2);
     F = fmm_params(h);
     if (tables) {
