@@ -149,12 +149,23 @@ __global__ __launch_bounds__(RS_THREADS) void radix_hist_kernel(const uint64_t* 
                                                                 uint32_t* totals) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
-    __syncthreads();
+    // every key of the thread is loaded before the first is counted: the
+    // loads are in flight together (counted one by one, each waited for the
+    // one before: 16 memory round trips in a row)
     const uint32_t base = blockIdx.x * RS_TILE;
+    uint32_t d[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        uint32_t i = base + r * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        // (unconditional loads, clamped into the array: a load under a
+        // branch is waited for at the branch's end)
+        const uint32_t i = base + r * RS_THREADS + threadIdx.x;
+        const uint64_t k = keys[min(i, n - 1)];
+        d[r] = i < n ? (uint32_t)(k >> shift) & 255u : 256u;
     }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r)
+        if (d[r] < 256u) atomicAdd(&h[d[r]], 1u);
     __syncthreads();
     const uint32_t c = h[threadIdx.x];
     counts[threadIdx.x * nblocks + blockIdx.x] = c;
@@ -238,11 +249,21 @@ __global__ __launch_bounds__(RS_THREADS) void radix_scatter_kernel(const uint64_
     __syncthreads();
     const uint32_t base = blockIdx.x * RS_TILE;
     const uint64_t lt = lanemask_lt();
+    // the tile's keys and values loaded up front (all in flight together)
+    uint64_t kr[RS_ROUNDS];
+    uint32_t vr[RS_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const uint32_t i = base + r * RS_THREADS + tid;
+        kr[r] = i < n ? kin[i] : 0;
+        vr[r] = i < n ? vin[i] : 0;
+    }
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
         const uint32_t i = base + r * RS_THREADS + tid;
         const bool valid = i < n;
-        uint64_t k = valid ? kin[i] : 0;
-        uint32_t v = valid ? vin[i] : 0;
+        const uint64_t k = kr[r];
+        const uint32_t v = vr[r];
         const uint32_t d = (uint32_t)(k >> shift) & 255u;
         uint64_t m = __ballot(valid);
 #pragma unroll
