@@ -553,7 +553,8 @@ def main():
     # the timed steps run without the library's per-phase HIP events (about
     # 1% of a step: each event is a marker in its stream); a phase pass after
     # them, streams as in the timed steps, gives phases_ms
-    timing = args.phase_timing and not args.graph
+    # (--serial needs them: the library serializes its streams in timing mode 2)
+    timing = (args.phase_timing or args.serial) and not args.graph
     timed = time_steps(sim, args.warmup, K, serial=args.serial, totals=tot, timing=timing)
     gstats = {k: sim.stats()[k] for k in ("graph_captures", "graph_replays", "graph_fallbacks")}
     PK, ptot = K, tot
