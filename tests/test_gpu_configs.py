@@ -128,6 +128,43 @@ def test_single_particle_and_root_leaf():
     sim.close()
 
 
+@pytest.mark.parametrize("n,seed,depth,has_m2l", [(30, 11, 1, False), (150, 12, 2, True)])
+def test_shallow_trees_match_oracle(n, seed, depth, has_m2l):
+    """Trees of depth 1 (no tables: P2P only) and 2 (the same-level kernel of
+    level 2 alone): the side stream's moments and tables are queued after the
+    walk's first depths, the cases where the walk has fewer depths than that
+    or exactly as many.  Lists bit-exact, forces against the FP64 oracle FMM,
+    then three KDK steps (dt 1e-3, from rest) against the FP64 direct-sum
+    integrator: at N=150 the FMM's forces differ from the direct sum by up to
+    2.7e-4 of a particle's force (oracle, same tree), so the velocities by up
+    to that share of the largest and the positions (moved ~3e-5) by ~1e-8."""
+    _case(f"shallow_uniform{n}_depth{depth}")
+    x, v, m = nb.ics.uniform(n, seed)
+    eps = 1e-2
+    sim = make_sim(x, v, m, leaf_capacity=8, eps=eps, order=4, dt=1e-3)
+    acc, _ = sim.forces()
+    sizes = sim.debug_sizes()
+    assert sizes["depth"] == depth and (sizes["m2l"] > 0) == has_m2l, sizes
+    keys, ids, box = sim.debug_keys()
+    keys_o, box_o = po.morton(x)
+    order = np.argsort(keys_o, kind="stable")
+    assert np.array_equal(ids, order.astype(np.uint32))
+    tree = po.build_tree(keys_o[order], cap=8, max_depth=21)
+    lists = po.walk(tree, 1, 2)
+    mo, ms, pofs, ps = sim.debug_lists()
+    assert np.array_equal(mo, lists.m2l_off) and np.array_equal(ms, lists.m2l_src)
+    assert np.array_equal(pofs, lists.p2p_off) and np.array_equal(ps, lists.p2p_src)
+    acc_o, _ = po.fmm(tree, lists, 4, x[ids], m[ids], box_o, G=1.0, eps=eps, m2l_order=sim.m2l_order,
+                      m2l_terms=sim.m2l_terms, same_level_order=sim.same_level_order)
+    assert rms_rel(acc[ids], acc_o) < 2e-5
+    assert max_rel_floor(acc[ids], acc_o) < 1e-3
+    sim.step(3)
+    xs, vs = sim.state()[:2]
+    xd, vd = po.integrate_direct(x, v, m, m, 1.0, eps, 1e-3, 3)
+    assert np.max(np.abs(xs - xd)) < 1e-7 and np.max(np.abs(vs - vd)) / np.max(np.abs(vd)) < 2e-3
+    sim.close()
+
+
 def test_coincident_particles_reach_depth_limit():
     _case("edge_coincident")
     n = 200
