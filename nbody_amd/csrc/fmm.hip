@@ -348,14 +348,7 @@ __device__ __forceinline__ void p2p_staged_body(const FmmParams& F, const State&
     }
     const uint32_t nw = gridDim.x * P2P_WAVES;
     unsigned long long inter = 0;  // particle pairs, summed over the lanes at the end
-    // Workgroups are dealt to the 8 XCDs in turn (block b to XCD b mod 8), each
-    // XCD with its own L2.  Block b takes the leaves of block (b mod 8) G/8 +
-    // b / 8 (G = the grid rounded down to a multiple of 8; the rest as they
-    // are): each XCD works through a contiguous run of leaves, whose near
-    // fields share most of their source leaves, instead of every 8th group.
-    const uint32_t g8 = gridDim.x & ~7u;
-    const uint32_t blk = blockIdx.x < g8 ? (blockIdx.x & 7u) * (g8 >> 3) + (blockIdx.x >> 3) : blockIdx.x;
-    for (uint32_t w = blk * P2P_WAVES + wslot; w < num_leaves; w += nw) {
+    for (uint32_t w = blockIdx.x * P2P_WAVES + wslot; w < num_leaves; w += nw) {
         const int A = (int)F.leaf_ids[w];
         const int4 ga = F.nodes.geo[A];
         const int4 la = F.nodes.link[A];
